@@ -1,0 +1,1152 @@
+// gkm_kernels.hip -- MI355X (gfx950) kernels of the gapped k-mer engine and
+// the C-ABI that exposes them (declared in include/gkmhip.h).
+//
+// Device data (all caller-owned, device pointers):
+//   planes     [n][nw] uint2   bit-plane words of each sequence (.x = code bit
+//                              0, .y = code bit 1), word 0 a zero pad; base t
+//                              in bit t & 31 of word (t >> 5) + 1;
+//                              nw >= Lmax / 32 + 4
+//   planes_rc  [n][nw] uint2   the same for each sequence's reverse complement
+//                              (base t = 3 - code[len - 1 - t]), so an N, read
+//                              as A on the forward strand, is T on the reverse
+//   lengths    [n] int32
+//   wtab       [Wmax/2 + 1] u8 window weight by distance from the centre
+//                              (weighted kernels -t 4/-t 5), else NULL
+//   hist       [B][S][nm] u64  per-pair mismatch histograms N[m], m < nm
+//
+// Kernels:
+//   k_pack_planes    codes -> planes, planes_rc
+//   k_pack_windows   codes -> 2-bit packed l-mer windows (mutant self-kernels)
+//   k_cross_hist     N[m] of query x against SV y (+ RC) -- the hot loop.  One
+//                    wave = one query x 64 SVs (lanes); the query's words are
+//                    wave-uniform, the SVs' words come from an LDS tile; the
+//                    wave walks diagonals, 32 window pairs per lane-step.
+//   k_self_hist      N[m] of x against (x, rc x): lanes = diagonals.
+//   k_combine_*      FP64: sum_m h[m] N[m], normalisation, RBF, alpha sums.
+//   k_mutant_diag    K_raw(mutant, mutant) for every single-base substitution.
+//   k_sweep          GkmExplain and ISM: lanes = diagonals of one (x, SV)
+//                    pair, per-position FP64 accumulation in LDS.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+
+#include "gkm_core.h"
+#include "gkmhip.h"
+
+#define GKM_API extern "C" __attribute__((visibility("default")))
+
+using namespace gkm;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[1024];
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define GKM_CHECK_LAUNCH(what)                                                  \
+    do {                                                                        \
+        hipError_t e_ = hipGetLastError();                                      \
+        if (e_ != hipSuccess)                                                   \
+            return fail(-3, "%s: launch failed: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+GKM_API const char *gkm_last_error(void) { return g_err; }
+
+GKM_API int gkm_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// Packing
+// ---------------------------------------------------------------------------
+
+// One thread per (sequence, word): 32 codes -> two plane words, forward and
+// reverse complement.  Words past the sequence are zero (read as A, and every
+// window touching them is masked out by length).
+__global__ void k_pack_planes(const uint8_t *__restrict__ codes,
+                              const int32_t *__restrict__ lengths, int n, int lmax,
+                              int nw, P2 *__restrict__ planes,
+                              P2 *__restrict__ planes_rc)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * nw)
+        return;
+    int r = (int)(idx / nw), a = (int)(idx % nw);
+    int len = lengths[r];
+    const uint8_t *row = codes + (int64_t)r * lmax;
+    uint32_t f0 = 0, f1 = 0, r0 = 0, r1 = 0;
+    int t0 = (a - 1) * 32;
+    if (a >= 1) {
+        for (int i = 0; i < 32; i++) {
+            int t = t0 + i;
+            if (t < len) {
+                uint32_t c = row[t] & 3u;
+                f0 |= (c & 1u) << i;
+                f1 |= (c >> 1) << i;
+                uint32_t rc = 3u - (row[len - 1 - t] & 3u);
+                r0 |= (rc & 1u) << i;
+                r1 |= (rc >> 1) << i;
+            }
+        }
+    }
+    planes[idx] = P2{f0, f1};
+    if (planes_rc)
+        planes_rc[idx] = P2{r0, r1};
+}
+
+// gkmsvm-lite's packed windows (kernels/packed.py:130-169): window i packs
+// bases i..i+l-1 at bits 2k; PAD past the last window.  The RC row here holds
+// the windows of the reverse complement from its first base (window j of
+// rc(x), j < len - l + 1), PAD after them.
+__global__ void k_pack_windows(const uint8_t *__restrict__ codes,
+                               const int32_t *__restrict__ lengths, int n, int lmax,
+                               int l, int W, uint32_t *__restrict__ fwd,
+                               uint32_t *__restrict__ rcw)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * W)
+        return;
+    int r = (int)(idx / W), i = (int)(idx % W);
+    int len = lengths[r];
+    const uint8_t *row = codes + (int64_t)r * lmax;
+    uint32_t f = 0xFFFFFFFFu, v = 0xFFFFFFFFu;
+    if (i < len - l + 1) {
+        f = 0;
+        v = 0;
+        for (int k = 0; k < l; k++) {
+            f |= (uint32_t)(row[i + k] & 3u) << (2 * k);
+            v |= (3u - (uint32_t)(row[len - 1 - (i + k)] & 3u)) << (2 * k);
+        }
+    }
+    fwd[idx] = f;
+    if (rcw)
+        rcw[idx] = v;
+}
+
+// ---------------------------------------------------------------------------
+// k_cross_hist
+// ---------------------------------------------------------------------------
+
+constexpr int CROSS_THREADS = 256;  // 4 waves
+constexpr int WAVE = 64;
+
+struct CrossArgs {
+    const P2 *xplanes;       // [B][nwx]
+    const int32_t *xlen;     // [B]
+    const P2 *yplanes;       // [S][nwy]
+    const P2 *yplanes_rc;    // [S][nwy] or NULL (no RC)
+    const int32_t *ylen;     // [S]
+    const int32_t *ygrp;     // [n_groups][2]: min, max length of each 64-SV group
+    const uint8_t *wtab;     // weight by distance, or NULL
+    int wtab_len;
+    unsigned long long *hist;  // [B][S][nm] (row b = x row b, column s)
+    int B, S, nwx, nwy, l, nm;
+    int rows_per_block;      // x rows handled by one block
+    int nsplit;              // diagonal chunks per (row, orientation)
+    int64_t col_min;         // >= 0: column s counts only when s >= row + col_min
+    int64_t row_base;        // global index of x row 0 (for col_min)
+};
+
+template <int LT, int NMC, bool WEIGHTED>
+__global__ __launch_bounds__(CROSS_THREADS) void k_cross_hist(CrossArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int g = blockIdx.x;  // SV group
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int norient = a.yplanes_rc ? 2 : 1;
+    const int nwy = a.nwy;
+    // LDS: SV tile [orient][word][lane] plane words, then the weight table.
+    P2 *tile = (P2 *)smem;
+    uint8_t *wt = smem + (size_t)norient * nwy * WAVE * sizeof(P2);
+
+    const int s0 = g * WAVE;
+    for (int idx = threadIdx.x; idx < norient * nwy * WAVE; idx += CROSS_THREADS) {
+        int o = idx / (nwy * WAVE);
+        int rem = idx % (nwy * WAVE);
+        int sl = rem / nwy, w = rem % nwy;  // coalesced over (sv, word)
+        int s = s0 + sl;
+        P2 v = {0u, 0u};
+        if (s < a.S)
+            v = (o ? a.yplanes_rc : a.yplanes)[(int64_t)s * nwy + w];
+        tile[((size_t)o * nwy + w) * WAVE + sl] = v;
+    }
+    if (WEIGHTED)
+        for (int i = threadIdx.x; i < a.wtab_len; i += CROSS_THREADS)
+            wt[i] = a.wtab[i];
+    __syncthreads();
+
+    const int l = LT > 0 ? LT : a.l;
+    const int s = s0 + lane;
+    const int ly = s < a.S ? a.ylen[s] : 0;
+    const int ly_min = a.ygrp[2 * g], ly_max = a.ygrp[2 * g + 1];
+    if (ly_max < l)
+        return;
+    const int row0 = blockIdx.y * a.rows_per_block;
+    const int nrows = min(a.rows_per_block, a.B - row0);
+    const int units = nrows * norient * a.nsplit;
+
+    for (int unit = wave; unit < units; unit += CROSS_THREADS / WAVE) {
+        const int split = unit % a.nsplit;
+        const int o = (unit / a.nsplit) % norient;
+        const int b = row0 + unit / (a.nsplit * norient);
+        const int lx = a.xlen[b];
+        if (lx < l)
+            continue;
+        bool lane_on = s < a.S && ly >= l;
+        if (a.col_min >= 0) {
+            int64_t lo = a.row_base + b + a.col_min;
+            if ((int64_t)s0 + WAVE - 1 < lo)
+                continue;  // the whole group lies below the diagonal
+            lane_on = lane_on && (int64_t)s >= lo;
+        }
+        const P2 *xp = a.xplanes + (int64_t)b * a.nwx;
+        const P2 *yt = tile + (size_t)o * nwy * WAVE + lane;
+        // Diagonals delta = ty - tx in [-(lx - l), ly_max - l], split evenly.
+        const int dlo = -(lx - l), dhi = ly_max - l;
+        const int nd = dhi - dlo + 1;
+        const int per = (nd + a.nsplit - 1) / a.nsplit;
+        const int d0 = dlo + split * per, d1 = min(dhi + 1, d0 + per);
+
+        uint32_t cnt[NMC];
+        unsigned long long wcnt[NMC];
+#pragma unroll
+        for (int m = 0; m < NMC; m++) {
+            cnt[m] = 0;
+            wcnt[m] = 0;
+        }
+        cross_lane<LT, NMC, WEIGHTED>([&](int k) { return xp[k]; },
+                                      [&](int u) { return yt[(size_t)u * WAVE]; }, lx, ly, lane_on,
+                                      ly_min, ly_max, l, a.nm, o, d0, d1, wt, cnt, wcnt);
+        if (lane_on) {
+            unsigned long long *h = a.hist + ((int64_t)b * a.S + s) * a.nm;
+#pragma unroll
+            for (int m = 0; m < NMC; m++) {
+                if (m < a.nm) {
+                    unsigned long long v = WEIGHTED ? wcnt[m] : (unsigned long long)cnt[m];
+                    if (v)
+                        atomicAdd(h + m, v);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_self_hist: K_raw(x, x) histograms, lanes = diagonals
+// ---------------------------------------------------------------------------
+
+constexpr int SELF_THREADS = 256;
+
+struct SelfArgs {
+    const P2 *planes, *planes_rc;  // [n][nw]
+    const int32_t *len;
+    const uint8_t *wtab;
+    int wtab_len;
+    unsigned long long *hist;  // [n][nm]
+    int n, nw, l, nm, nchunks;
+};
+
+template <int LT, int NMC, bool WEIGHTED>
+__global__ __launch_bounds__(SELF_THREADS) void k_self_hist(SelfArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int r = blockIdx.x;
+    const int nw = a.nw;
+    P2 *xs = (P2 *)smem;
+    P2 *rs = xs + nw;
+    unsigned long long *red = (unsigned long long *)(rs + nw);  // [16][SELF_THREADS/64]
+    uint8_t *wt = (uint8_t *)(red + 16 * (SELF_THREADS / WAVE));
+    const int norient = a.planes_rc ? 2 : 1;
+    for (int i = threadIdx.x; i < nw; i += SELF_THREADS) {
+        xs[i] = a.planes[(int64_t)r * nw + i];
+        rs[i] = norient > 1 ? a.planes_rc[(int64_t)r * nw + i] : P2{0u, 0u};
+    }
+    if (WEIGHTED)
+        for (int i = threadIdx.x; i < a.wtab_len; i += SELF_THREADS)
+            wt[i] = a.wtab[i];
+    __syncthreads();
+    const int l = LT > 0 ? LT : a.l;
+    const int lx = a.len[r];
+    uint32_t cnt[NMC];
+    unsigned long long wcnt[NMC];
+#pragma unroll
+    for (int m = 0; m < NMC; m++) {
+        cnt[m] = 0;
+        wcnt[m] = 0;
+    }
+    if (lx >= l) {
+        const int nd = 2 * (lx - l) + 1;
+        const int total = nd * norient;
+        for (int d = blockIdx.y * SELF_THREADS + threadIdx.x; d < total;
+             d += SELF_THREADS * a.nchunks) {
+            const int o = d / nd;
+            const int delta = d % nd - (lx - l);
+            const P2 *ys = o ? rs : xs;
+            self_lane<LT, NMC, WEIGHTED>([&](int k) { return xs[k]; }, [&](int u) { return ys[u]; },
+                                         lx, l, a.nm, delta, o, wt, cnt, wcnt);
+        }
+    }
+    // Block reduction (integers: exact in any order).
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int m = 0; m < NMC; m++) {
+        unsigned long long v = WEIGHTED ? wcnt[m] : (unsigned long long)cnt[m];
+        for (int off = WAVE / 2; off > 0; off >>= 1)
+            v += __shfl_down(v, off, WAVE);
+        if (lane == 0)
+            red[m * (SELF_THREADS / WAVE) + wave] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NMC && (int)threadIdx.x < a.nm) {
+        unsigned long long v = 0;
+        for (int w = 0; w < SELF_THREADS / WAVE; w++)
+            v += red[threadIdx.x * (SELF_THREADS / WAVE) + w];
+        if (v)
+            atomicAdd(a.hist + (int64_t)r * a.nm + threadIdx.x, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 combine kernels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double kraw_of(const unsigned long long *h, const double *tab, int nm)
+{
+    double k = 0.0;
+    for (int m = 0; m < nm; m++)
+        k += tab[m] * (double)h[m];
+    return k;
+}
+
+// K_raw [n] = sum_m h[m] N[m].
+__global__ void k_combine_kraw(const unsigned long long *__restrict__ hist, int64_t n,
+                               int nm, const double *__restrict__ tab, double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = kraw_of(hist + i * nm, tab, nm);
+}
+
+// Normalised, finished kernel values [B][S]: F(K / max(sqrt(dx ds), 1e-10)),
+// F = exp(gamma (k - 1)) when rbf, else the identity (gram.py:94-97).
+__global__ void k_combine_gram(const unsigned long long *__restrict__ hist, int B, int S,
+                               int nm, const double *__restrict__ tab,
+                               const double *__restrict__ dx, const double *__restrict__ ds,
+                               int normalize, int rbf, double gamma,
+                               double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * S)
+        return;
+    int b = (int)(i / S), s = (int)(i % S);
+    double k = kraw_of(hist + i * nm, tab, nm);
+    if (normalize)
+        k = k / fmax(sqrt(dx[b] * ds[s]), 1e-10);
+    if (rbf)
+        k = exp(gamma * (k - 1.0));
+    out[i] = k;
+}
+
+constexpr int RED_THREADS = 256;
+
+// Block-wide sum in a fixed order (deterministic).
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+        v += __shfl_down(v, off, WAVE);
+    if (lane == 0)
+        red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x / WAVE); w++)
+            t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// Decision values: out[b] = sum_s coef_s F(K[b,s] / max(sqrt(dx ds), 1e-10)) + bias
+// (svm.py:43-61, _normalized_dot), one block per row.
+__global__ __launch_bounds__(RED_THREADS) void k_combine_score(
+    const unsigned long long *__restrict__ hist, int S, int nm, const double *__restrict__ tab,
+    const double *__restrict__ dx, const double *__restrict__ ds,
+    const double *__restrict__ coef, int normalize, int rbf, double gamma, double bias,
+    double *__restrict__ out)
+{
+    __shared__ double red[RED_THREADS / WAVE];
+    __shared__ double stab[16];
+    if (threadIdx.x < 16)
+        stab[threadIdx.x] = (int)threadIdx.x < nm ? tab[threadIdx.x] : 0.0;
+    __syncthreads();
+    const int b = blockIdx.x;
+    double acc = 0.0;
+    const double dxb = dx ? dx[b] : 1.0;
+    for (int s = threadIdx.x; s < S; s += RED_THREADS) {
+        double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
+        if (normalize)
+            k = k / fmax(sqrt(dxb * ds[s]), 1e-10);
+        if (rbf)
+            k = exp(gamma * (k - 1.0));
+        acc += coef[s] * k;
+    }
+    double t = block_sum(acc, red);
+    if (threadIdx.x == 0)
+        out[b] = t + bias;
+}
+
+// Mutation impact (explain/impact.py:22-48): rows [ref; alt] of hist (2n rows),
+// impact[b] = sum_s coef_s dk (RBF: dk (exp(g(k-1)) - exp(-g)) / k for k > 0).
+__global__ __launch_bounds__(RED_THREADS) void k_combine_impact(
+    const unsigned long long *__restrict__ hist, int n, int S, int nm,
+    const double *__restrict__ tab, const double *__restrict__ dref,
+    const double *__restrict__ ds, const double *__restrict__ coef, int normalize, int rbf,
+    double gamma, double *__restrict__ out)
+{
+    __shared__ double red[RED_THREADS / WAVE];
+    __shared__ double stab[16];
+    if (threadIdx.x < 16)
+        stab[threadIdx.x] = (int)threadIdx.x < nm ? tab[threadIdx.x] : 0.0;
+    __syncthreads();
+    const int b = blockIdx.x;
+    const double floor_ = rbf ? exp(-gamma) : 0.0;
+    double acc = 0.0;
+    for (int s = threadIdx.x; s < S; s += RED_THREADS) {
+        double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
+        double dk = kraw_of(hist + ((int64_t)(n + b) * S + s) * nm, stab, nm) - k;
+        if (normalize) {
+            double z = fmax(sqrt(dref[b] * ds[s]), 1e-10);
+            k /= z;
+            dk /= z;
+        }
+        if (rbf) {
+            if (k <= 0.0)
+                continue;
+            dk *= (exp(gamma * (k - 1.0)) - floor_) / k;
+        }
+        acc += coef[s] * dk;
+    }
+    double t = block_sum(acc, red);
+    if (threadIdx.x == 0)
+        out[b] = t;
+}
+
+// GkmExplain effective coefficients (explain/core.py:203-216):
+// ec[b,s] = coef_s / max(sqrt(dx ds), 1e-10), times _rbf_scale(k) for RBF
+// (explain/common.py:27-32).
+__global__ void k_explain_coef(const unsigned long long *__restrict__ hist, int B, int S,
+                               int nm, const double *__restrict__ tab,
+                               const double *__restrict__ dx, const double *__restrict__ ds,
+                               const double *__restrict__ coef, int normalize, int rbf,
+                               double gamma, double *__restrict__ ec)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * S)
+        return;
+    int b = (int)(i / S), s = (int)(i % S);
+    double e = coef[s];
+    double z = normalize ? fmax(sqrt(dx[b] * ds[s]), 1e-10) : 1.0;
+    if (normalize)
+        e = e / z;
+    if (rbf) {
+        double k = kraw_of(hist + i * nm, tab, nm) / z;
+        e = e * (k > 0.0 ? (exp(gamma * (k - 1.0)) - exp(-gamma)) / k : 0.0);
+    }
+    ec[i] = e;
+}
+
+// ---------------------------------------------------------------------------
+// k_mutant_diag: K_raw(mutant, mutant) [4][Lmax][n]
+// ---------------------------------------------------------------------------
+
+constexpr int MUT_THREADS = 128;
+
+__device__ __forceinline__ int mism2(uint32_t a, uint32_t b)
+{
+    uint32_t x = a ^ b;
+    return __builtin_popcount(((x >> 1) | x) & 0x55555555u);
+}
+
+// One block per (row, position).  The substitution at p changes the forward
+// windows i in C = [p-l+1, p] and the RC windows j in C' = [q-l+1, q],
+// q = len-1-p (base 3 - v there).  K_raw(x', x') - K_raw(x, x) is
+//   forward block: 2 sum_{i in C, j not in C} (g(x'_i, x_j) - g(x_i, x_j))
+//                  + sum_{i, j in C} (g(x'_i, x'_j) - g(x_i, x_j))
+//   RC block:      sum_{i in C, all j} (g(x'_i, r'_j) - g(x_i, r_j))
+//                  + sum_{i not in C, j in C'} (g(x_i, r'_j) - g(x_i, r_j))
+// with g(u, v) = w_u w_v h[m(u, v)] -- the pairs kernels/direct.py:141-201
+// (_block_delta, _mutant_diag_numba) and kernels/weighted.py:122-197 visit.
+__global__ __launch_bounds__(MUT_THREADS) void k_mutant_diag(
+    const uint32_t *__restrict__ fwd, const uint32_t *__restrict__ rcw,
+    const int32_t *__restrict__ len, const uint8_t *__restrict__ codes, int lmax, int n,
+    int W, int l, const double *__restrict__ tab, const uint8_t *__restrict__ wtab,
+    const double *__restrict__ diag, double *__restrict__ out)
+{
+    __shared__ double stab[16];
+    __shared__ double red[4][MUT_THREADS / WAVE];
+    const int r = blockIdx.y, p = blockIdx.x;
+    if (threadIdx.x < 16)
+        stab[threadIdx.x] = (int)threadIdx.x <= l ? tab[threadIdx.x] : 0.0;
+    __syncthreads();
+    const int L = len[r];
+    const int w = L - l + 1;
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint8_t xb = p < L ? (codes[(int64_t)r * lmax + p] & 3u) : 0u;
+    if (p < L && w > 0) {
+        const uint32_t *f = fwd + (int64_t)r * W;
+        const uint32_t *rv = rcw ? rcw + (int64_t)r * W : nullptr;
+        const int c0 = max(0, p - l + 1), c1 = min(p, w - 1);
+        const int q = L - 1 - p;
+        const int r0 = max(0, q - l + 1), r1 = min(q, w - 1);
+        auto wgt = [&](int nwin, int j) -> double {
+            return wtab ? (double)window_weight(wtab, nwin, j) : 1.0;
+        };
+        for (int j = threadIdx.x; j < w; j += MUT_THREADS) {
+            const uint32_t xj = f[j];
+            const double wj = wgt(w, j);
+            const bool jin = j >= c0 && j <= c1;
+            const uint32_t rj = rv ? rv[j] : 0u;
+            const double wrj = wgt(w, w - 1 - j);
+            const bool jrin = rv && j >= r0 && j <= r1;
+            for (int i = c0; i <= c1; i++) {
+                const uint32_t xi = f[i];
+                const double wi = wgt(w, i);
+                const int shi = 2 * (p - i);
+                const double g_old = wi * wj * stab[mism2(xi, xj)];
+                const double gr_old = rv ? wi * wrj * stab[mism2(xi, rj)] : 0.0;
+                for (int v = 0; v < 4; v++) {
+                    if (v == xb)
+                        continue;
+                    const uint32_t xi2 = (xi & ~(3u << shi)) | ((uint32_t)v << shi);
+                    if (!jin) {
+                        d[v] += 2.0 * (wi * wj * stab[mism2(xi2, xj)] - g_old);
+                    } else {
+                        const int shj = 2 * (p - j);
+                        const uint32_t xj2 = (xj & ~(3u << shj)) | ((uint32_t)v << shj);
+                        d[v] += wi * wj * stab[mism2(xi2, xj2)] - g_old;
+                    }
+                    if (rv) {
+                        uint32_t rj2 = rj;
+                        if (jrin) {
+                            const int shr = 2 * (q - j);
+                            rj2 = (rj & ~(3u << shr)) | ((uint32_t)(3 - v) << shr);
+                        }
+                        d[v] += wi * wrj * stab[mism2(xi2, rj2)] - gr_old;
+                    }
+                }
+            }
+            if (rv && !jin) {
+                // row j (not in C) against the changed RC windows
+                for (int jr = r0; jr <= r1; jr++) {
+                    const uint32_t rr = rv[jr];
+                    const double wr = wgt(w, w - 1 - jr);
+                    const int shr = 2 * (q - jr);
+                    const double g_old = wj * wr * stab[mism2(xj, rr)];
+                    for (int v = 0; v < 4; v++) {
+                        if (v == xb)
+                            continue;
+                        const uint32_t rr2 = (rr & ~(3u << shr)) | ((uint32_t)(3 - v) << shr);
+                        d[v] += wj * wr * stab[mism2(xj, rr2)] - g_old;
+                    }
+                }
+            }
+        }
+    }
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    for (int v = 0; v < 4; v++) {
+        double t = d[v];
+        for (int off = WAVE / 2; off > 0; off >>= 1)
+            t += __shfl_down(t, off, WAVE);
+        if (lane == 0)
+            red[v][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int k = 0; k < MUT_THREADS / WAVE; k++)
+            t += red[threadIdx.x][k];
+        out[((int64_t)threadIdx.x * lmax + p) * n + r] = diag[r] + t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep: GkmExplain (hypothetical / perturbation) and ISM deltas
+// ---------------------------------------------------------------------------
+
+constexpr int SWEEP_THREADS = 256;
+
+struct SweepArgs {
+    const P2 *xplanes;  // [B][nwx]
+    const int32_t *xlen;
+    const uint8_t *xcodes;  // [B][lmax] (ISM: the reference base at each position)
+    int lmax;
+    const P2 *yplanes, *yplanes_rc;  // [S][nwy]
+    const int32_t *ylen;
+    const uint8_t *wtab;
+    int wtab_len;
+    int B, S, nwx, nwy, l, nm, cmax;  // hits are windows with count <= cmax
+    int sv_per_block;
+    const double *ta, *tb;  // explain tables [l + 2]
+    const double *ec;       // explain: [B][S] effective coefficients
+    const double *tab;      // ISM: h[m] [l + 2] (zero past dmax)
+    const double *coef, *ds, *dx;  // ISM: coefficients, K_raw(sv, sv), K_raw(x, x)
+    const double *dmut;     // ISM: K_raw(mutant, mutant) [4][lmax][B]
+    int normalize, rbf;
+    double gamma;
+    double *out;  // [B][4][lmax], accumulated with FP64 atomics
+};
+
+template <int LT, int NMC, int MODE>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int b = blockIdx.x;
+    const int lx = a.xlen[b];
+    const int l = LT > 0 ? LT : a.l;
+    const int norient = a.yplanes_rc ? 2 : 1;
+    const int nwx = a.nwx, nwy = a.nwy;
+    const int lmax = a.lmax;
+    // LDS layout
+    double *res = (double *)smem;                   // [4][lmax] per-query sums
+    double *dm = res + 4 * (size_t)lmax;            // ISM: [lmax] match deltas
+    double *da = dm + (MODE == SW_ISM ? lmax : 0);  // ISM: [4][lmax] mismatch deltas
+    double *stab = da + (MODE == SW_ISM ? 4 * (size_t)lmax : 0);  // [2][16]
+    unsigned long long *hred = (unsigned long long *)(stab + 32);  // [16][4]
+    P2 *xs = (P2 *)(hred + 64);
+    P2 *ys = xs + nwx;  // [norient][nwy]
+    uint8_t *wt = (uint8_t *)(ys + 2 * nwy);
+
+    for (int i = threadIdx.x; i < 4 * lmax; i += SWEEP_THREADS)
+        res[i] = 0.0;
+    if (MODE == SW_ISM)
+        for (int i = threadIdx.x; i < 5 * lmax; i += SWEEP_THREADS)
+            dm[i] = 0.0;
+    if (threadIdx.x < 16) {
+        int t = threadIdx.x;
+        stab[t] = t < a.l + 2 ? (MODE == SW_ISM ? a.tab[t] : a.ta[t]) : 0.0;
+        stab[16 + t] = (MODE != SW_ISM && t < a.l + 2) ? a.tb[t] : 0.0;
+    }
+    for (int i = threadIdx.x; i < nwx; i += SWEEP_THREADS)
+        xs[i] = a.xplanes[(int64_t)b * nwx + i];
+    if (a.wtab)
+        for (int i = threadIdx.x; i < a.wtab_len; i += SWEEP_THREADS)
+            wt[i] = a.wtab[i];
+    __syncthreads();
+    if (lx < l)
+        return;
+    const int sv0 = blockIdx.y * a.sv_per_block;
+    const int sv1 = min(a.S, sv0 + a.sv_per_block);
+    const uint8_t *xc = MODE == SW_ISM ? a.xcodes + (int64_t)b * lmax : nullptr;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const uint8_t *wtp = a.wtab ? wt : nullptr;
+
+    auto add = [&](int kind, int yb, int pos, double v) {
+        if (kind == 0)
+            atomicAdd(&res[yb * lmax + pos], v);
+        else if (kind == 1)
+            atomicAdd(&dm[pos], v);
+        else
+            atomicAdd(&da[yb * lmax + pos], v);
+    };
+
+    for (int s = sv0; s < sv1; s++) {
+        const int ly = a.ylen[s];
+        if (ly < l)
+            continue;
+        for (int i = threadIdx.x; i < norient * nwy; i += SWEEP_THREADS) {
+            int o = i / nwy, k = i % nwy;
+            ys[o * nwy + k] = (o ? a.yplanes_rc : a.yplanes)[(int64_t)s * nwy + k];
+        }
+        __syncthreads();
+        const double ecs = MODE == SW_ISM ? 1.0 : a.ec[(int64_t)b * a.S + s];
+        unsigned long long cnt[NMC];
+#pragma unroll
+        for (int m = 0; m < NMC; m++)
+            cnt[m] = 0;
+        const int nd = lx + ly - 2 * l + 1;
+        for (int d = threadIdx.x; d < nd * norient; d += SWEEP_THREADS) {
+            const int o = d / nd;
+            const int delta = d % nd - (lx - l);
+            const P2 *yo = ys + o * nwy;
+            sweep_lane<LT, NMC, MODE>([&](int k) { return xs[k]; }, [&](int u) { return yo[u]; }, lx,
+                                      ly, l, a.nm, a.cmax, delta, o, wtp, ecs, stab, stab + 16, cnt,
+                                      add);
+        }
+        if (MODE == SW_ISM) {
+            // K_raw(x, s) from the block's histogram, then the F-sweep.
+#pragma unroll
+            for (int m = 0; m < NMC; m++) {
+                unsigned long long v = cnt[m];
+                for (int off = WAVE / 2; off > 0; off >>= 1)
+                    v += __shfl_down(v, off, WAVE);
+                if (lane == 0)
+                    hred[m * 4 + wave] = v;
+            }
+            __syncthreads();
+            double kraw = 0.0;
+            for (int m = 0; m < a.nm; m++) {
+                unsigned long long v = 0;
+                for (int w = 0; w < SWEEP_THREADS / WAVE; w++)
+                    v += hred[m * 4 + w];
+                kraw += stab[m] * (double)v;
+            }
+            const double cs = a.coef[s];
+            const double dss = a.ds[s];
+            const double zr = a.normalize ? fmax(sqrt(a.dx[b] * dss), 1e-10) : 1.0;
+            double fref = kraw / zr;
+            if (a.rbf)
+                fref = exp(a.gamma * (fref - 1.0));
+            for (int p = threadIdx.x; p < lx; p += SWEEP_THREADS) {
+                const int xb_ = xc[p] & 3;
+                const double dmp = dm[p];
+                for (int v = 0; v < 4; v++) {
+                    if (v == xb_)
+                        continue;
+                    const double dk = dmp + da[v * lmax + p];
+                    double k = kraw + dk;
+                    if (a.normalize)
+                        k = k / fmax(sqrt(a.dmut[((int64_t)v * lmax + p) * a.B + b] * dss), 1e-10);
+                    if (a.rbf)
+                        k = exp(a.gamma * (k - 1.0));
+                    res[v * lmax + p] += cs * (k - fref);
+                    da[v * lmax + p] = 0.0;
+                }
+                dm[p] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    double *ob = a.out + (int64_t)b * 4 * lmax;
+    for (int i = threadIdx.x; i < 4 * lx; i += SWEEP_THREADS) {
+        int v = i / lx, p = i % lx;
+        double r = res[v * lmax + p];
+        if (r != 0.0)
+            atomicAdd(&ob[v * lmax + p], r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch helpers
+// ---------------------------------------------------------------------------
+
+#define GKM_L_CASES(X) X(10) X(11) X(12)
+
+template <template <int, int, bool> class K>
+struct Unused;
+
+typedef void (*cross_fn)(CrossArgs);
+typedef void (*self_fn)(SelfArgs);
+typedef void (*sweep_fn)(SweepArgs);
+
+template <int LT, int NMC>
+static cross_fn pick_cross_w(bool weighted)
+{
+    return weighted ? (cross_fn)k_cross_hist<LT, NMC, true> : (cross_fn)k_cross_hist<LT, NMC, false>;
+}
+template <int LT>
+static cross_fn pick_cross_n(int nm, bool weighted)
+{
+    return nm <= 4 ? pick_cross_w<LT, 4>(weighted) : pick_cross_w<LT, 16>(weighted);
+}
+static cross_fn pick_cross(int l, int nm, bool weighted)
+{
+    switch (l) {
+    case 10: return pick_cross_n<10>(nm, weighted);
+    case 11: return pick_cross_n<11>(nm, weighted);
+    case 12: return pick_cross_n<12>(nm, weighted);
+    default: return pick_cross_n<0>(nm, weighted);
+    }
+}
+
+template <int LT, int NMC>
+static self_fn pick_self_w(bool weighted)
+{
+    return weighted ? (self_fn)k_self_hist<LT, NMC, true> : (self_fn)k_self_hist<LT, NMC, false>;
+}
+template <int LT>
+static self_fn pick_self_n(int nm, bool weighted)
+{
+    return nm <= 4 ? pick_self_w<LT, 4>(weighted) : pick_self_w<LT, 16>(weighted);
+}
+static self_fn pick_self(int l, int nm, bool weighted)
+{
+    switch (l) {
+    case 10: return pick_self_n<10>(nm, weighted);
+    case 11: return pick_self_n<11>(nm, weighted);
+    case 12: return pick_self_n<12>(nm, weighted);
+    default: return pick_self_n<0>(nm, weighted);
+    }
+}
+
+template <int LT, int NMC>
+static sweep_fn pick_sweep_m(int mode)
+{
+    switch (mode) {
+    case SW_HYP: return (sweep_fn)k_sweep<LT, NMC, SW_HYP>;
+    case SW_PERT: return (sweep_fn)k_sweep<LT, NMC, SW_PERT>;
+    default: return (sweep_fn)k_sweep<LT, NMC, SW_ISM>;
+    }
+}
+template <int LT>
+static sweep_fn pick_sweep_n(int nm, int mode)
+{
+    return nm <= 4 ? pick_sweep_m<LT, 4>(mode) : pick_sweep_m<LT, 16>(mode);
+}
+static sweep_fn pick_sweep(int l, int nm, int mode)
+{
+    switch (l) {
+    case 10: return pick_sweep_n<10>(nm, mode);
+    case 11: return pick_sweep_n<11>(nm, mode);
+    case 12: return pick_sweep_n<12>(nm, mode);
+    default: return pick_sweep_n<0>(nm, mode);
+    }
+}
+
+static inline hipStream_t S_(void *stream) { return (hipStream_t)stream; }
+
+static int check_l_nm(int l, int nm)
+{
+    if (l < 1 || l > 15)
+        return fail(-1, "l must be in 1..15, got %d", l);
+    if (nm < 1 || nm > 16 || nm > l + 1)
+        return fail(-1, "histogram levels must be in 1..min(16, l + 1), got %d", nm);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+GKM_API int gkm_planes_words(int lmax) { return lmax / 32 + 4; }
+
+GKM_API int gkm_pack_planes(const uint8_t *codes, const int32_t *lengths, int n, int lmax,
+                            int nw, void *planes, void *planes_rc, void *stream)
+{
+    if (n < 0 || lmax < 1 || nw < lmax / 32 + 4)
+        return fail(-1, "gkm_pack_planes: need n >= 0, lmax >= 1, nw >= lmax/32 + 4 (n=%d lmax=%d nw=%d)",
+                    n, lmax, nw);
+    if (n == 0)
+        return 0;
+    int64_t total = (int64_t)n * nw;
+    int threads = 256;
+    int64_t blocks = (total + threads - 1) / threads;
+    hipLaunchKernelGGL(k_pack_planes, dim3((unsigned)blocks), dim3(threads), 0, S_(stream), codes,
+                       lengths, n, lmax, nw, (P2 *)planes, (P2 *)planes_rc);
+    GKM_CHECK_LAUNCH("gkm_pack_planes");
+    return 0;
+}
+
+GKM_API int gkm_pack_windows(const uint8_t *codes, const int32_t *lengths, int n, int lmax,
+                             int l, uint32_t *fwd, uint32_t *rcw, void *stream)
+{
+    if (l < 1 || l > 15 || lmax < l)
+        return fail(-1, "gkm_pack_windows: need 1 <= l <= 15 and lmax >= l (l=%d lmax=%d)", l, lmax);
+    if (n == 0)
+        return 0;
+    int W = lmax - l + 1;
+    int64_t total = (int64_t)n * W;
+    int threads = 256;
+    int64_t blocks = (total + threads - 1) / threads;
+    hipLaunchKernelGGL(k_pack_windows, dim3((unsigned)blocks), dim3(threads), 0, S_(stream), codes,
+                       lengths, n, lmax, l, W, fwd, rcw);
+    GKM_CHECK_LAUNCH("gkm_pack_windows");
+    return 0;
+}
+
+// Fill a u64 buffer with zeros with a kernel (graph-capture friendly).
+__global__ void k_zero64(unsigned long long *p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0ull;
+}
+
+GKM_API int gkm_zero_u64(void *p, int64_t n, void *stream)
+{
+    if (n <= 0)
+        return 0;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero64, dim3((unsigned)blocks), dim3(256), 0, S_(stream),
+                       (unsigned long long *)p, n);
+    GKM_CHECK_LAUNCH("gkm_zero_u64");
+    return 0;
+}
+
+GKM_API int gkm_cross_hist(const void *xplanes, const int32_t *xlen, int B, int nwx,
+                           const void *yplanes, const void *yplanes_rc, const int32_t *ylen,
+                           const int32_t *ygroups, int S, int nwy, int l, int nm,
+                           const uint8_t *wtab, int wtab_len, int rows_per_block, int nsplit,
+                           int64_t col_min, int64_t row_base, void *hist, void *stream)
+{
+    int rc = check_l_nm(l, nm);
+    if (rc)
+        return rc;
+    if (B < 0 || S < 0 || rows_per_block < 1 || nsplit < 1)
+        return fail(-1, "gkm_cross_hist: bad sizes (B=%d S=%d rows_per_block=%d nsplit=%d)", B, S,
+                    rows_per_block, nsplit);
+    if (B == 0 || S == 0)
+        return 0;
+    CrossArgs a;
+    a.xplanes = (const P2 *)xplanes;
+    a.xlen = xlen;
+    a.yplanes = (const P2 *)yplanes;
+    a.yplanes_rc = (const P2 *)yplanes_rc;
+    a.ylen = ylen;
+    a.ygrp = ygroups;
+    a.wtab = wtab;
+    a.wtab_len = wtab ? wtab_len : 0;
+    a.hist = (unsigned long long *)hist;
+    a.B = B;
+    a.S = S;
+    a.nwx = nwx;
+    a.nwy = nwy;
+    a.l = l;
+    a.nm = nm;
+    a.rows_per_block = rows_per_block;
+    a.nsplit = nsplit;
+    a.col_min = col_min;
+    a.row_base = row_base;
+    int norient = yplanes_rc ? 2 : 1;
+    size_t lds = (size_t)norient * nwy * WAVE * sizeof(P2) + (wtab ? (size_t)wtab_len : 0);
+    if (lds > 160 * 1024)
+        return fail(-2, "gkm_cross_hist: SV tile needs %zu bytes of LDS (SVs too long)", lds);
+    dim3 grid((unsigned)((S + WAVE - 1) / WAVE), (unsigned)((B + rows_per_block - 1) / rows_per_block));
+    cross_fn f = pick_cross(l, nm, wtab != nullptr);
+    if (lds > 64 * 1024)
+        if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
+    hipLaunchKernelGGL(f, grid, dim3(CROSS_THREADS), lds, S_(stream), a);
+    GKM_CHECK_LAUNCH("gkm_cross_hist");
+    return 0;
+}
+
+GKM_API int gkm_self_hist(const void *planes, const void *planes_rc, const int32_t *len, int n,
+                          int nw, int l, int nm, const uint8_t *wtab, int wtab_len, int nchunks,
+                          void *hist, void *stream)
+{
+    int rc = check_l_nm(l, nm);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return 0;
+    if (nchunks < 1)
+        nchunks = 1;
+    SelfArgs a;
+    a.planes = (const P2 *)planes;
+    a.planes_rc = (const P2 *)planes_rc;
+    a.len = len;
+    a.wtab = wtab;
+    a.wtab_len = wtab ? wtab_len : 0;
+    a.hist = (unsigned long long *)hist;
+    a.n = n;
+    a.nw = nw;
+    a.l = l;
+    a.nm = nm;
+    a.nchunks = nchunks;
+    size_t lds = 2 * (size_t)nw * sizeof(P2) + 16 * (SELF_THREADS / WAVE) * sizeof(unsigned long long) +
+                 (wtab ? (size_t)wtab_len : 0);
+    if (lds > 160 * 1024)
+        return fail(-2, "gkm_self_hist: needs %zu bytes of LDS", lds);
+    self_fn f = pick_self(l, nm, wtab != nullptr);
+    if (lds > 64 * 1024)
+        if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
+    hipLaunchKernelGGL(f, dim3((unsigned)n, (unsigned)nchunks), dim3(SELF_THREADS), lds, S_(stream), a);
+    GKM_CHECK_LAUNCH("gkm_self_hist");
+    return 0;
+}
+
+GKM_API int gkm_combine_kraw(const void *hist, int64_t n, int nm, const double *tab, double *out,
+                             void *stream)
+{
+    if (n <= 0)
+        return 0;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_combine_kraw, dim3((unsigned)blocks), dim3(256), 0, S_(stream),
+                       (const unsigned long long *)hist, n, nm, tab, out);
+    GKM_CHECK_LAUNCH("gkm_combine_kraw");
+    return 0;
+}
+
+GKM_API int gkm_combine_gram(const void *hist, int B, int S, int nm, const double *tab,
+                             const double *dx, const double *ds, int normalize, int rbf,
+                             double gamma, double *out, void *stream)
+{
+    int64_t n = (int64_t)B * S;
+    if (n <= 0)
+        return 0;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_combine_gram, dim3((unsigned)blocks), dim3(256), 0, S_(stream),
+                       (const unsigned long long *)hist, B, S, nm, tab, dx, ds, normalize, rbf,
+                       gamma, out);
+    GKM_CHECK_LAUNCH("gkm_combine_gram");
+    return 0;
+}
+
+GKM_API int gkm_combine_score(const void *hist, int B, int S, int nm, const double *tab,
+                              const double *dx, const double *ds, const double *coef,
+                              int normalize, int rbf, double gamma, double bias, double *out,
+                              void *stream)
+{
+    if (nm > 16)
+        return fail(-1, "gkm_combine_score: nm > 16");
+    if (B <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_combine_score, dim3((unsigned)B), dim3(RED_THREADS), 0, S_(stream),
+                       (const unsigned long long *)hist, S, nm, tab, dx, ds, coef, normalize, rbf,
+                       gamma, bias, out);
+    GKM_CHECK_LAUNCH("gkm_combine_score");
+    return 0;
+}
+
+GKM_API int gkm_combine_impact(const void *hist, int n, int S, int nm, const double *tab,
+                               const double *dref, const double *ds, const double *coef,
+                               int normalize, int rbf, double gamma, double *out, void *stream)
+{
+    if (nm > 16)
+        return fail(-1, "gkm_combine_impact: nm > 16");
+    if (n <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_combine_impact, dim3((unsigned)n), dim3(RED_THREADS), 0, S_(stream),
+                       (const unsigned long long *)hist, n, S, nm, tab, dref, ds, coef, normalize,
+                       rbf, gamma, out);
+    GKM_CHECK_LAUNCH("gkm_combine_impact");
+    return 0;
+}
+
+GKM_API int gkm_explain_coef(const void *hist, int B, int S, int nm, const double *tab,
+                             const double *dx, const double *ds, const double *coef, int normalize,
+                             int rbf, double gamma, double *ec, void *stream)
+{
+    int64_t n = (int64_t)B * S;
+    if (n <= 0)
+        return 0;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_explain_coef, dim3((unsigned)blocks), dim3(256), 0, S_(stream),
+                       (const unsigned long long *)hist, B, S, nm, tab, dx, ds, coef, normalize,
+                       rbf, gamma, ec);
+    GKM_CHECK_LAUNCH("gkm_explain_coef");
+    return 0;
+}
+
+GKM_API int gkm_mutant_diag(const uint32_t *fwd, const uint32_t *rcw, const int32_t *len,
+                            const uint8_t *codes, int lmax, int n, int l, const double *tab,
+                            const uint8_t *wtab, const double *diag, double *out, void *stream)
+{
+    if (l < 1 || l > 15 || lmax < l)
+        return fail(-1, "gkm_mutant_diag: need 1 <= l <= 15 and lmax >= l");
+    if (n == 0)
+        return 0;
+    if (n > 65535)
+        return fail(-1, "gkm_mutant_diag: at most 65535 rows per call, got %d", n);
+    int W = lmax - l + 1;
+    hipLaunchKernelGGL(k_mutant_diag, dim3((unsigned)lmax, (unsigned)n), dim3(MUT_THREADS), 0,
+                       S_(stream), fwd, rcw, len, codes, lmax, n, W, l, tab, wtab, diag, out);
+    GKM_CHECK_LAUNCH("gkm_mutant_diag");
+    return 0;
+}
+
+static int sweep_launch(int mode, const void *xplanes, const int32_t *xlen, const uint8_t *xcodes,
+                        int lmax, int B, int nwx, const void *yplanes, const void *yplanes_rc,
+                        const int32_t *ylen, int S, int nwy, int l, int nm, int cmax,
+                        const uint8_t *wtab, int wtab_len, int sv_per_block, const double *ta,
+                        const double *tb, const double *ec, const double *tab, const double *coef,
+                        const double *ds, const double *dx, const double *dmut, int normalize,
+                        int rbf, double gamma, double *out, void *stream)
+{
+    int rc = check_l_nm(l, nm);
+    if (rc)
+        return rc;
+    if (cmax >= nm)
+        return fail(-1, "sweep: cmax (%d) must be < nm (%d)", cmax, nm);
+    if (B == 0 || S == 0)
+        return 0;
+    if (sv_per_block < 1)
+        sv_per_block = 1;
+    SweepArgs a;
+    a.xplanes = (const P2 *)xplanes;
+    a.xlen = xlen;
+    a.xcodes = xcodes;
+    a.lmax = lmax;
+    a.yplanes = (const P2 *)yplanes;
+    a.yplanes_rc = (const P2 *)yplanes_rc;
+    a.ylen = ylen;
+    a.wtab = wtab;
+    a.wtab_len = wtab ? wtab_len : 0;
+    a.B = B;
+    a.S = S;
+    a.nwx = nwx;
+    a.nwy = nwy;
+    a.l = l;
+    a.nm = nm;
+    a.cmax = cmax;
+    a.sv_per_block = sv_per_block;
+    a.ta = ta;
+    a.tb = tb;
+    a.ec = ec;
+    a.tab = tab;
+    a.coef = coef;
+    a.ds = ds;
+    a.dx = dx;
+    a.dmut = dmut;
+    a.normalize = normalize;
+    a.rbf = rbf;
+    a.gamma = gamma;
+    a.out = out;
+    size_t dbl = 4 * (size_t)lmax + (mode == SW_ISM ? 5 * (size_t)lmax : 0) + 32;
+    size_t lds = dbl * sizeof(double) + 64 * sizeof(unsigned long long) +
+                 ((size_t)nwx + 2 * (size_t)nwy) * sizeof(P2) + (wtab ? (size_t)wtab_len : 0);
+    if (lds > 160 * 1024)
+        return fail(-2, "sweep: needs %zu bytes of LDS (sequence too long)", lds);
+    sweep_fn f = pick_sweep(l, nm, mode);
+    if (lds > 64 * 1024)
+        if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
+    dim3 grid((unsigned)B, (unsigned)((S + sv_per_block - 1) / sv_per_block));
+    hipLaunchKernelGGL(f, grid, dim3(SWEEP_THREADS), lds, S_(stream), a);
+    GKM_CHECK_LAUNCH("sweep");
+    return 0;
+}
+
+GKM_API int gkm_explain(int perturbation, const void *xplanes, const int32_t *xlen, int lmax,
+                        int B, int nwx, const void *yplanes, const void *yplanes_rc,
+                        const int32_t *ylen, int S, int nwy, int l, int nm, int cmax,
+                        const uint8_t *wtab, int wtab_len, int sv_per_block, const double *ta,
+                        const double *tb, const double *ec, double *out, void *stream)
+{
+    return sweep_launch(perturbation ? SW_PERT : SW_HYP, xplanes, xlen, nullptr, lmax, B, nwx,
+                        yplanes, yplanes_rc, ylen, S, nwy, l, nm, cmax, wtab, wtab_len,
+                        sv_per_block, ta, tb, ec, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                        0, 0.0, out, stream);
+}
+
+GKM_API int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int lmax,
+                    int B, int nwx, const void *yplanes, const void *yplanes_rc,
+                    const int32_t *ylen, int S, int nwy, int l, int nm, const uint8_t *wtab,
+                    int wtab_len, int sv_per_block, const double *tab, const double *coef,
+                    const double *ds, const double *dx, const double *dmut, int normalize, int rbf,
+                    double gamma, double *out, void *stream)
+{
+    // Hits: windows within one mismatch of the cutoff (count <= dmax + 1).
+    return sweep_launch(SW_ISM, xplanes, xlen, xcodes, lmax, B, nwx, yplanes, yplanes_rc, ylen, S,
+                        nwy, l, nm, nm - 1, wtab, wtab_len, sv_per_block, nullptr, nullptr, nullptr,
+                        tab, coef, ds, dx, dmut, normalize, rbf, gamma, out, stream);
+}
+
+GKM_API int gkm_sync(void *stream)
+{
+    hipError_t e = hipStreamSynchronize(S_(stream));
+    if (e != hipSuccess)
+        return fail(-4, "gkm_sync: %s", hipGetErrorString(e));
+    return 0;
+}

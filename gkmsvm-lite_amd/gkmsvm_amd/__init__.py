@@ -1,0 +1,33 @@
+"""gkmsvm_amd: an MI355X (gfx950) engine for gapped k-mer SVM kernels.
+
+Drop-in for gkmsvm-lite's scoring, variant scoring, GkmExplain, ISM and the
+Gram build behind train_gkmsvm; the work runs in hand-written HIP kernels
+(libgkmhip.so, C-ABI in include/gkmhip.h) called through ctypes.
+"""
+
+from .codec import codes_to_one_hot, encode_codes, one_hot_encode, to_codes
+from .explain import gkmexplain, mutation_impact
+from .gram import compute_gram
+from .importers import load_lsgkm_model, read_fasta
+from .ism import ism
+from .kernels import (
+    KERNEL_ALIASES,
+    KERNEL_BUILDERS,
+    CenterWeightedGkmKernel,
+    CenterWeightedRbfGkmKernel,
+    DirectGkmKernel,
+    EstTruncGkmKernel,
+    RbfGkmKernel,
+    RbfKernel,
+    resolve_kernel_type,
+)
+from .svm import GkmSVM, to_cpu
+from .train import train_gkmsvm
+
+__all__ = [
+    "GkmSVM", "gkmexplain", "mutation_impact", "ism", "compute_gram", "train_gkmsvm",
+    "load_lsgkm_model", "read_fasta", "one_hot_encode", "encode_codes", "to_codes",
+    "codes_to_one_hot", "to_cpu", "resolve_kernel_type", "KERNEL_BUILDERS", "KERNEL_ALIASES",
+    "DirectGkmKernel", "EstTruncGkmKernel", "CenterWeightedGkmKernel", "RbfKernel",
+    "RbfGkmKernel", "CenterWeightedRbfGkmKernel",
+]

@@ -1,0 +1,84 @@
+"""DNA strings, one-hot arrays and base codes (A=0, C=1, G=2, T=3).
+
+N (an all-zero one-hot column) and, with ``allow_other``, any other letter are
+stored as A, which is how the kernels read them on the forward strand (as
+LS-GKM does); their reverse complement then holds T.  Mirrors
+codec.py:40-137 of gkmsvm-lite (one_hot_encode, encode_codes, to_codes).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+BASES = "ACGT"
+
+# byte -> code; 4 marks N/n, 5 any other letter
+_LUT = np.full(256, 5, dtype=np.uint8)
+for _i, _b in enumerate(BASES):
+    _LUT[ord(_b)] = _i
+    _LUT[ord(_b.lower())] = _i
+_LUT[ord("N")] = 4
+_LUT[ord("n")] = 4
+
+
+def _raw_codes(sequence: str, allow_n: bool, allow_other: bool) -> np.ndarray:
+    if len(sequence) == 0:
+        raise ValueError("Sequence must be non-empty")
+    raw = _LUT[np.frombuffer(sequence.encode("ascii"), dtype=np.uint8)]
+    if (raw == 4).any() and not (allow_n or allow_other):
+        raise ValueError("Invalid base in sequence (pass allow_n=True for N)")
+    if (raw == 5).any() and not allow_other:
+        raise ValueError(
+            "Invalid base in sequence (pass allow_other=True to read non-ACGTN letters as N)"
+        )
+    return raw
+
+
+def encode_codes(sequence: str, *, allow_n: bool = False, allow_other: bool = False) -> np.ndarray:
+    """[L] uint8 base codes of a DNA string; N (and other letters with
+    allow_other) stored as A."""
+    raw = _raw_codes(sequence, allow_n, allow_other)
+    return np.where(raw > 3, 0, raw).astype(np.uint8)
+
+
+def one_hot_encode(sequence: str, *, dtype=np.float32, allow_n: bool = False,
+                   allow_other: bool = False) -> np.ndarray:
+    """[4, L] one-hot (channels A, C, G, T); N is an all-zero column."""
+    raw = _raw_codes(sequence, allow_n, allow_other)
+    out = np.zeros((4, len(raw)), dtype=dtype)
+    keep = raw < 4
+    out[raw[keep], np.nonzero(keep)[0]] = 1
+    return out
+
+
+def to_codes(x) -> np.ndarray:
+    """[B, L] uint8 codes of a [B, 4, L] one-hot batch (an all-zero column, N,
+    becomes A), or [B, L] integer codes as they are.  Accepts NumPy arrays or
+    torch tensors (moved to the host)."""
+    if hasattr(x, "detach"):  # torch tensor
+        x = x.detach().cpu().numpy()
+    x = np.asarray(x)
+    if x.ndim == 2:
+        if x.dtype == np.uint8:
+            return x
+        if not (np.issubdtype(x.dtype, np.integer) or x.dtype == np.bool_):
+            raise ValueError(f"[B, L] base codes must be integers, got {x.dtype}")
+        return x.astype(np.uint8)
+    if x.ndim != 3 or x.shape[1] != 4:
+        raise ValueError(
+            f"expected a [B, 4, L] one-hot array or [B, L] base codes, got {list(x.shape)}"
+        )
+    return np.argmax(x, axis=1).astype(np.uint8)
+
+
+def codes_to_one_hot(codes, dtype=np.float64) -> np.ndarray:
+    """[B, 4, L] one-hot of [B, L] codes (an N stored as A is one-hot A)."""
+    codes = np.asarray(codes)
+    B, L = codes.shape
+    out = np.zeros((B, 4, L), dtype=dtype)
+    out[np.arange(B)[:, None], codes.astype(np.int64), np.arange(L)[None, :]] = 1
+    return out
+
+
+def decode_codes(codes) -> str:
+    return "".join(BASES[int(c)] for c in np.asarray(codes).ravel())

@@ -1,0 +1,236 @@
+"""Device-side orchestration: sequence tables in HBM, launch shapes and the
+calls into libgkmhip (include/gkmhip.h) that every public function uses.
+
+Layout on the GPU (torch tensors, caller-owned device memory):
+  DeviceSeqs.planes / planes_rc  [n, nw, 2] int32  bit-planes of each sequence
+                                                   and of its reverse complement
+  DeviceSeqs.codes               [n, Lmax] uint8   base codes
+  hist                           [B, S, nm] int64  mismatch histograms (uint64 bits)
+SV tables (SVModel) are sorted by length so that each 64-SV group (one wave's
+lanes) holds SVs of nearly one length, and stay resident until dropped.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .kernels import DirectGkmKernel, GkmKernel
+
+WAVE = 64
+# Histogram bytes per cross-kernel call (rows are chunked to stay under it)
+MAX_HIST_BYTES = 1 << 30
+
+
+def device_of(device=None) -> torch.device:
+    if device is None or str(device) in ("rocm", "cuda", "gpu"):
+        if not torch.cuda.is_available():
+            raise RuntimeError(
+                "no ROCm GPU visible: the gkm engine runs on MI355X (gfx950) only"
+            )
+        return torch.device("cuda", torch.cuda.current_device())
+    d = torch.device(device)
+    if d.type != "cuda":
+        raise RuntimeError(f"the gkm engine computes on ROCm GPUs only, not {device!r}")
+    return d
+
+
+def stream_handle(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream or None
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+class DeviceSeqs:
+    """Bit-planes (and, on request, packed windows) of sequences on the GPU."""
+
+    def __init__(self, codes: np.ndarray, lengths: np.ndarray, device, *, rc: bool = True,
+                 stream=None):
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+        self.n, self.lmax = codes.shape
+        self.nw = self.lmax // 32 + 4
+        self.lengths = lengths
+        self.device = device
+        self.codes = torch.from_numpy(codes).to(device, non_blocking=False)
+        self.len = torch.from_numpy(lengths.astype(np.int32)).to(device)
+        self.planes = torch.empty((self.n, self.nw, 2), dtype=torch.int32, device=device)
+        self.planes_rc = torch.empty_like(self.planes) if rc else None
+        _lib.call("gkm_pack_planes", _p(self.codes), _p(self.len), self.n, self.lmax, self.nw,
+                  _p(self.planes), _p(self.planes_rc), stream_handle(stream))
+        self._windows = {}
+
+    def windows(self, l: int, stream=None):
+        """(fwd, rc) packed l-mer windows [n, Lmax - l + 1] int32 (uint32 bits)."""
+        if l not in self._windows:
+            W = self.lmax - l + 1
+            fwd = torch.empty((self.n, W), dtype=torch.int32, device=self.device)
+            rcw = torch.empty_like(fwd)
+            _lib.call("gkm_pack_windows", _p(self.codes), _p(self.len), self.n, self.lmax, l,
+                      _p(fwd), _p(rcw), stream_handle(stream))
+            self._windows[l] = (fwd, rcw)
+        return self._windows[l]
+
+    @property
+    def max_windows(self) -> int:
+        return int(self.lengths.max()) if self.n else 0
+
+
+class KernelParams:
+    """What the kernels need of a GkmKernel, on one device."""
+
+    def __init__(self, kernel: GkmKernel, device):
+        base = kernel.base
+        if not isinstance(base, DirectGkmKernel):
+            raise TypeError(f"unsupported kernel {type(base).__name__}")
+        self.kernel = kernel
+        self.base = base
+        self.l = base.l
+        self.dmax = base.dmax
+        self.nm = max(self.dmax + 1, 1)
+        self.rc = base.include_rc
+        self.normalize = base.normalize
+        self.rbf = kernel.gamma is not None
+        self.gamma = float(kernel.gamma) if self.rbf else 0.0
+        self.device = device
+        self.h = base.h_padded()
+        self.tab = torch.tensor(self.h, dtype=torch.float64, device=device)
+        self.weighted = base.weight_table(1) is not None
+        self._wtab = None
+        self._wtab_n = 0
+
+    def wtab(self, max_len: int):
+        """(uint8 weight-by-distance table on the device covering sequences up to
+        max_len bases, its length) or (None, 0)."""
+        if not self.weighted:
+            return None, 0
+        need = max(1, max_len - self.l + 1) // 2 + 1
+        if self._wtab is None or self._wtab_n < need:
+            n = max(need, 1024)
+            t = self.base.weight_table(2 * (n - 1))
+            self._wtab = torch.from_numpy(np.ascontiguousarray(t)).to(self.device)
+            self._wtab_n = len(t)
+        return self._wtab, self._wtab_n
+
+
+def group_bounds(lengths: np.ndarray) -> np.ndarray:
+    """[ceil(n/64), 2] int32 (min, max) length of each 64-sequence group."""
+    n = len(lengths)
+    g = (n + WAVE - 1) // WAVE
+    pad = np.full(g * WAVE, -1, dtype=np.int64)
+    pad[:n] = lengths
+    blk = pad.reshape(g, WAVE)
+    mx = blk.max(axis=1)
+    mn = np.where(blk < 0, np.iinfo(np.int64).max, blk).min(axis=1)
+    return np.stack([mn, mx], axis=1).astype(np.int32)
+
+
+def launch_shape(B: int, S: int, norient: int, n_cu: int = 256):
+    """(rows_per_block, nsplit) for gkm_cross_hist: enough waves to fill the
+    chip (>= ~16 per CU), each block's 4 waves busy, and grid.y <= 65535."""
+    groups = max(1, (S + WAVE - 1) // WAVE)
+    target_units = 16 * 4 * n_cu  # waves of work
+    per_row = groups * norient
+    nsplit = int(min(16, max(1, math.ceil(target_units / max(1, B * per_row)))))
+    units_per_row = norient * nsplit
+    rows_per_block = max(1, min(B, math.ceil(4 / units_per_row)))
+    rows_per_block = max(rows_per_block, math.ceil(B / 65535))
+    return rows_per_block, nsplit
+
+
+def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeqs,
+               ygroups: torch.Tensor, *, col_min: int = -1, out=None, shape=None,
+               stream=None) -> torch.Tensor:
+    """[x1 - x0, S, nm] int64 histograms of rows x0..x1 of xs against ys."""
+    B = x1 - x0
+    S = ys.n
+    if out is None:
+        out = torch.zeros((B, S, kp.nm), dtype=torch.int64, device=kp.device)
+    else:
+        _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
+    norient = 2 if (kp.rc and ys.planes_rc is not None) else 1
+    rpb, nsplit = shape if shape is not None else launch_shape(B, S, norient)
+    wt, wn = kp.wtab(max(xs.lmax, ys.lmax))
+    xpl = xs.planes[x0:x1]
+    _lib.call(
+        "gkm_cross_hist", _p(xpl), _p(xs.len[x0:x1]), B, xs.nw, _p(ys.planes),
+        _p(ys.planes_rc) if norient == 2 else None, _p(ys.len), _p(ygroups), S, ys.nw, kp.l,
+        kp.nm, _p(wt), wn, rpb, nsplit, col_min, x0, _p(out), stream_handle(stream),
+    )
+    return out
+
+
+def self_kraw(kp: KernelParams, xs: DeviceSeqs, stream=None) -> torch.Tensor:
+    """K_raw(x, x) [n] float64 of every sequence (forward x forward and RC)."""
+    hist = torch.zeros((xs.n, kp.nm), dtype=torch.int64, device=kp.device)
+    wt, wn = kp.wtab(xs.lmax)
+    longest = xs.max_windows
+    # enough diagonal chunks that a lone long sequence still spreads over CUs
+    nd = 2 * (2 * max(longest - kp.l, 0) + 1)
+    nchunks = max(1, min(64, math.ceil(nd / 256), math.ceil(4096 / max(1, xs.n))))
+    _lib.call("gkm_self_hist", _p(xs.planes), _p(xs.planes_rc) if kp.rc else None, _p(xs.len),
+              xs.n, xs.nw, kp.l, kp.nm, _p(wt), wn, nchunks, _p(hist), stream_handle(stream))
+    out = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
+    _lib.call("gkm_combine_kraw", _p(hist), xs.n, kp.nm, _p(kp.tab), _p(out),
+              stream_handle(stream))
+    return out
+
+
+class SVModel:
+    """An SVM's support vectors resident on the GPU: bit-planes sorted by
+    length, 64-SV group bounds, coefficients and self-kernels in that order."""
+
+    def __init__(self, kp: KernelParams, codes: np.ndarray, lengths: np.ndarray,
+                 coef: np.ndarray, stream=None):
+        self.kp = kp
+        order = np.argsort(lengths, kind="stable")
+        self.order = order
+        self.seqs = DeviceSeqs(codes[order], lengths[order], kp.device, rc=kp.rc, stream=stream)
+        self.groups = torch.from_numpy(group_bounds(lengths[order])).to(kp.device)
+        self.coef = torch.from_numpy(np.ascontiguousarray(coef[order], dtype=np.float64)).to(
+            kp.device)
+        self.S = len(order)
+        if kp.normalize:
+            self.ds = self_kraw(kp, self.seqs, stream)
+        else:
+            self.ds = torch.ones(self.S, dtype=torch.float64, device=kp.device)
+
+    def diag_original_order(self) -> torch.Tensor:
+        inv = np.empty_like(self.order)
+        inv[self.order] = np.arange(len(self.order))
+        return self.ds[torch.from_numpy(inv).to(self.ds.device)]
+
+    def nbytes(self) -> int:
+        t = [self.seqs.planes, self.seqs.codes, self.coef, self.ds, self.groups]
+        if self.seqs.planes_rc is not None:
+            t.append(self.seqs.planes_rc)
+        return sum(x.numel() * x.element_size() for x in t)
+
+
+def rows_per_chunk(S: int, nm: int, cap: int = MAX_HIST_BYTES) -> int:
+    return max(1, cap // max(1, S * nm * 8))
+
+
+def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stream=None,
+               shape=None):
+    """out[b] = sum_s coef_s F(K(x_b, s)) + bias for every row of xs."""
+    kp = sv.kp
+    dx = self_kraw(kp, xs, stream) if kp.normalize else None
+    step = rows_per_chunk(sv.S, kp.nm)
+    hist = None
+    for r0 in range(0, xs.n, step):
+        r1 = min(xs.n, r0 + step)
+        if hist is None or hist.shape[0] != r1 - r0:
+            hist = torch.empty((r1 - r0, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
+        cross_hist(kp, xs, r0, r1, sv.seqs, sv.groups, out=hist, stream=stream, shape=shape)
+        _lib.call("gkm_combine_score", _p(hist), r1 - r0, sv.S, kp.nm, _p(kp.tab),
+                  _p(dx[r0:r1]) if dx is not None else None, _p(sv.ds), _p(sv.coef),
+                  int(kp.normalize), int(kp.rbf), kp.gamma, float(bias), _p(out[r0:r1]),
+                  stream_handle(stream))
+    return out

@@ -1,0 +1,168 @@
+"""GkmExplain attributions and the mutation impact score on the GPU
+(gkmsvm-lite explain/core.py:34-248, explain/weights.py:9-57,
+explain/impact.py:51-194)."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import engine as E
+from .codec import codes_to_one_hot
+from .inputs import (
+    allele_arrays,
+    as_batch,
+    full_lengths,
+    joint_lengths,
+    nan_rows,
+    per_position,
+    short_rows,
+)
+
+_MODE_ALIASES = {
+    0: "importance",
+    1: "hypothetical",
+    2: "perturbation",
+    "importance": "importance",
+    "hypothetical": "hypothetical",
+    "perturbation": "perturbation",
+}
+
+
+def explain_tables(base, perturbation: bool):
+    """(table_a, table_b, cmax): per-mismatch-count attribution weights, l + 2
+    entries each, and the largest count with a non-zero weight.
+
+    Importance (explain/weights.py:19-34): alpha[m] = h[m] / (l - m) for a
+    position where the bases agree, kappa[m] = h[m-1] / (l - m + 1) where they
+    differ, m <= d.  Perturbation (explain/weights.py:37-57): the kernel change
+    of one substitution, h[m+1] - h[m] (match -> mismatch) and h[m-1] - h[m]
+    (mismatch -> match)."""
+    l = base.l
+    h = base._mismatch_table
+    d = l if base.d is None else base.d
+    a = np.zeros(l + 2)
+    b = np.zeros(l + 2)
+    if perturbation:
+        for m in range(l + 1):
+            a[m] = (h[m + 1] if m + 1 <= l else 0.0) - h[m]
+            if m >= 1:
+                b[m] = h[m - 1] - h[m]
+    else:
+        for m in range(min(d, l) + 1):
+            if l - m > 0:
+                a[m] = h[m] / (l - m)
+            if m >= 1:
+                b[m] = h[m - 1] / (l - m + 1)
+    nz = [m for m in range(l + 1) if a[m] != 0 or b[m] != 0]
+    return a, b, (max(nz) if nz else -1)
+
+
+def _sv_chunk(B: int, S: int) -> int:
+    """SVs per block of the sweep kernel: enough blocks to fill the chip."""
+    spb = max(1, (B * S) // 8192)
+    return max(spb, -(-S // 65535))
+
+
+def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 50,
+               verbose: bool = False) -> torch.Tensor:
+    """[B, 4, L] float64 GkmExplain attributions on the model's GPU: importance
+    (hypothetical x one-hot; sums to score - bias), hypothetical, or
+    perturbation (lsgkm mode 4).  0 past each sequence's length, NaN rows for
+    sequences shorter than l."""
+    resolved = _MODE_ALIASES.get(mode)
+    if resolved is None:
+        raise ValueError(
+            "mode must be 'importance', 'hypothetical', or 'perturbation' "
+            f"(or 0/1/2), got {mode!r}"
+        )
+    dev = model._require_device()
+    codes, lengths = as_batch(x, lengths)
+    base = model.kernel.base
+    short = short_rows(lengths, base.l, "attributions")
+    lens = full_lengths(codes, lengths)
+    B, L = codes.shape
+    out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
+    if L >= base.l:
+        bs = max(1, int(batch_size)) if batch_size else B
+        starts = range(0, B, bs)
+        if verbose:
+            from tqdm import tqdm
+
+            starts = tqdm(starts, desc="GkmExplain")
+        for s in starts:
+            e = min(B, s + bs)
+            _explain_rows(model, codes[s:e], lens[s:e], resolved, out[s:e])
+    if resolved == "importance":
+        out = out * torch.from_numpy(codes_to_one_hot(codes)).to(dev)
+    return per_position(out, lengths, short)
+
+
+def _explain_rows(model, codes, lens, resolved, out):
+    kp = model._params()
+    sv = model._svs()
+    pert = resolved == "perturbation"
+    ta, tb, cmax = explain_tables(kp.base, pert)
+    if cmax < 0:
+        return
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    B = xs.n
+    st = E.stream_handle()
+    dx = E.self_kraw(kp, xs) if kp.normalize else torch.ones(B, dtype=torch.float64,
+                                                            device=kp.device)
+    hist = None
+    if kp.rbf:
+        hist = E.cross_hist(kp, xs, 0, B, sv.seqs, sv.groups)
+    ec = torch.empty((B, sv.S), dtype=torch.float64, device=kp.device)
+    _lib.call("gkm_explain_coef", E._p(hist), B, sv.S, kp.nm, E._p(kp.tab), E._p(dx),
+              E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma, E._p(ec), st)
+    ta_d = torch.from_numpy(ta).to(kp.device)
+    tb_d = torch.from_numpy(tb).to(kp.device)
+    nm = cmax + 1
+    wt, wn = kp.wtab(max(xs.lmax, sv.seqs.lmax))
+    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
+    _lib.call("gkm_explain", int(pert), E._p(xs.planes), E._p(xs.len), xs.lmax, B, xs.nw,
+              E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
+              E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
+              _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st)
+    out += buf
+
+
+def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
+                    batch_size: int | None = None, verbose: bool = False) -> torch.Tensor:
+    """[B, 1] GkmExplain mutation impact (lsgkm gkmexplain -m 5): sum_j coef_j
+    dK_j with K and dK normalised by the reference allele's norm; RBF kernels
+    scale dK_j by (exp(g(K_j - 1)) - exp(-g)) / K_j (0 for K_j <= 0)."""
+    dev = model._require_device()
+    kp = model._params()
+    sv = model._svs()
+    ref, alt, ref_lengths, alt_lengths = allele_arrays(ref, alt, ref_lengths, alt_lengths)
+    B, L = ref.shape
+    short = None
+    if ref_lengths is not None or alt_lengths is not None:
+        both = joint_lengths(L, (B, ref_lengths), (B, alt_lengths))
+        short = short_rows(both, kp.l, "mutation impacts")
+        if short is not None:
+            short = short.reshape(2, -1).any(axis=0)
+    rl = full_lengths(ref, ref_lengths)
+    al = full_lengths(alt, alt_lengths)
+    out = torch.zeros(B, dtype=torch.float64, device=dev)
+    bs = B if batch_size is None else max(1, int(batch_size))
+    st = E.stream_handle()
+    for s in range(0, B, bs):
+        e = min(B, s + bs)
+        n = e - s
+        codes = np.concatenate([ref[s:e], alt[s:e]])
+        lens = np.concatenate([rl[s:e], al[s:e]])
+        xs = E.DeviceSeqs(codes, lens, dev, rc=kp.rc)
+        if kp.normalize:
+            rs = E.DeviceSeqs(ref[s:e], rl[s:e], dev, rc=kp.rc)
+            dref = E.self_kraw(kp, rs)
+        else:
+            dref = torch.ones(n, dtype=torch.float64, device=dev)
+        hist = E.cross_hist(kp, xs, 0, 2 * n, sv.seqs, sv.groups)
+        _lib.call("gkm_combine_impact", E._p(hist), n, sv.S, kp.nm, E._p(kp.tab), E._p(dref),
+                  E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma,
+                  E._p(out[s:e]), st)
+    return nan_rows(out[:, None], short)

@@ -1,0 +1,72 @@
+"""Gram matrices on the GPU (gkmsvm-lite gram.py:21-164)."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import engine as E
+from .inputs import as_batch, full_lengths
+
+
+def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = False,
+                 out: np.ndarray | None = None, lengths=None, y_lengths=None,
+                 device=None) -> np.ndarray:
+    """[N, M] (or [N, N]) normalised, finished kernel matrix in host memory.
+
+    Without Y, only the upper triangle (with the diagonal) is computed and
+    mirrored, so the matrix is exactly symmetric.  Rows go to the GPU in blocks
+    whose [rows, N] histograms stay under engine.MAX_HIST_BYTES; sequences are
+    sorted by length on the device (64-column groups of one length) and the
+    result is put back in input order.  chunk_size is accepted for API
+    compatibility; the block size follows from memory.
+    """
+    dev = E.device_of(device)
+    kp = E.KernelParams(kernel, dev)
+    xc, xl = as_batch(X, lengths)
+    xl = full_lengths(xc, xl)
+    symmetric = Y is None
+    if symmetric:
+        yc, yl = xc, xl
+    else:
+        yc, yl = as_batch(Y, y_lengths)
+        yl = full_lengths(yc, yl)
+    N, M = xc.shape[0], yc.shape[0]
+    gram = np.zeros((N, M), dtype=np.float64) if out is None else out
+    st = E.stream_handle()
+    ox = np.argsort(xl, kind="stable")
+    xs = E.DeviceSeqs(xc[ox], xl[ox], dev, rc=kp.rc)
+    if symmetric:
+        oy, ys = ox, xs
+    else:
+        oy = np.argsort(yl, kind="stable")
+        ys = E.DeviceSeqs(yc[oy], yl[oy], dev, rc=kp.rc)
+    groups = torch.from_numpy(E.group_bounds(yl[oy])).to(dev)
+    ones_x = torch.ones(N, dtype=torch.float64, device=dev)
+    dx = E.self_kraw(kp, xs) if kp.normalize else ones_x
+    dy = dx if symmetric else (E.self_kraw(kp, ys) if kp.normalize
+                               else torch.ones(M, dtype=torch.float64, device=dev))
+    step = E.rows_per_chunk(M, kp.nm)
+    sorted_g = np.empty((N, M), dtype=np.float64)
+    tile = None
+    rows = range(0, N, step)
+    if verbose:
+        from tqdm import tqdm
+
+        rows = tqdm(rows, desc="Gram matrix")
+    for r0 in rows:
+        r1 = min(N, r0 + step)
+        hist = E.cross_hist(kp, xs, r0, r1, ys, groups, col_min=0 if symmetric else -1)
+        if tile is None or tile.shape[0] != r1 - r0:
+            tile = torch.empty((r1 - r0, M), dtype=torch.float64, device=dev)
+        _lib.call("gkm_combine_gram", E._p(hist), r1 - r0, M, kp.nm, E._p(kp.tab),
+                  E._p(dx[r0:r1]), E._p(dy), int(kp.normalize), int(kp.rbf), kp.gamma,
+                  E._p(tile), st)
+        sorted_g[r0:r1] = tile.cpu().numpy()
+        del hist
+    if symmetric:
+        upper = np.triu(sorted_g)
+        sorted_g = upper + np.triu(upper, 1).T
+    gram[np.ix_(ox, oy)] = sorted_g
+    return gram

@@ -1,0 +1,77 @@
+"""In silico mutagenesis on the GPU (gkmsvm-lite ism.py:21-320).
+
+For every position p and base v, the score change f(x with x_p = v) - f(x).
+The kernel change of a substitution touches only the windows covering p: on a
+diagonal of the (x, SV) pair, a window with c mismatches gains one
+(h[c+1] - h[c]) where x_p matched the SV, and loses one (h[c-1] - h[c]) for the
+base v equal to the SV's where it did not.  One sweep per (x, SV) pair
+gathers those deltas for every (p, v) in LDS; the mutant self-kernels come
+from gkm_mutant_diag; the normalisation, RBF and coefficient sum follow per SV.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from . import engine as E
+from .inputs import as_batch, full_lengths, per_position, short_rows
+
+
+def _sv_chunk(B: int, S: int) -> int:
+    spb = max(1, (B * S) // 8192)
+    return max(spb, -(-S // 65535))
+
+
+def ism(model, x, *, lengths=None, batch_size: int | None = None,
+        verbose: bool = False) -> torch.Tensor:
+    """[B, 4, L] float64 score deltas for every single-base substitution (0 for
+    the reference base and past each sequence's length, NaN rows for sequences
+    shorter than l)."""
+    dev = model._require_device()
+    codes, lengths = as_batch(x, lengths)
+    kp = model._params()
+    short = short_rows(lengths, kp.l, "ISM deltas")
+    lens = full_lengths(codes, lengths)
+    B, L = codes.shape
+    out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
+    if L >= kp.l and kp.dmax >= 0:
+        bs = B if batch_size is None else max(1, int(batch_size))
+        bs = min(bs, 65535)
+        starts = range(0, B, bs)
+        if verbose:
+            from tqdm import tqdm
+
+            starts = tqdm(starts, desc="ISM")
+        for s in starts:
+            e = min(B, s + bs)
+            _ism_rows(model, codes[s:e], lens[s:e], out[s:e])
+    return per_position(out, lengths, short)
+
+
+def _ism_rows(model, codes, lens, out):
+    kp = model._params()
+    sv = model._svs()
+    st = E.stream_handle()
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    B = xs.n
+    if kp.normalize:
+        dx = E.self_kraw(kp, xs)
+        fwd, rcw = xs.windows(kp.l)
+        dmut = torch.empty((4, xs.lmax, B), dtype=torch.float64, device=kp.device)
+        wt, _ = kp.wtab(xs.lmax)
+        _lib.call("gkm_mutant_diag", E._p(fwd), E._p(rcw) if kp.rc else None, E._p(xs.len),
+                  E._p(xs.codes), xs.lmax, B, kp.l, E._p(kp.tab), E._p(wt), E._p(dx),
+                  E._p(dmut), st)
+    else:
+        dx = torch.ones(B, dtype=torch.float64, device=kp.device)
+        dmut = torch.ones((4, xs.lmax, B), dtype=torch.float64, device=kp.device)
+    nm = min(kp.dmax + 2, kp.l + 1)
+    wt, wn = kp.wtab(max(xs.lmax, sv.seqs.lmax))
+    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
+    _lib.call("gkm_ism", E._p(xs.planes), E._p(xs.len), E._p(xs.codes), xs.lmax, B, xs.nw,
+              E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
+              E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, E._p(wt), wn, _sv_chunk(B, sv.S),
+              E._p(kp.tab), E._p(sv.coef), E._p(sv.ds), E._p(dx), E._p(dmut), int(kp.normalize),
+              int(kp.rbf), kp.gamma, E._p(buf), st)
+    out += buf

@@ -1,0 +1,144 @@
+/*
+ * gkmhip.h -- C-ABI of the MI355X (gfx950) gapped k-mer engine
+ * (libgkmhip.so, built from gkmsvm-lite_amd/csrc/gkm_kernels.hip).
+ *
+ * Every pointer argument is a DEVICE pointer owned by the caller (in the
+ * Python package: torch tensors on the ROCm device), except where noted.
+ * `stream` is a hipStream_t (NULL = the default stream).  Every call is
+ * asynchronous on `stream`; it returns 0 or a negative status, with the
+ * message in gkm_last_error() (thread-local).  No call allocates device
+ * memory.
+ *
+ * Data layout (see DESIGN.md for the full description):
+ *   planes / planes_rc  [n][nw] pairs of uint32 (bit-plane 0, bit-plane 1):
+ *                       base t of a sequence (or of its reverse complement)
+ *                       at bit t & 31 of word (t >> 5) + 1, word 0 zero;
+ *                       nw = gkm_planes_words(Lmax).
+ *   hist                [rows][cols][nm] uint64 mismatch histograms N[m].
+ *   tab                 h[m], m = 0..l (+1 zero), float64: the kernel type's
+ *                       pair weight by mismatch count.
+ *   wtab                [Wmax/2 + 1] uint8 window weight by distance from a
+ *                       sequence's centre (-t 4/-t 5), or NULL.
+ *
+ * Citations name the gkmsvm-lite function (src/gkmsvm/...) each entry point
+ * replaces; the Python wrappers in gkmsvm-lite_amd/gkmsvm_amd keep the
+ * reference's public API on top of these.
+ */
+#ifndef GKMHIP_H
+#define GKMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread-local message of the last failed call. */
+const char *gkm_last_error(void);
+/* ABI version (1). */
+int gkm_version(void);
+/* Plane words per sequence for sequences up to lmax bases: lmax / 32 + 4. */
+int gkm_planes_words(int lmax);
+/* Wait for `stream`. */
+int gkm_sync(void *stream);
+/* Zero n uint64 with a kernel (usable under stream capture). */
+int gkm_zero_u64(void *p, int64_t n, void *stream);
+
+/* Base codes [n][lmax] uint8 (A=0 C=1 G=2 T=3; N stored as A) -> planes and
+ * reverse-complement planes (planes_rc may be NULL).
+ * Replaces PackedGkmKernel.pack / pack_both (kernels/packed.py:130-187). */
+int gkm_pack_planes(const uint8_t *codes, const int32_t *lengths, int n, int lmax, int nw,
+                    void *planes, void *planes_rc, void *stream);
+
+/* Base codes -> 2-bit packed l-mer windows [n][lmax-l+1] (PAD 0xFFFFFFFF past
+ * each row's windows); rcw: windows of the reverse complement from its first
+ * base (may be NULL).  kernels/packed.py:40-53,130-169. */
+int gkm_pack_windows(const uint8_t *codes, const int32_t *lengths, int n, int lmax, int l,
+                     uint32_t *fwd, uint32_t *rcw, void *stream);
+
+/* Mismatch histograms of rows x against columns y and their reverse
+ * complements, ADDED into hist [B][S][nm] (zero it first): N[m] = number (or,
+ * with wtab, weight sum wx*wy) of window pairs with m mismatches, m < nm.
+ * ygroups [ceil(S/64)][2] int32 = min and max length of each 64-column group.
+ * rows_per_block, nsplit: launch shape (rows per block, diagonal chunks).
+ * col_min >= 0: only columns s >= row_base + b + col_min (self-Gram upper
+ * triangle).  The integer content of K_raw in PackedGkmKernel.raw_packed
+ * (kernels/packed.py:191-228; kernels/direct.py:24-42; kernels/weighted.py:47-69)
+ * and LmerIndex.pairwise (kernels/lmer_index.py:239-268). */
+int gkm_cross_hist(const void *xplanes, const int32_t *xlen, int B, int nwx,
+                   const void *yplanes, const void *yplanes_rc, const int32_t *ylen,
+                   const int32_t *ygroups, int S, int nwy, int l, int nm, const uint8_t *wtab,
+                   int wtab_len, int rows_per_block, int nsplit, int64_t col_min,
+                   int64_t row_base, void *hist, void *stream);
+
+/* Self-kernel histograms of x against x and rc(x), ADDED into hist [n][nm].
+ * raw_diag_packed (kernels/packed.py:234-238; kernels/direct.py:110-137;
+ * kernels/weighted.py:91-118). */
+int gkm_self_hist(const void *planes, const void *planes_rc, const int32_t *len, int n, int nw,
+                  int l, int nm, const uint8_t *wtab, int wtab_len, int nchunks, void *hist,
+                  void *stream);
+
+/* out[i] = sum_m tab[m] hist[i][m] (K_raw), i < n. */
+int gkm_combine_kraw(const void *hist, int64_t n, int nm, const double *tab, double *out,
+                     void *stream);
+
+/* out[b][s] = F(K_raw / max(sqrt(dx[b] ds[s]), 1e-10)), F = exp(gamma(k-1))
+ * when rbf: one Gram tile (gram.py:82-106 tile normalise + finish). */
+int gkm_combine_gram(const void *hist, int B, int S, int nm, const double *tab, const double *dx,
+                     const double *ds, int normalize, int rbf, double gamma, double *out,
+                     void *stream);
+
+/* out[b] = sum_s coef[s] F(K_raw[b][s] / max(sqrt(dx ds), 1e-10)) + bias, a
+ * fixed-order reduction per row.  svm.py:43-61 (_normalized_dot),
+ * svm.py:368-405 (_sv_scores). */
+int gkm_combine_score(const void *hist, int B, int S, int nm, const double *tab,
+                      const double *dx, const double *ds, const double *coef, int normalize,
+                      int rbf, double gamma, double bias, double *out, void *stream);
+
+/* Mutation impact from hist rows [ref (n); alt (n)]: explain/impact.py:22-48
+ * (_impact_cpu). */
+int gkm_combine_impact(const void *hist, int n, int S, int nm, const double *tab,
+                       const double *dref, const double *ds, const double *coef, int normalize,
+                       int rbf, double gamma, double *out, void *stream);
+
+/* GkmExplain effective coefficients ec [B][S] = coef / max(sqrt(dx ds),1e-10)
+ * (x _rbf_scale for RBF): explain/core.py:203-216, explain/common.py:27-32. */
+int gkm_explain_coef(const void *hist, int B, int S, int nm, const double *tab, const double *dx,
+                     const double *ds, const double *coef, int normalize, int rbf, double gamma,
+                     double *ec, void *stream);
+
+/* K_raw(mutant, mutant) [4][lmax][n] for every single-base substitution, from
+ * the packed windows (gkm_pack_windows), the codes and diag = K_raw(x, x).
+ * raw_diag_mutants (kernels/packed.py:244-253; kernels/direct.py:141-201;
+ * kernels/weighted.py:122-197).  n <= 65535. */
+int gkm_mutant_diag(const uint32_t *fwd, const uint32_t *rcw, const int32_t *len,
+                    const uint8_t *codes, int lmax, int n, int l, const double *tab,
+                    const uint8_t *wtab, const double *diag, double *out, void *stream);
+
+/* GkmExplain per-SV attributions ADDED into out [B][4][lmax]: hypothetical
+ * (perturbation = 0: ta = alpha, tb = kappa) or perturbation (ta = gamma,
+ * tb = kappa deltas) over window pairs with count <= cmax; ec from
+ * gkm_explain_coef.  explain/cpu.py:14-65 (_explain_cpu), explain/cuda.py:22-171
+ * (fused_explain).  S / sv_per_block <= 65535. */
+int gkm_explain(int perturbation, const void *xplanes, const int32_t *xlen, int lmax, int B,
+                int nwx, const void *yplanes, const void *yplanes_rc, const int32_t *ylen, int S,
+                int nwy, int l, int nm, int cmax, const uint8_t *wtab, int wtab_len,
+                int sv_per_block, const double *ta, const double *tb, const double *ec,
+                double *out, void *stream);
+
+/* ISM: out [B][4][lmax] ADDED with sum_s coef_s (F(k_mut) - F(k_ref)) for every
+ * substitution (0 for the reference base); nm = dmax + 2; tab = h (l + 2
+ * entries, zero past dmax); ds, dx = K_raw self-kernels of SVs and rows;
+ * dmut from gkm_mutant_diag.  ism.py:194-251 (_ism_kernel) and
+ * kernels/lmer_index_cpu.py:203-285 (_query_mutants_cpu). */
+int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int lmax, int B,
+            int nwx, const void *yplanes, const void *yplanes_rc, const int32_t *ylen, int S,
+            int nwy, int l, int nm, const uint8_t *wtab, int wtab_len, int sv_per_block,
+            const double *tab, const double *coef, const double *ds, const double *dx,
+            const double *dmut, int normalize, int rbf, double gamma, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKMHIP_H */

@@ -1,0 +1,293 @@
+"""GPU parity: the HIP engine (through its C-ABI, via the package) against the
+CPU oracle on the same inputs -- histograms bit-exact, FP64 values within
+1e-10 relative (helpers.assert_close) -- and against the LS-GKM golden files
+at their printed precision (rtol 2e-5)."""
+
+import numpy as np
+import pytest
+import torch
+
+import helpers as H
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_codes(rng, n, lo, hi, binary_frac=0.3):
+    lens = rng.integers(lo, hi + 1, n)
+    codes = np.zeros((n, int(lens.max())), np.uint8)
+    for i in range(n):
+        alphabet = 2 if rng.random() < binary_frac else 4  # low-complexity rows: many hits
+        codes[i, : lens[i]] = rng.integers(0, alphabet, lens[i])
+    return codes, lens.astype(np.int64)
+
+
+def _params(kt, l, k, rng):
+    p = {"L": l, "k": k, "include_rc": bool(rng.integers(0, 2))}
+    if kt in (4, 5):
+        p["M"], p["H"] = int(rng.integers(1, 60)), float(rng.uniform(5, 60))
+    if kt in (3, 5, 6):
+        p["gamma"] = float(rng.uniform(0.5, 3))
+    if kt == 0 and rng.integers(0, 2):
+        p["d"] = int(rng.integers(1, 4))
+    return p
+
+
+def _model(kt, p, codes, lens, coef, bias=0.0):
+    from gkmsvm_amd import GkmSVM
+
+    sv_l = None if (lens == codes.shape[1]).all() else lens
+    return GkmSVM(codes, coef, bias, kt, p, sv_lengths=sv_l)
+
+
+def _oracle_model(m):
+    from gkmsvm_amd.inputs import full_lengths
+
+    kern = O.Kern.make(m.kernel_type, m.kernel_params)
+    return O.Model(kern, m.support_sequences, full_lengths(m.support_sequences, m.sv_lengths),
+                   m.coefficients, m.bias)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_cross_and_self_histograms_bit_exact(gpu, seed):
+    """N[m] of every (query, SV) pair and every self pair equals the oracle's
+    brute-force count (weighted: the sum of wx * wy), for l from 3 to 13,
+    ragged lengths (some shorter than l), RC on/off, split launch shapes."""
+    from gkmsvm_amd import engine as E
+    from gkmsvm_amd.kernels import build_kernel
+
+    rng = np.random.default_rng(seed)
+    l = int(rng.choice([3, 6, 9, 10, 11, 12, 13]))
+    k = max(1, l - int(rng.integers(2, 5)))
+    kt = int(rng.choice([0, 1, 2, 4]))
+    p = _params(kt, l, k, rng)
+    kern = build_kernel(kt, p)
+    okern = O.Kern.make(kt, p)
+    print(f"[case] seed={seed} l={l} k={k} kt={kt} p={p}", flush=True)
+    xc, xl = _rand_codes(rng, int(rng.integers(1, 9)), 1, 300)
+    yc, yl = _rand_codes(rng, int(rng.integers(1, 200)), 1, 400)
+    if xc.shape[1] < l or yc.shape[1] < l:
+        pytest.skip("no windows")
+    kp = E.KernelParams(kern, gpu)
+    xs = E.DeviceSeqs(xc, xl, gpu, rc=kp.rc)
+    ys = E.DeviceSeqs(yc, yl, gpu, rc=kp.rc)
+    groups = torch.from_numpy(E.group_bounds(yl)).to(gpu)
+    want = O.pair_hist(okern, O.pack_both(okern, xc, xl), O.pack_both(okern, yc, yl))
+    for shape in (None, (1, 1), (3, 5)):
+        got = E.cross_hist(kp, xs, 0, xs.n, ys, groups, shape=shape)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got.cpu().numpy().view(np.uint64), want,
+                                      err_msg=f"shape={shape}")
+    # self-kernels
+    hist = torch.zeros((ys.n, kp.nm), dtype=torch.int64, device=gpu)
+    wt, wn = kp.wtab(ys.lmax)
+    from gkmsvm_amd import _lib
+
+    _lib.call("gkm_self_hist", E._p(ys.planes), E._p(ys.planes_rc) if kp.rc else None,
+              E._p(ys.len), ys.n, ys.nw, kp.l, kp.nm, E._p(wt), wn, 3, E._p(hist), None)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hist.cpu().numpy().view(np.uint64),
+                                  O.self_hist(okern, O.pack_both(okern, yc, yl)))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_upper_triangle_columns(gpu, seed):
+    """col_min restricts row b to columns >= b + row_base; the rest stay 0."""
+    from gkmsvm_amd import engine as E
+    from gkmsvm_amd.kernels import build_kernel
+
+    rng = np.random.default_rng(100 + seed)
+    kern = build_kernel(2, {"L": 11, "k": 7})
+    okern = O.Kern.make(2, {"L": 11, "k": 7})
+    yc, yl = _rand_codes(rng, 150, 20, 120)
+    kp = E.KernelParams(kern, gpu)
+    ys = E.DeviceSeqs(yc, yl, gpu)
+    groups = torch.from_numpy(E.group_bounds(yl)).to(gpu)
+    want = O.pair_hist(okern, O.pack_both(okern, yc, yl), O.pack_both(okern, yc, yl))
+    r0, r1 = 37, 121
+    got = E.cross_hist(kp, ys, r0, r1, ys, groups, col_min=0).cpu().numpy().view(np.uint64)
+    for b in range(r1 - r0):
+        np.testing.assert_array_equal(got[b, r0 + b:], want[r0 + b, r0 + b:])
+        assert (got[b, : r0 + b] == 0).all()
+
+
+@pytest.mark.parametrize("kind", H.WGKM_TYPES)
+@pytest.mark.parametrize("n", (100, 85))
+def test_scores_lsgkm_fixtures(gpu, kind, n):
+    """-t 0..6 fixture models: scores equal the oracle to 1e-10 and gkmpredict
+    to its printed digits."""
+    from gkmsvm_amd import load_lsgkm_model
+
+    m = load_lsgkm_model(H.path(f"lsgkm_wgkm_{kind}.model.txt.gz"))
+    recs = O.read_fasta(H.path(f"lsgkm_wgkm_test{n}.fa"))
+    seqs = [s for _, s in recs]
+    got = m(seqs).cpu().numpy().ravel()
+    om = _oracle_model(m)
+    codes, lens = O.batch(seqs)
+    H.assert_close(got, O.score(om, codes, lens), 1e-10, f"{kind}_{n} vs oracle")
+    pred = H.read_pred(H.path(f"lsgkm_wgkm_{kind}_test{n}.pred.txt"))
+    want = np.array([pred[name] for name, _ in recs])
+    np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("kind", ("t2", "t4", "t5"))
+def test_ragged_models_gkmpredict(gpu, kind):
+    """Models trained on 60-140 bp sequences (ragged SVs) on 30-200 bp queries."""
+    from gkmsvm_amd import load_lsgkm_model
+
+    m = load_lsgkm_model(H.path(f"lsgkm_ragged_{kind}.model.txt.gz"))
+    recs = O.read_fasta(H.path("lsgkm_ragged_test.fa"))
+    seqs = [s for _, s in recs]
+    got = m(seqs).cpu().numpy().ravel()
+    om = _oracle_model(m)
+    codes, lens = O.batch(seqs)
+    H.assert_close(got, O.score(om, codes, lens), 1e-10, "vs oracle")
+    pred = H.read_pred(H.path(f"lsgkm_ragged_{kind}_test.pred.txt"))
+    want = np.array([pred[name] for name, _ in recs])
+    ok = ~np.isnan(got)
+    np.testing.assert_allclose(got[ok], want[ok], rtol=2e-5, atol=1e-5)
+
+
+def test_encode_10sv_gkmpredict(gpu):
+    import json
+    import random
+
+    from gkmsvm_amd import load_lsgkm_model
+
+    m = load_lsgkm_model(H.path("encode_ENCFF579AOX_10sv.model.txt.gz"))
+    gold = json.load(open(H.path("encode_10sv_gkmpredict.json")))["scores"]
+    random.seed(42)
+    rand = "".join(random.choice("ACGT") for _ in range(300))
+    from gkmsvm_amd.codec import decode_codes
+
+    qs = [decode_codes(m.support_sequences[i, : (m.sv_lengths[i] if m.sv_lengths is not None
+                                                    else m.support_sequences.shape[1])])
+          for i in range(3)] + [rand]
+    got = m(qs).cpu().numpy().ravel()
+    want = np.array([gold[k] for k in ("query_0", "query_1", "query_2", "random")])
+    np.testing.assert_allclose(got, want, atol=1e-4)
+
+
+@pytest.mark.parametrize("name", sorted(H.EXPLAIN_SETS))
+@pytest.mark.parametrize("mode,lm", [("importance", 0), ("hypothetical", 1), ("perturbation", 4)])
+def test_gkmexplain_fixtures(gpu, name, mode, lm):
+    """GkmExplain equals the oracle's per-SV loop to 1e-10 and the kundajelab
+    lsgkm gkmexplain output at its printed precision."""
+    from gkmsvm_amd import gkmexplain, load_lsgkm_model
+
+    mf, fa, pre = H.EXPLAIN_SETS[name]
+    m = load_lsgkm_model(H.path(mf))
+    seqs = [s for _, s in O.read_fasta(H.path(fa))]
+    got = gkmexplain(m, seqs, mode=mode, batch_size=8).cpu().numpy()
+    om = _oracle_model(m)
+    codes, lens = O.batch(seqs)
+    H.assert_close(got, O.explain(om, codes, lens, mode), 1e-10, f"{name} {mode}")
+    exp = H.read_explain(H.path(f"{pre}.explain{lm}.txt.gz"))
+    scale = max(np.abs(e).max() for e in exp)
+    for i, e in enumerate(exp):
+        np.testing.assert_allclose(got[i, :, : e.shape[1]], e, rtol=2e-5, atol=5e-6 * scale)
+
+
+@pytest.mark.parametrize("name", sorted(k for k in H.EXPLAIN_SETS if not k.endswith("_51")))
+def test_mutation_impact_fixtures(gpu, name):
+    """score_variants(method='gkmexplain') equals the oracle and gkmexplain -m 5."""
+    from gkmsvm_amd import load_lsgkm_model
+
+    mf, fa, pre = H.EXPLAIN_SETS[name]
+    m = load_lsgkm_model(H.path(mf))
+    seqs = [s for _, s in O.read_fasta(H.path(fa))]
+    ref, alt = H.mode5_pairs(seqs, fa)
+    got = m.score_variants(ref, alt, method="gkmexplain", batch_size=16).cpu().numpy().ravel()
+    om = _oracle_model(m)
+    rc, rl = O.batch(ref)
+    ac, al = O.batch(alt)
+    H.assert_close(got, O.impact(om, rc, rl, ac, al), 1e-10, name)
+    exp = np.stack([e[:, 0] for e in H.read_explain(H.path(f"{pre}.explain5.txt.gz"))])
+    np.testing.assert_allclose(got.reshape(-1, 4), exp, rtol=2e-5, atol=5e-6 * np.abs(exp).max())
+
+
+@pytest.mark.parametrize("mf", ["lsgkm_wgkm_t0.model.txt.gz", "lsgkm_wgkm_t1.model.txt.gz",
+                                "lsgkm_mode5_est.model.txt.gz", "lsgkm_mode5_rbf.model.txt.gz",
+                                "lsgkm_wgkm_t4.model.txt.gz", "lsgkm_wgkm_t5.model.txt.gz",
+                                "lsgkm_wgkm_t6.model.txt.gz", "lsgkm_ragged_t5.model.txt.gz"])
+def test_ism_matches_rescoring(gpu, mf):
+    """ISM equals f(mutant) - f(x) with every mutant scored from scratch by the
+    oracle (ragged queries, N bases, lengths with odd and even window counts)."""
+    from gkmsvm_amd import ism, load_lsgkm_model
+
+    m = load_lsgkm_model(H.path(mf))
+    seqs = [s for _, s in O.read_fasta(H.path("lsgkm_ragged_query.fa"))][:5]
+    got = ism(m, seqs).cpu().numpy()
+    om = _oracle_model(m)
+    codes, lens = O.batch(seqs)
+    want = O.ism(om, codes, lens)
+    H.assert_close(got, want, 1e-9, mf)
+    # the reference base changes nothing
+    for b, s in enumerate(seqs):
+        for pos in range(len(s)):
+            assert got[b, codes[b, pos], pos] == 0.0
+
+
+def test_score_variants_kernel(gpu):
+    from gkmsvm_amd import load_lsgkm_model
+
+    m = load_lsgkm_model(H.path("lsgkm_mode5_rbf.model.txt.gz"))
+    seqs = [s for _, s in O.read_fasta(H.path("lsgkm_mode5_test51.fa"))]
+    ref, alt = H.mode5_pairs(seqs, "x")
+    got = m.score_variants(ref, alt).cpu().numpy().ravel()
+    om = _oracle_model(m)
+    rc, rl = O.batch(ref)
+    ac, al = O.batch(alt)
+    H.assert_close(got, O.score(om, ac, al) - O.score(om, rc, rl), 1e-9, "variants")
+
+
+@pytest.mark.parametrize("kt", [2, 3, 4, 0])
+def test_gram_matches_oracle(gpu, kt):
+    """Symmetric Gram (upper triangle computed, mirrored, exactly symmetric) and
+    a rectangular one, ragged lengths, against the oracle."""
+    from gkmsvm_amd import compute_gram
+    from gkmsvm_amd.kernels import build_kernel
+
+    rng = np.random.default_rng(kt)
+    p = {"L": 10, "k": 6, "gamma": 1.5} if kt == 3 else {"L": 10, "k": 6}
+    kern = build_kernel(kt, p)
+    okern = O.Kern.make(kt, p)
+    xc, xl = _rand_codes(rng, 140, 40, 160)
+    g = compute_gram(kern, xc, lengths=xl)
+    assert (g == g.T).all()
+    H.assert_close(g, O.gram(okern, xc, xl), 1e-10, "symmetric")
+    yc, yl = _rand_codes(rng, 70, 30, 90)
+    g2 = compute_gram(kern, xc, yc, lengths=xl, y_lengths=yl)
+    H.assert_close(g2, O.gram(okern, xc, xl, yc, yl), 1e-10, "rectangular")
+
+
+def test_plain_scoring_config_against_index_oracle(gpu):
+    """The plain -t 2 l=11 k=7 d=3 workload (500 SVs x 300 bp, 1,000 queries x
+    300 bp, sign-balanced N(0,1) coefficients) equals the oracle's l-mer index
+    path (the reference's CPU algorithm) to 1e-10."""
+    rng = np.random.default_rng(1)
+    svc = rng.integers(0, 4, (500, 300)).astype(np.uint8)
+    coef = np.abs(np.random.default_rng(2).normal(size=500))
+    coef[250:] *= -1
+    q = np.random.default_rng(3).integers(0, 4, (1000, 300)).astype(np.uint8)
+    p = {"L": 11, "k": 7, "d": 3}
+    m = _model(2, p, svc, np.full(500, 300), coef)
+    got = m(q).cpu().numpy().ravel()
+    om = O.Model(O.Kern.make(2, p), svc, np.full(500, 300), coef, 0.0)
+    want = O.LmerIndex(om).score(q, np.full(1000, 300))
+    H.assert_close(got, want, 1e-10, "C1")
+    # and the brute-force oracle on a slice
+    H.assert_close(got[:50], O.score(om, q[:50], np.full(50, 300)), 1e-10, "C1 brute")
+
+
+def test_short_and_n_rows(gpu):
+    """Rows shorter than l score NaN; N bases score as A (RC: T)."""
+    from gkmsvm_amd import load_lsgkm_model
+
+    m = load_lsgkm_model(H.path("lsgkm_mode5_est.model.txt.gz"))
+    seqs = ["ACGT", "ACGTNNNNNACGTACGGATCGATTTACGA" * 3, "NNNNNNNNNNNNNNNNNNNN"]
+    got = m(seqs).cpu().numpy().ravel()
+    assert np.isnan(got[0]) and np.isfinite(got[1:]).all()
+    om = _oracle_model(m)
+    codes, lens = O.batch(seqs)
+    H.assert_close(got, O.score(om, codes, lens), 1e-10, "N rows")
