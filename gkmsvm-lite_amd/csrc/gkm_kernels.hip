@@ -37,6 +37,9 @@
 
 #define GKM_API extern "C" __attribute__((visibility("default")))
 
+static int check_l_nm(int l, int nm);
+static inline hipStream_t S_(void *stream);
+
 using namespace gkm;
 
 // ---------------------------------------------------------------------------
@@ -733,6 +736,152 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
         if (r != 0.0)
             atomicAdd(&ob[v * lmax + p], r);
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_index_tile: the same histograms through an l-mer index of the SV windows
+// ---------------------------------------------------------------------------
+
+constexpr int INDEX_THREADS = 256;
+
+struct IndexArgs {
+    const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
+    const int32_t *xlen;     // [B]
+    const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
+    const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
+    const uint8_t *vals;     // [entries] window weight of each entry, or NULL
+    const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood (sorted)
+    const uint8_t *levels;   // [n_masks] mismatch count of each mask
+    const uint8_t *wtab;     // weight by distance, or NULL
+    uint32_t *partial;       // [nchunks][B][S][nm] per-chunk counts
+    int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
+    int64_t col_min, row_base;
+};
+
+// Block = (SV tile, chunk of query windows, query row).  Its LDS holds the
+// tile's counts [tile][nm]; each wave takes a query window and its lanes 64
+// neighbour masks at a time: v = w ^ mask is an l-mer within the mask's
+// mismatch count of w, and every window of a tile SV equal to v adds (its
+// weight times the query window's) to N[level] of that pair -- the pairs
+// kernels/lmer_index_cpu.py:23-37 (_add_window) visits.  The tile's counts go
+// out with plain stores to the chunk's slice of `partial`.
+__global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t *cnt = (uint32_t *)smem;
+    const int tile_id = blockIdx.x % a.n_tiles;
+    const int chunk = blockIdx.x / a.n_tiles;
+    const int b = blockIdx.y;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ncnt = a.tile * a.nm;
+    for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS)
+        cnt[i] = 0u;
+    __syncthreads();
+    const uint32_t nl = 1u << (2 * a.l);
+    const int wx = a.xlen[b] - a.l + 1;
+    const int i0 = chunk * a.chunk_windows;
+    const int i1 = min(wx, i0 + a.chunk_windows);
+    const int s_base = tile_id * a.tile;
+    const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
+    for (int i = i0 + wave; i < i1; i += INDEX_THREADS / WAVE) {
+        const uint32_t w = a.xwin[(int64_t)b * a.Wx + i];
+        if (w >= nl)
+            continue;  // PAD
+        const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, i) : 1u;
+        for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
+            const int t = t0 + lane;
+            if (t >= a.n_masks)
+                break;
+            const uint32_t v = w ^ a.masks[t];
+            const int lv = a.levels[t];
+            const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
+            const uint32_t e0 = a.offsets[key], e1 = a.offsets[key + 1];
+            for (uint32_t e = e0; e < e1; e++) {
+                const int sid = a.ids[e];
+                if (sid < lo_col)
+                    continue;
+                const uint32_t add = a.vals ? aw * a.vals[e] : 1u;
+                atomicAdd(&cnt[sid * a.nm + lv], add);
+            }
+        }
+    }
+    __syncthreads();
+    const int ns = min(a.tile, a.S - s_base);
+    uint32_t *dst = a.partial + (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
+    for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS)
+        dst[i] = cnt[i];
+}
+
+// hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m].
+__global__ void k_index_reduce(const uint32_t *__restrict__ partial, int64_t n, int nchunks,
+                               unsigned long long *__restrict__ hist)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    unsigned long long v = 0;
+    for (int c = 0; c < nchunks; c++)
+        v += partial[(int64_t)c * n + i];
+    hist[i] += v;
+}
+
+GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xlen, int B, int Wx,
+                           const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                           const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                           int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                           int chunk_windows, int64_t col_min, int64_t row_base, void *partial,
+                           void *hist, void *stream)
+{
+    int rc = check_l_nm(l, nm);
+    if (rc)
+        return rc;
+    if (l > 13)
+        return fail(-1, "gkm_index_hist: the l-mer index needs l <= 13, got %d", l);
+    if (tile < 1 || tile > 65536 || n_tiles != (S + tile - 1) / tile || chunk_windows < 1)
+        return fail(-1, "gkm_index_hist: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", tile,
+                    n_tiles, S, chunk_windows);
+    if (B > 65535)
+        return fail(-1, "gkm_index_hist: at most 65535 rows per call, got %d", B);
+    if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
+        return 0;
+    size_t lds = (size_t)tile * nm * sizeof(uint32_t);
+    if (lds > 160 * 1024)
+        return fail(-2, "gkm_index_hist: a tile of %d SVs x %d levels exceeds LDS", tile, nm);
+    IndexArgs a;
+    a.xwin = xwin;
+    a.xlen = xlen;
+    a.offsets = offsets;
+    a.ids = ids;
+    a.vals = vals;
+    a.masks = masks;
+    a.levels = levels;
+    a.wtab = wtab;
+    a.partial = (uint32_t *)partial;
+    a.B = B;
+    a.Wx = Wx;
+    a.n_masks = n_masks;
+    a.l = l;
+    a.nm = nm;
+    a.S = S;
+    a.tile = tile;
+    a.n_tiles = n_tiles;
+    a.chunk_windows = chunk_windows;
+    a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
+    a.col_min = col_min;
+    a.row_base = row_base;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_index_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
+    dim3 grid((unsigned)(n_tiles * a.nchunks), (unsigned)B);
+    hipLaunchKernelGGL(k_index_tile, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    GKM_CHECK_LAUNCH("gkm_index_hist");
+    int64_t n = (int64_t)B * S * nm;
+    hipLaunchKernelGGL(k_index_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S_(stream),
+                       (const uint32_t *)partial, n, a.nchunks, (unsigned long long *)hist);
+    GKM_CHECK_LAUNCH("gkm_index_hist reduce");
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

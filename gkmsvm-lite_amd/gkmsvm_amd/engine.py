@@ -144,6 +144,37 @@ def launch_shape(B: int, S: int, norient: int, n_cu: int = 256):
     return rows_per_block, nsplit
 
 
+class KernelTimer:
+    """Collects HIP-event timings of gkm_cross_hist launches (on the stream each
+    is launched on) with their algorithmic work: window-pair comparisons."""
+
+    def __init__(self):
+        self.records = []
+
+    def add(self, ev0, ev1, comparisons: int):
+        self.records.append((ev0, ev1, comparisons))
+
+    def summary(self):
+        """(launches, mean launch ms, comparisons per launch)."""
+        if not self.records:
+            return 0, 0.0, 0.0
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b, _ in self.records]
+        comps = [c for _, _, c in self.records]
+        return len(ms), float(np.mean(ms)), float(np.mean(comps))
+
+
+TIMER: KernelTimer | None = None
+
+
+def comparisons(kp: KernelParams, x_lengths, y_lengths, norient: int) -> int:
+    """Window-pair comparisons of rows x against columns y: sum_b sum_s
+    norient * Wx_b * Wy_s (W = windows of a sequence)."""
+    wx = np.clip(np.asarray(x_lengths, dtype=np.int64) - kp.l + 1, 0, None)
+    wy = np.clip(np.asarray(y_lengths, dtype=np.int64) - kp.l + 1, 0, None)
+    return int(norient * wx.sum() * wy.sum())
+
+
 def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeqs,
                ygroups: torch.Tensor, *, col_min: int = -1, out=None, shape=None,
                stream=None) -> torch.Tensor:
@@ -158,11 +189,19 @@ def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeq
     rpb, nsplit = shape if shape is not None else launch_shape(B, S, norient)
     wt, wn = kp.wtab(max(xs.lmax, ys.lmax))
     xpl = xs.planes[x0:x1]
+    timer = TIMER
+    if timer is not None:
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
     _lib.call(
         "gkm_cross_hist", _p(xpl), _p(xs.len[x0:x1]), B, xs.nw, _p(ys.planes),
         _p(ys.planes_rc) if norient == 2 else None, _p(ys.len), _p(ygroups), S, ys.nw, kp.l,
         kp.nm, _p(wt), wn, rpb, nsplit, col_min, x0, _p(out), stream_handle(stream),
     )
+    if timer is not None:
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record(stream)
+        timer.add(ev0, ev1, comparisons(kp, xs.lengths[x0:x1], ys.lengths, norient))
     return out
 
 
@@ -178,6 +217,129 @@ def self_kraw(kp: KernelParams, xs: DeviceSeqs, stream=None) -> torch.Tensor:
               xs.n, xs.nw, kp.l, kp.nm, _p(wt), wn, nchunks, _p(hist), stream_handle(stream))
     out = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
     _lib.call("gkm_combine_kraw", _p(hist), xs.n, kp.nm, _p(kp.tab), _p(out),
+              stream_handle(stream))
+    return out
+
+
+def neighbor_masks(l: int, table, dmax: int):
+    """(masks uint32, levels uint8) of every l-mer XOR pattern with m <= dmax
+    non-zero 2-bit digits whose pair weight table[m] is non-zero, sorted by
+    mask (kernels/lmer_index.py:67-87 enumerates the same set)."""
+    from itertools import combinations, product
+
+    out = []
+    for m in range(dmax + 1):
+        if table[m] == 0:
+            continue
+        for pos in combinations(range(l), m):
+            for digits in product((1, 2, 3), repeat=m):
+                v = 0
+                for p_, d_ in zip(pos, digits):
+                    v |= d_ << (2 * p_)
+                out.append((v, m))
+    out.sort()
+    masks = np.array([v for v, _ in out], dtype=np.uint32)
+    levels = np.array([m for _, m in out], dtype=np.uint8)
+    return masks, levels
+
+
+# Throughput of the two scoring paths on one MI355X (measured, l=11 d=3, 300-1000
+# bp rows, 500-72,000 SVs), for choosing between them: window-pair comparisons
+# per second of gkm_cross_hist; (window, mask, SV tile) probes and index
+# entries per second of gkm_index_hist.
+BRUTE_COMPARISONS_PER_S = 1.8e13
+INDEX_PROBES_PER_S = 6.0e10
+INDEX_ENTRIES_PER_S = 7.7e10
+INDEX_MAX_L = 12
+
+
+class DeviceIndex:
+    """The l-mers of every SV window (forward and RC) as a CSR in HBM, split by
+    SV tile: keys lmer * n_tiles + tile, offsets [4^l n_tiles + 1] uint32, ids
+    uint16 (SV within its tile), vals uint8 window weights (-t 4/-t 5); the
+    neighbourhood masks and their levels.  kernels/lmer_index.py:161-200 builds
+    the same index without the tiles (a tile's counts fit one block's LDS)."""
+
+    def __init__(self, kp: KernelParams, seqs: DeviceSeqs, stream=None, tile: int | None = None):
+        l = kp.l
+        nl = 1 << (2 * l)
+        dev = kp.device
+        S = seqs.n
+        self.tile = tile or index_tile(kp.nm)
+        self.n_tiles = (S + self.tile - 1) // self.tile
+        fwd, rcw = seqs.windows(l, stream)
+        blocks = [fwd] + ([rcw] if kp.rc else [])
+        keys = torch.cat(blocks, dim=1).to(torch.int64) & 0xFFFFFFFF  # [S, nb*W]
+        nw = keys.shape[1]
+        W = fwd.shape[1]
+        flat = keys.reshape(-1)
+        valid = flat < nl
+        sv = torch.arange(S, device=dev, dtype=torch.int64).repeat_interleave(nw)
+        vals = None
+        if kp.weighted:
+            wt, _ = kp.wtab(seqs.lmax)
+            ws = torch.from_numpy(np.clip(seqs.lengths - l + 1, 0, None)).to(dev)[:, None]
+            j = torch.arange(W, device=dev)[None, :]
+            d = [(ws // 2 - j).abs()] + ([(ws // 2 - (ws - 1 - j)).abs()] if kp.rc else [])
+            dist = torch.cat(d, dim=1).reshape(-1).clamp(0, wt.numel() - 1)
+            vals = wt[dist][valid]
+        flat = flat[valid]
+        sv = sv[valid]
+        key = flat * self.n_tiles + sv // self.tile
+        order = torch.argsort(key, stable=True)
+        self.ids = (sv[order] % self.tile).to(torch.int16).contiguous()  # uint16 bits
+        self.vals = None if vals is None else vals[order].contiguous()
+        counts = torch.bincount(key, minlength=nl * self.n_tiles)
+        offsets = torch.zeros(nl * self.n_tiles + 1, dtype=torch.int64, device=dev)
+        offsets[1:] = torch.cumsum(counts, 0)
+        self.n_entries = int(offsets[-1].item())
+        if self.n_entries >= 2**31:
+            raise MemoryError("l-mer index over 2^31 entries")
+        self.offsets = offsets.to(torch.int32)  # uint32 bits
+        masks, levels = neighbor_masks(l, kp.h, kp.dmax)
+        self.masks = torch.from_numpy(masks.view(np.int32)).to(dev)
+        self.levels = torch.from_numpy(levels).to(dev)
+        self.n_masks = len(masks)
+        self.per_lmer = self.n_entries / nl
+        self.max_weight = int(kp.base.weight_table(1)[0]) if kp.weighted else 1
+        self.max_sv_windows = int(np.clip(seqs.lengths - l + 1, 0, None).max()) * len(blocks)
+        del keys, flat, sv, order, counts, key
+
+    def nbytes(self) -> int:
+        t = [self.ids, self.offsets, self.masks, self.levels]
+        if self.vals is not None:
+            t.append(self.vals)
+        return sum(x.numel() * x.element_size() for x in t)
+
+
+def index_tile(nm: int) -> int:
+    """SVs per index tile: the tile's [tile, nm] uint32 counts take 128 KiB of
+    LDS (the index is probe-bound, so the largest tile that fits wins)."""
+    return max(256, (32768 // nm) // 256 * 256)
+
+
+def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: int, S: int, *,
+               col_min: int = -1, out=None, stream=None, target_blocks: int = 2048) -> torch.Tensor:
+    """[x1 - x0, S, nm] int64 histograms of rows x0..x1 through the index."""
+    B = x1 - x0
+    if out is None:
+        out = torch.zeros((B, S, kp.nm), dtype=torch.int64, device=kp.device)
+    else:
+        _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
+    fwd, _ = xs.windows(kp.l, stream)
+    Wx = fwd.shape[1]
+    wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
+    nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)), -(-wx_max // 16)))
+    cw = -(-wx_max // nchunks)
+    if idx.vals is not None:  # per-block uint32 counts must not overflow
+        cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
+        cw = max(1, min(cw, cap))
+    nchunks = -(-Wx // cw)
+    partial = torch.empty((nchunks, B, S, kp.nm), dtype=torch.int32, device=kp.device)
+    wt, _ = kp.wtab(xs.lmax)
+    _lib.call("gkm_index_hist", _p(fwd[x0:x1]), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
+              _p(idx.ids), _p(idx.vals), _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l, kp.nm,
+              _p(wt), S, idx.tile, idx.n_tiles, cw, col_min, x0, _p(partial), _p(out),
               stream_handle(stream))
     return out
 
@@ -201,6 +363,36 @@ class SVModel:
         else:
             self.ds = torch.ones(self.S, dtype=torch.float64, device=kp.device)
 
+        self._index = None
+
+    def index(self):
+        """The l-mer index of the SVs (built on first use), or None when l is
+        too large for 4^l offsets."""
+        if self._index is None and self.kp.l <= INDEX_MAX_L and self.kp.dmax >= 0:
+            self._index = DeviceIndex(self.kp, self.seqs)
+        return self._index
+
+    def use_index_for(self, x_lengths, mode="auto") -> bool:
+        """Whether scoring rows of these lengths goes through the index: forced
+        by mode True/False, else the cheaper of the two paths by the cost model."""
+        if mode is False or self.kp.l > INDEX_MAX_L or self.kp.dmax < 0:
+            return False
+        if mode is True:
+            return True
+        from math import comb
+
+        n_masks = sum(comb(self.kp.l, m) * 3**m for m in range(self.kp.dmax + 1)
+                      if self.kp.h[m] != 0)
+        norient = 2 if self.kp.rc else 1
+        ws = np.clip(self.seqs.lengths - self.kp.l + 1, 0, None)
+        per_lmer = norient * ws.sum() / 4**self.kp.l
+        wx = np.clip(np.asarray(x_lengths) - self.kp.l + 1, 0, None).sum()
+        probes = float(wx) * n_masks
+        n_tiles = -(-self.S // index_tile(self.kp.nm))
+        index_s = probes * n_tiles / INDEX_PROBES_PER_S + probes * per_lmer / INDEX_ENTRIES_PER_S
+        brute_s = float(wx) * norient * ws.sum() / BRUTE_COMPARISONS_PER_S
+        return index_s < brute_s
+
     def diag_original_order(self) -> torch.Tensor:
         inv = np.empty_like(self.order)
         inv[self.order] = np.arange(len(self.order))
@@ -217,8 +409,17 @@ def rows_per_chunk(S: int, nm: int, cap: int = MAX_HIST_BYTES) -> int:
     return max(1, cap // max(1, S * nm * 8))
 
 
+def sv_hist(sv: SVModel, xs: DeviceSeqs, r0: int, r1: int, *, out=None, stream=None,
+            shape=None, use_index="auto") -> torch.Tensor:
+    """[r1 - r0, S, nm] histograms of rows r0..r1 against the SVs, through the
+    l-mer index or the brute-force kernel (identical integers either way)."""
+    if sv.use_index_for(xs.lengths[r0:r1], use_index):
+        return index_hist(sv.kp, sv.index(), xs, r0, r1, sv.S, out=out, stream=stream)
+    return cross_hist(sv.kp, xs, r0, r1, sv.seqs, sv.groups, out=out, stream=stream, shape=shape)
+
+
 def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stream=None,
-               shape=None):
+               shape=None, use_index="auto"):
     """out[b] = sum_s coef_s F(K(x_b, s)) + bias for every row of xs."""
     kp = sv.kp
     dx = self_kraw(kp, xs, stream) if kp.normalize else None
@@ -228,7 +429,7 @@ def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stre
         r1 = min(xs.n, r0 + step)
         if hist is None or hist.shape[0] != r1 - r0:
             hist = torch.empty((r1 - r0, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
-        cross_hist(kp, xs, r0, r1, sv.seqs, sv.groups, out=hist, stream=stream, shape=shape)
+        sv_hist(sv, xs, r0, r1, out=hist, stream=stream, shape=shape, use_index=use_index)
         _lib.call("gkm_combine_score", _p(hist), r1 - r0, sv.S, kp.nm, _p(kp.tab),
                   _p(dx[r0:r1]) if dx is not None else None, _p(sv.ds), _p(sv.coef),
                   int(kp.normalize), int(kp.rbf), kp.gamma, float(bias), _p(out[r0:r1]),

@@ -113,7 +113,7 @@ def _explain_rows(model, codes, lens, resolved, out):
                                                             device=kp.device)
     hist = None
     if kp.rbf:
-        hist = E.cross_hist(kp, xs, 0, B, sv.seqs, sv.groups)
+        hist = E.sv_hist(sv, xs, 0, B, use_index=model.use_lmer_index)
     ec = torch.empty((B, sv.S), dtype=torch.float64, device=kp.device)
     _lib.call("gkm_explain_coef", E._p(hist), B, sv.S, kp.nm, E._p(kp.tab), E._p(dx),
               E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma, E._p(ec), st)
@@ -161,7 +161,7 @@ def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
             dref = E.self_kraw(kp, rs)
         else:
             dref = torch.ones(n, dtype=torch.float64, device=dev)
-        hist = E.cross_hist(kp, xs, 0, 2 * n, sv.seqs, sv.groups)
+        hist = E.sv_hist(sv, xs, 0, 2 * n, use_index=model.use_lmer_index)
         _lib.call("gkm_combine_impact", E._p(hist), n, sv.S, kp.nm, E._p(kp.tab), E._p(dref),
                   E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma,
                   E._p(out[s:e]), st)

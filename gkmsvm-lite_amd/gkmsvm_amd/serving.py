@@ -1,0 +1,255 @@
+"""A continuous-batching request loop over one resident model.
+
+The model's SV tables, coefficients and self-kernels stay in HBM.  Clients
+submit scoring, variant (score_variants, kernel method) and GkmExplain
+requests from any thread and get a Future.  One worker thread admits pending
+requests into GPU steps while earlier steps run: each step packs every
+admitted scoring/variant row into one batch (rows of any length: each row's
+diagonal walk uses its own length) and launches the cross kernel once for all
+of them on its own HIP stream; explain requests follow in the same step.  Up
+to ``streams`` steps are in flight, so a request that arrives while others
+are on the GPU is admitted as soon as a stream frees up and its work overlaps
+theirs.  Admission is by memory: a step takes requests while the bytes it
+would hold (histograms, planes, outputs) stay within ``hbm_budget``; a
+request larger than the whole budget is rejected with MemoryError.
+
+Each row's result depends only on that row (the per-row reduction runs in a
+fixed order), so a request scores identically alone and in any mix.
+"""
+
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import engine as E
+from .codec import encode_codes
+from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
+
+
+@dataclass
+class Request:
+    kind: str                 # "score" | "variant" | "explain"
+    seqs: list                # score/explain: [str]; variant: [(ref, alt)]
+    mode: str = "hypothetical"
+    future: Future = field(default_factory=Future)
+    t_submit: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+
+    def rows(self) -> int:
+        return 2 * len(self.seqs) if self.kind == "variant" else len(self.seqs)
+
+    def max_len(self) -> int:
+        if self.kind == "variant":
+            return max(max(len(r), len(a)) for r, a in self.seqs)
+        return max(len(s) for s in self.seqs)
+
+
+@dataclass
+class _Step:
+    reqs: list
+    stream: torch.cuda.Stream
+    done: torch.cuda.Event
+    score_host: torch.Tensor | None
+    spans: list               # (request, start row, n rows) for scoring/variant rows
+    explain_out: list         # (request, host tensor [n, 4, L])
+    keep: list                # device tensors kept alive until the step completes
+
+
+class GkmServer:
+    """Serve one model; see the module docstring."""
+
+    def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
+                 streams: int = 2, max_explain: int = 8):
+        model.rocm()
+        self.model = model
+        self.sv = model._svs()
+        self.kp = model._params()
+        self.hbm_budget = int(hbm_budget)
+        self.max_rows = int(max_rows)
+        self.max_explain = int(max_explain)
+        self._streams = [torch.cuda.Stream(device=self.kp.device) for _ in range(streams)]
+        self._free = list(self._streams)
+        self._pending: deque = deque()
+        self._inflight: list = []
+        self._lock = threading.Condition()
+        self._stop = False
+        self._thread = None
+        self.completed: list = []
+
+    # -- client side ----------------------------------------------------------------
+
+    def submit(self, kind: str, seqs, mode: str = "hypothetical") -> Future:
+        if kind not in ("score", "variant", "explain"):
+            raise ValueError(f"unknown request kind {kind!r}")
+        req = Request(kind, list(seqs), mode)
+        req.t_submit = time.perf_counter()
+        need = self._bytes(req)
+        if need > self.hbm_budget:
+            req.future.set_exception(MemoryError(
+                f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))
+            return req.future
+        with self._lock:
+            self._pending.append(req)
+            self._lock.notify()
+        return req.future
+
+    def start(self):
+        if self._thread is None:
+            self._stop = False
+            self._thread = threading.Thread(target=self._loop, name="gkm-server", daemon=True)
+            self._thread.start()
+        return self
+
+    def stop(self):
+        with self._lock:
+            self._stop = True
+            self._lock.notify()
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    # -- admission ----------------------------------------------------------------------
+
+    def _bytes(self, req: Request) -> int:
+        S, nm = self.sv.S, self.kp.nm
+        L = req.max_len()
+        planes = 2 * (L // 32 + 4) * 8 + L
+        if req.kind == "explain":
+            return len(req.seqs) * (S * (8 + nm * 8) + 4 * L * 8 * 2 + planes)
+        return req.rows() * (S * nm * 8 + planes + 16)
+
+    def _admit(self):
+        """Pop the requests of the next step (at least one if any is pending)."""
+        batch, used, rows, n_explain = [], 0, 0, 0
+        while self._pending:
+            req = self._pending[0]
+            need = self._bytes(req)
+            is_ex = req.kind == "explain"
+            if batch and (used + need > self.hbm_budget
+                          or (not is_ex and rows + req.rows() > self.max_rows)
+                          or (is_ex and n_explain >= self.max_explain)):
+                break
+            self._pending.popleft()
+            batch.append(req)
+            used += need
+            if is_ex:
+                n_explain += 1
+            else:
+                rows += req.rows()
+        return batch
+
+    # -- the worker ---------------------------------------------------------------------
+
+    def _loop(self):
+        torch.cuda.set_device(self.kp.device)
+        while True:
+            self._harvest()
+            with self._lock:
+                if self._stop and not self._pending and not self._inflight:
+                    return
+                batch = self._admit() if (self._pending and self._free) else []
+                if not batch:
+                    if self._inflight:
+                        self._lock.wait(timeout=5e-5)
+                    else:
+                        self._lock.wait(timeout=1e-3)
+                    continue
+            stream = self._free.pop()
+            try:
+                self._inflight.append(self._launch(batch, stream))
+            except Exception as exc:  # surface launch errors on the requests
+                self._free.append(stream)
+                for r in batch:
+                    if not r.future.done():
+                        r.future.set_exception(exc)
+
+    def _harvest(self):
+        still = []
+        for st in self._inflight:
+            if not st.done.query():
+                still.append(st)
+                continue
+            now = time.perf_counter()
+            for req, r0, n in st.spans:
+                vals = st.score_host[r0:r0 + n].numpy()
+                if req.kind == "variant":
+                    half = n // 2
+                    res = vals[half:] - vals[:half]  # alt - ref
+                else:
+                    res = vals.copy()
+                req.t_done = now
+                req.future.set_result(res)
+                self.completed.append(req)
+            for req, host in st.explain_out:
+                req.t_done = now
+                req.future.set_result(host.numpy())
+                self.completed.append(req)
+            self._free.append(st.stream)
+        self._inflight = still
+
+    def _launch(self, batch, stream) -> _Step:
+        kp, sv, model = self.kp, self.sv, self.model
+        t = time.perf_counter()
+        spans, codes_rows, keep = [], [], []
+        for req in batch:
+            req.t_admit = t
+            if req.kind == "score":
+                rows = [encode_codes(s, allow_n=True) for s in req.seqs]
+            elif req.kind == "variant":
+                rows = ([encode_codes(r, allow_n=True) for r, _ in req.seqs]
+                        + [encode_codes(a, allow_n=True) for _, a in req.seqs])
+            else:
+                continue
+            spans.append((req, len(codes_rows), len(rows)))
+            codes_rows.extend(rows)
+        score_host = None
+        explains = [r for r in batch if r.kind == "explain"]
+        with torch.cuda.stream(stream):
+            if codes_rows:
+                lens = np.array([len(r) for r in codes_rows], dtype=np.int64)
+                codes = np.zeros((len(codes_rows), max(int(lens.max()), kp.l)), dtype=np.uint8)
+                for i, r in enumerate(codes_rows):
+                    codes[i, : len(r)] = r
+                xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+                out = torch.empty(len(codes_rows), dtype=torch.float64, device=kp.device)
+                E.score_rows(sv, xs, model.bias, out)
+                short = torch.from_numpy(lens < kp.l).to(kp.device)
+                out = torch.where(short, torch.full_like(out, float("nan")), out)
+                score_host = torch.empty(out.shape, dtype=torch.float64, pin_memory=True)
+                score_host.copy_(out, non_blocking=True)
+                keep += [xs, out]
+            ex_out = []
+            for req in explains:
+                from .explain import gkmexplain
+
+                res = gkmexplain(model, req.seqs, mode=req.mode, batch_size=len(req.seqs))
+                host = torch.empty(res.shape, dtype=torch.float64, pin_memory=True)
+                host.copy_(res, non_blocking=True)
+                keep.append(res)
+                ex_out.append((req, host))
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return _Step(batch, stream, ev, score_host, spans, ex_out, keep)
+
+
+def latency_stats(reqs) -> dict:
+    """p50 / p99 / mean request latency (submit to result) in milliseconds."""
+    lat = np.array([(r.t_done - r.t_submit) * 1e3 for r in reqs])
+    if lat.size == 0:
+        return {"p50_ms": None, "p99_ms": None, "mean_ms": None, "n": 0}
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "n": int(lat.size)}

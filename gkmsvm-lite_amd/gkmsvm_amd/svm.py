@@ -64,6 +64,9 @@ class GkmSVM:
         self._kernel_params = dict(kernel_params)
         self.kernel = build_kernel(self.kernel_type, kernel_params)
         self.sv_chunk_size = sv_chunk_size
+        # Score through the l-mer index of the SVs: "auto" picks the cheaper of
+        # the index and the brute-force kernel per call; True/False force it
+        self.use_lmer_index = "auto"
         self._device = None
         self._kp = None
         self._sv = None
@@ -136,7 +139,7 @@ class GkmSVM:
         sv = self._svs()
         xs = E.DeviceSeqs(codes, lengths, self._device, rc=self._params().rc)
         out = torch.empty(codes.shape[0], dtype=torch.float64, device=self._device)
-        return E.score_rows(sv, xs, self.bias, out)
+        return E.score_rows(sv, xs, self.bias, out, use_index=self.use_lmer_index)
 
     def __call__(self, x, *, lengths=None, batch_size: int | None = None,
                  verbose: bool = False) -> torch.Tensor:

@@ -291,3 +291,49 @@ def test_short_and_n_rows(gpu):
     om = _oracle_model(m)
     codes, lens = O.batch(seqs)
     H.assert_close(got, O.score(om, codes, lens), 1e-10, "N rows")
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_index_histograms_equal_brute_force(gpu, seed):
+    """The l-mer index path gives the brute-force kernel's integers exactly
+    (random l, kernel types incl. -t 0 without d and weighted, ragged SVs,
+    RC on/off), and model scores agree bit for bit between the two paths."""
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(300 + seed)
+    l = int(rng.choice([5, 8, 10, 11, 12]))
+    k = max(1, l - int(rng.integers(2, 5)))
+    kt = int(rng.choice([0, 2, 4, 5]))
+    p = _params(kt, l, k, rng)
+    print(f"[case] seed={seed} l={l} k={k} kt={kt} p={p}", flush=True)
+    yc, yl = _rand_codes(rng, int(rng.integers(1, 300)), l, 400)
+    m = _model(kt, p, yc, yl, rng.normal(size=len(yl)), 0.1)
+    sv = m._svs()
+    xc, xl = _rand_codes(rng, int(rng.integers(1, 9)), 1, 300)
+    if xc.shape[1] < l:
+        pytest.skip("no windows")
+    xs = E.DeviceSeqs(xc, xl, gpu, rc=sv.kp.rc)
+    a = E.index_hist(sv.kp, sv.index(), xs, 0, xs.n, sv.S)
+    b = E.cross_hist(sv.kp, xs, 0, xs.n, sv.seqs, sv.groups)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    m.use_lmer_index = True
+    s_idx = m(xc, lengths=xl).cpu().numpy()
+    m.use_lmer_index = False
+    s_bru = m(xc, lengths=xl).cpu().numpy()
+    np.testing.assert_array_equal(s_idx, s_bru)
+
+
+def test_index_plain_config(gpu):
+    """Plain -t 2 scoring through the l-mer index equals the oracle's index path."""
+    rng = np.random.default_rng(1)
+    svc = rng.integers(0, 4, (500, 300)).astype(np.uint8)
+    coef = np.abs(np.random.default_rng(2).normal(size=500))
+    coef[250:] *= -1
+    q = np.random.default_rng(3).integers(0, 4, (200, 300)).astype(np.uint8)
+    p = {"L": 11, "k": 7, "d": 3}
+    m = _model(2, p, svc, np.full(500, 300), coef)
+    m.use_lmer_index = True
+    got = m(q).cpu().numpy().ravel()
+    om = O.Model(O.Kern.make(2, p), svc, np.full(500, 300), coef, 0.0)
+    H.assert_close(got, O.LmerIndex(om).score(q, np.full(200, 300)), 1e-10, "index C1")
