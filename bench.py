@@ -14,10 +14,14 @@ latencies from arrival to result.  The plain path -- the same requests, one
 API call each (model(x), model.score_variants, gkmexplain) -- is timed
 beside it.
 
-roofline: the dominant kernel (gkm_cross_hist), timed with HIP events on its
-own stream during the timed region; achieved = window-pair comparisons x 6
+roofline: every launch of the hot kernels (k_cross_hist, k_index_tile,
+k_sweep) is timed with HIP events on the stream it runs on during the timed
+steps; the object describes the kernel with the most GPU time (the others
+under "others").  Brute-force kernels: achieved = window-pair comparisons x 6
 integer operations (XOR, shift, OR, AND, popcount, accumulate: the direct
 method's count per comparison) / launch time, against the int32 VALU peak.
+The l-mer index: algorithmic bytes (8 per probe, 2-3 per entry) / launch
+time, against HBM.
 cpu_baseline: the reference's CPU scoring algorithm (l-mer index + fused
 reduce), restated in C with OpenMP (the oracle), on a bounded sample of the
 stream's scoring queries.
@@ -161,6 +165,71 @@ def cpu_baseline(sv, coef, reqs, seconds):
     }
 
 
+HBM_PEAK = 8.0e12  # bytes/s
+
+
+def roofline_of(summary: dict) -> dict:
+    """The roofline object of the kernel with the most GPU time; the others as
+    "others"."""
+    if not summary:
+        return {}
+    rows = {}
+    for kind, d in summary.items():
+        per_s = d["units_per_launch"] / (d["mean_ms"] * 1e-3)
+        if kind == "k_index_tile":
+            r = {"bound": "hbm", "achieved": per_s, "peak": HBM_PEAK, "unit": "B/s",
+                 "frac": per_s / HBM_PEAK,
+                 "work": "8 B offsets per (window, mask, SV tile) probe + 2 B id per entry"}
+        else:
+            ops = per_s * OPS_PER_COMPARISON
+            r = {"bound": "valu-int32", "achieved": ops, "peak": INT_VALU_PEAK,
+                 "unit": "int32 op/s", "frac": ops / INT_VALU_PEAK,
+                 "comparisons_per_s": per_s,
+                 "work": "6 int ops per window-pair comparison (XOR, SHR, OR, AND, POPC, add)"}
+        r.update({"kernel": kind, "launches": d["launches"], "mean_launch_ms": d["mean_ms"],
+                  "total_ms": d["total_ms"], "traffic": None})
+        rows[kind] = r
+    top = max(rows.values(), key=lambda r: r["total_ms"])
+    out = dict(top)
+    out["others"] = [r for k, r in rows.items() if k != top["kernel"]]
+    return out
+
+
+def kernel_only(args):
+    """Run one hot kernel on the serving shape (16 x 1 kb queries against the
+    72,000 resident SVs) --launches times: for rocprofv3 profiles."""
+    import torch
+
+    from gkmsvm_amd import engine as E
+
+    if args.mode == "copy":
+        for _ in range(args.launches):
+            src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+            torch.mul(src, 2.0)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
+        return
+    model, _, _ = make_model()
+    sv = model._svs()
+    kp = model._params()
+    rng = np.random.default_rng(7)
+    q = rng.integers(0, 4, (16, 1000)).astype(np.uint8)
+    xs = E.DeviceSeqs(q, np.full(16, 1000), kp.device)
+    out = torch.empty((16, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
+    if args.mode == "index":
+        sv.index()
+    for _ in range(args.launches):
+        if args.mode == "index":
+            E.index_hist(kp, sv.index(), xs, 0, 16, sv.S, out=out)
+        elif args.mode == "cross":
+            E.cross_hist(kp, xs, 0, 16, sv.seqs, sv.groups, out=out)
+        else:  # an elementwise pass over 256 MiB (reads 256 MiB, writes 256 MiB)
+            src = torch.ones(32 << 20, dtype=torch.float64, device=kp.device)
+            torch.mul(src, 2.0)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,11 +244,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--mode", default="serve", choices=["serve", "cross", "index", "copy"],
+                    help="serve: the benchmark; cross/index/copy: one kernel, for profiling")
+    ap.add_argument("--launches", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine's multi-GPU sharding is a design "
               "note, not part of this build (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode != "serve":
+        return kernel_only(args)
 
     import torch
 
@@ -230,20 +305,7 @@ def main():
         plain_s = time.perf_counter() - t
         plain = {"value": n_evals / plain_s, "ms_per_step": plain_s / len(timed) * 1e3}
 
-    launches, ms, comps = timer.summary()
-    achieved = comps * OPS_PER_COMPARISON / (ms * 1e-3) if ms else 0.0
-    roofline = {
-        "bound": "valu-int32",
-        "kernel": "k_cross_hist",
-        "achieved": achieved,
-        "peak": INT_VALU_PEAK,
-        "unit": "int32 op/s (6 per window-pair comparison)",
-        "frac": achieved / INT_VALU_PEAK,
-        "comparisons_per_s": comps / (ms * 1e-3) if ms else 0.0,
-        "launches": launches,
-        "mean_launch_ms": ms,
-        "traffic": None,
-    }
+    roofline = roofline_of(timer.summary())
     line = {
         "metric": "query x SV kernel evals/sec (served), with p50/p99 request latency and the plain path",
         "value": served_rate,

@@ -145,26 +145,46 @@ def launch_shape(B: int, S: int, norient: int, n_cu: int = 256):
 
 
 class KernelTimer:
-    """Collects HIP-event timings of gkm_cross_hist launches (on the stream each
-    is launched on) with their algorithmic work: window-pair comparisons."""
+    """HIP-event timings of the hot kernels' launches (events recorded on the
+    stream each launch goes to) with each launch's algorithmic work: window-pair
+    comparisons (k_cross_hist, k_sweep) or bytes (k_index_tile)."""
 
     def __init__(self):
         self.records = []
 
-    def add(self, ev0, ev1, comparisons: int):
-        self.records.append((ev0, ev1, comparisons))
+    def add(self, kind: str, ev0, ev1, units: float):
+        self.records.append((kind, ev0, ev1, units))
 
-    def summary(self):
-        """(launches, mean launch ms, comparisons per launch)."""
-        if not self.records:
-            return 0, 0.0, 0.0
+    def summary(self) -> dict:
+        """kind -> {launches, total_ms, mean_ms, units_per_launch}."""
         torch.cuda.synchronize()
-        ms = [a.elapsed_time(b) for a, b, _ in self.records]
-        comps = [c for _, _, c in self.records]
-        return len(ms), float(np.mean(ms)), float(np.mean(comps))
+        out = {}
+        for kind, a, b, u in self.records:
+            d = out.setdefault(kind, {"launches": 0, "total_ms": 0.0, "units": 0.0})
+            d["launches"] += 1
+            d["total_ms"] += a.elapsed_time(b)
+            d["units"] += u
+        for d in out.values():
+            d["mean_ms"] = d["total_ms"] / d["launches"]
+            d["units_per_launch"] = d.pop("units") / d["launches"]
+        return out
 
 
 TIMER: KernelTimer | None = None
+
+
+def _timed(kind: str, units_fn, stream, launch):
+    """Run launch(); with TIMER set, bracket it with events on the stream."""
+    timer = TIMER
+    if timer is None:
+        return launch()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    r = launch()
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev1.record(stream)
+    timer.add(kind, ev0, ev1, units_fn())
+    return r
 
 
 def comparisons(kp: KernelParams, x_lengths, y_lengths, norient: int) -> int:
@@ -189,19 +209,12 @@ def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeq
     rpb, nsplit = shape if shape is not None else launch_shape(B, S, norient)
     wt, wn = kp.wtab(max(xs.lmax, ys.lmax))
     xpl = xs.planes[x0:x1]
-    timer = TIMER
-    if timer is not None:
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-    _lib.call(
-        "gkm_cross_hist", _p(xpl), _p(xs.len[x0:x1]), B, xs.nw, _p(ys.planes),
-        _p(ys.planes_rc) if norient == 2 else None, _p(ys.len), _p(ygroups), S, ys.nw, kp.l,
-        kp.nm, _p(wt), wn, rpb, nsplit, col_min, x0, _p(out), stream_handle(stream),
-    )
-    if timer is not None:
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev1.record(stream)
-        timer.add(ev0, ev1, comparisons(kp, xs.lengths[x0:x1], ys.lengths, norient))
+    _timed("k_cross_hist", lambda: comparisons(kp, xs.lengths[x0:x1], ys.lengths, norient),
+           stream, lambda: _lib.call(
+               "gkm_cross_hist", _p(xpl), _p(xs.len[x0:x1]), B, xs.nw, _p(ys.planes),
+               _p(ys.planes_rc) if norient == 2 else None, _p(ys.len), _p(ygroups), S, ys.nw,
+               kp.l, kp.nm, _p(wt), wn, rpb, nsplit, col_min, x0, _p(out),
+               stream_handle(stream)))
     return out
 
 
@@ -337,11 +350,22 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     nchunks = -(-Wx // cw)
     partial = torch.empty((nchunks, B, S, kp.nm), dtype=torch.int32, device=kp.device)
     wt, _ = kp.wtab(xs.lmax)
-    _lib.call("gkm_index_hist", _p(fwd[x0:x1]), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
-              _p(idx.ids), _p(idx.vals), _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l, kp.nm,
-              _p(wt), S, idx.tile, idx.n_tiles, cw, col_min, x0, _p(partial), _p(out),
-              stream_handle(stream))
+    _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
+           lambda: _lib.call(
+               "gkm_index_hist", _p(fwd[x0:x1]), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
+               _p(idx.ids), _p(idx.vals), _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l,
+               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, col_min, x0, _p(partial), _p(out),
+               stream_handle(stream)))
     return out
+
+
+def index_bytes(kp: KernelParams, idx: DeviceIndex, x_lengths) -> float:
+    """Algorithmic bytes of an index query: per (window, mask, tile) probe the
+    8-byte offsets pair, per entry its 2-byte SV id (+1 byte weight)."""
+    wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
+    probes = wx * idx.n_masks
+    per_entry = 3.0 if idx.vals is not None else 2.0
+    return probes * idx.n_tiles * 8.0 + probes * idx.per_lmer * per_entry
 
 
 class SVModel:

@@ -122,10 +122,13 @@ def _explain_rows(model, codes, lens, resolved, out):
     nm = cmax + 1
     wt, wn = kp.wtab(max(xs.lmax, sv.seqs.lmax))
     buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
-    _lib.call("gkm_explain", int(pert), E._p(xs.planes), E._p(xs.len), xs.lmax, B, xs.nw,
-              E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
-              E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
-              _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st)
+    norient = 2 if kp.rc else 1
+    E._timed("k_sweep", lambda: E.comparisons(kp, xs.lengths, sv.seqs.lengths, norient), None,
+             lambda: _lib.call(
+                 "gkm_explain", int(pert), E._p(xs.planes), E._p(xs.len), xs.lmax, B, xs.nw,
+                 E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
+                 E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
+                 _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
     out += buf
 
 

@@ -9,7 +9,7 @@
  * message in gkm_last_error() (thread-local).  No call allocates device
  * memory.
  *
- * Data layout (see DESIGN.md for the full description):
+ * Data layout:
  *   planes / planes_rc  [n][nw] pairs of uint32 (bit-plane 0, bit-plane 1):
  *                       base t of a sequence (or of its reverse complement)
  *                       at bit t & 31 of word (t >> 5) + 1, word 0 zero;

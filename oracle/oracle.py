@@ -1,6 +1,6 @@
 """CPU oracle for the gapped k-mer hot path -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+Only ``tests/``, the smoke check and the CPU-baseline leg of
 ``bench.py`` may import this module, and only as the checker (or as the timed
 CPU baseline); the product never calls it.
 
