@@ -452,4 +452,125 @@ GKM_HD void sweep_lane(XW xw, YW yw, int lx, int ly, int l, int nm, int cmax, in
         });
 }
 
+
+// ---------------------------------------------------------------------------
+// Hit records: a window pair with few enough mismatches, reduced to what its
+// l positions need (used by the GkmExplain / ISM sweep, whose lanes share the
+// per-position work of the hits found by any lane of the wave).
+// ---------------------------------------------------------------------------
+
+struct HitRec {
+    int tx;         // x window start (positions tx .. tx + l - 1)
+    int c;          // mismatch count of the window pair
+    uint32_t mis;   // bit q: position q of the windows differs
+    uint32_t y0;    // bit q: bit 0 of the SV's base at position q
+    uint32_t y1;    // bit q: bit 1 of the SV's base at position q
+    uint32_t w;     // wx * wy (1 without window weights)
+};
+
+// The record of hit window j (bit j of the word at y position base) on
+// diagonal delta.
+GKM_HD HitRec make_hit(int l, int j, int base, int delta, int rc, uint32_t ep, uint32_t ec,
+                       P2 yp, P2 yc, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
+                       const uint8_t *wt, int wx, int wy)
+{
+    const uint32_t lm = (1u << l) - 1u;
+    HitRec h;
+    const int ty = base + j;
+    h.tx = ty - delta;
+    h.c = count_at(b0, b1, b2, b3, j);
+    h.mis = funnel(ec, ep, (uint32_t)j) & lm;
+    h.y0 = funnel(yc.x, yp.x, (uint32_t)j) & lm;
+    h.y1 = funnel(yc.y, yp.y, (uint32_t)j) & lm;
+    h.w = wt ? window_weight(wt, wx, h.tx) * window_weight(wt, wy, rc ? (wy - 1 - ty) : ty) : 1u;
+    return h;
+}
+
+// Position q of a hit: explain adds ew x (mismatch ? tb[c] : ta[c]) to the SV's
+// base (hypothetical) or, at a matching position, ew ta[c] to the three other
+// bases (perturbation); ISM adds w (h[c+1] - h[c]) to dm (match) or
+// w (h[c-1] - h[c]) to da[SV base] (mismatch); ta = h for ISM.  add(kind,
+// base, pos, value): kind 0 = res, 1 = dm, 2 = da.
+template <int MODE, class ADD>
+GKM_HD void apply_hit(const HitRec &h, int q, double ecs, const double *ta, const double *tb,
+                      ADD &&add)
+{
+    const int pos = h.tx + q;
+    const bool mis = (h.mis >> q) & 1u;
+    const int yb = (int)(((h.y0 >> q) & 1u) | (((h.y1 >> q) & 1u) << 1));
+    const int c = h.c;
+    if (MODE == SW_ISM) {
+        const double w = (double)h.w;
+        if (mis) {
+            if (c > 0) {
+                const double v = w * (ta[c - 1] - ta[c]);
+                if (v != 0.0)
+                    add(2, yb, pos, v);
+            }
+        } else {
+            const double v = w * (ta[c + 1] - ta[c]);
+            if (v != 0.0)
+                add(1, 0, pos, v);
+        }
+        return;
+    }
+    // the reference multiplies ec x wx x wy (explain/cpu.py:49)
+    const double ew = h.w == 1u ? ecs : ecs * (double)h.w;
+    if (MODE == SW_HYP) {
+        add(0, yb, pos, ew * (mis ? tb[c] : ta[c]));
+    } else if (mis) {
+        add(0, yb, pos, ew * tb[c]);
+    } else {
+        const double av = ew * ta[c];
+        if (av != 0.0)
+            for (int b = 0; b < 4; b++)
+                if (b != yb)
+                    add(0, b, pos, av);
+    }
+}
+
+// The walk of one lane's diagonal inside a wave whose lanes take consecutive
+// diagonals: the loop runs over the wave's common word range [U0, U1] so all
+// lanes stay converged (lanes outside their own range [u0, u1] get an empty
+// valid mask and read clamped, in-bounds words: nwx / nwy are the rows' word
+// counts).  Words are loaded fresh each step (no carried funnel state), so a
+// lane's words are exact from its first word on.  f(base, active, ep, ec, yp,
+// yc, b0, b1, b2, b3, valid).
+template <int LT, class XW, class YW, class F>
+GKM_HD void walk_converged(XW xw, YW yw, int nwx, int nwy, int l, int delta, bool lane_ok,
+                           int ty_lo, int ty_hi, int U0, int U1, F &&f)
+{
+    const uint32_t sh = (uint32_t)(-delta) & 31u;
+    const int u0 = ty_lo >> 5, u1 = ty_hi >> 5;
+    auto xk = [&](int u) {  // clamped x word index for y word u
+        int k = ((32 * u - delta) >> 5) + 1;
+        return k < 0 ? 0 : (k > nwx - 2 ? nwx - 2 : k);
+    };
+    auto yk = [&](int a) { return a < 0 ? 0 : (a > nwy - 1 ? nwy - 1 : a); };
+    for (int u = U0; u <= U1; u++) {
+        const int ka = xk(u), kb = xk(u + 1);
+        const P2 yp = yw(yk(u + 1)), yc = yw(yk(u + 2));
+        const uint32_t ep = mism_word(xw(ka), xw(ka + 1), yp, sh);
+        const uint32_t ec = mism_word(xw(kb), xw(kb + 1), yc, sh);
+        uint32_t b0, b1, b2, b3;
+        counts<LT>(ep, ec, l, b0, b1, b2, b3);
+        const bool active = lane_ok && u >= u0 && u <= u1;
+        const int base = 32 * u;
+        const uint32_t valid = active ? range_mask(ty_lo - base, ty_hi - base) : 0u;
+        f(base, active, ep, ec, yp, yc, b0, b1, b2, b3, valid);
+    }
+}
+
+
+// Word range [U0, U1] of a wave whose lanes take diagonals d0 .. d0 + 63 of an
+// (x, y) pair (d = delta + lx - l, nd diagonals): u0 and u1 grow with delta,
+// so the first and last valid diagonals bound the range.
+GKM_HD void wave_word_range(int lx, int ly, int l, int d0, int nd, int &U0, int &U1)
+{
+    const int dl = (d0 + 63 < nd ? d0 + 63 : nd - 1);
+    const int df = d0 - (lx - l), dl_ = dl - (lx - l);
+    U0 = imax(0, df) >> 5;
+    U1 = (imin(ly, lx + dl_) - l) >> 5;
+}
+
 }  // namespace gkm

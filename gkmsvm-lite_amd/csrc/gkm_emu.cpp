@@ -176,8 +176,11 @@ EMU_API int emu_self_hist(const P2 *pl, const P2 *pl_rc, const int32_t *len, int
 }
 
 // Explain (mode 0: hypothetical, 1: perturbation) or ISM deltas (mode 2) of
-// row b against every SV: res [B][4][lmax] (explain), or for ISM dk [B][S][5][lmax]
-// (dm then da[4]) plus hist [B][S][nm].
+// row b against every SV, with the sweep kernel's bookkeeping: 64-diagonal
+// blocks walked over the block's common word range (walk_converged), hit
+// records (make_hit) applied position by position (apply_hit).  res
+// [B][4][lmax] (explain), or for ISM dk [B][S][5][lmax] (dm then da[4]) plus
+// hist [B][S][nm].
 EMU_API int emu_sweep(int mode, const P2 *xpl, const int32_t *xlen, int lmax, int B, int nwx,
                       const P2 *ypl, const P2 *ypl_rc, const int32_t *ylen, int S, int nwy, int l,
                       int nm, int cmax, const uint8_t *wtab, const double *ta, const double *tb,
@@ -188,12 +191,15 @@ EMU_API int emu_sweep(int mode, const P2 *xpl, const int32_t *xlen, int lmax, in
             const int lx = xlen[b];
             if (lx < l)
                 continue;
+            const int wx = lx - l + 1;
             for (int s = 0; s < S; s++) {
                 const int ly = ylen[s];
                 if (ly < l)
                     continue;
+                const int wy = ly - l + 1;
                 const int norient = ypl_rc ? 2 : 1;
                 const int nd = lx + ly - 2 * l + 1;
+                const int nblk = (nd + 63) / 64;
                 unsigned long long cnt[16] = {0};
                 double *dks = dk ? dk + ((int64_t)b * S + s) * 5 * lmax : nullptr;
                 double *rb = res ? res + (int64_t)b * 4 * lmax : nullptr;
@@ -208,16 +214,52 @@ EMU_API int emu_sweep(int mode, const P2 *xpl, const int32_t *xlen, int lmax, in
                         dks[(1 + yb) * lmax + pos] += v;
                 };
                 const double ecs = mode == SW_ISM ? 1.0 : ec[(int64_t)b * S + s];
-                for (int d = 0; d < nd * norient; d++) {
-                    const int o = d / nd, delta = d % nd - (lx - l);
-                    Words xs{xpl + (int64_t)b * nwx, nwx};
-                    Words ys{(o ? ypl_rc : ypl) + (int64_t)s * nwy, nwy};
-                    if (mode == SW_HYP)
-                        sweep_lane<0, 16, SW_HYP>(xs, ys, lx, ly, l, nm, cmax, delta, o, wtab, ecs, ta, tb, cnt, add);
-                    else if (mode == SW_PERT)
-                        sweep_lane<0, 16, SW_PERT>(xs, ys, lx, ly, l, nm, cmax, delta, o, wtab, ecs, ta, tb, cnt, add);
-                    else
-                        sweep_lane<0, 16, SW_ISM>(xs, ys, lx, ly, l, nm, cmax, delta, o, wtab, ecs, ta, tb, cnt, add);
+                for (int blk = 0; blk < nblk * norient; blk++) {
+                    const int o = blk / nblk, d0 = (blk % nblk) * 64;
+                    int U0, U1;
+                    wave_word_range(lx, ly, l, d0, nd, U0, U1);
+                    for (int lane = 0; lane < 64; lane++) {
+                        const int d = d0 + lane;
+                        const bool lane_ok = d < nd;
+                        const int delta = (lane_ok ? d : nd - 1) - (lx - l);
+                        const int ty_lo = imax(0, delta);
+                        const int ty_hi = imin(ly, lx + delta) - l;
+                        Words xs{xpl + (int64_t)b * nwx, nwx};
+                        Words ys{(o ? ypl_rc : ypl) + (int64_t)s * nwy, nwy};
+                        walk_converged<0>(
+                            xs, ys, nwx, nwy, l, delta, lane_ok, ty_lo, ty_hi, U0, U1,
+                            [&](int base, bool, uint32_t ep, uint32_t ecw, P2 yp, P2 yc,
+                                uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
+                                uint32_t valid) {
+                                uint32_t mk[16];
+                                levels<16>(b0, b1, b2, b3, valid, nm, mk);
+                                uint32_t hits = 0;
+                                for (int m = 0; m < 16; m++) {
+                                    if (mode == SW_ISM && !wtab)
+                                        cnt[m] += popc(mk[m]);
+                                    if (m <= cmax)
+                                        hits |= mk[m];
+                                }
+                                while (hits) {
+                                    const int j = __builtin_ctz(hits);
+                                    hits &= hits - 1u;
+                                    const HitRec h = make_hit(l, j, base, delta, o, ep, ecw, yp, yc,
+                                                              b0, b1, b2, b3, wtab, wx, wy);
+                                    if (mode == SW_ISM && wtab)
+                                        cnt[h.c] += h.w;
+                                    for (int q = 0; q < l; q++) {
+                                        if (h.tx + q >= lx)
+                                            continue;
+                                        if (mode == SW_HYP)
+                                            apply_hit<SW_HYP>(h, q, ecs, ta, tb, add);
+                                        else if (mode == SW_PERT)
+                                            apply_hit<SW_PERT>(h, q, ecs, ta, tb, add);
+                                        else
+                                            apply_hit<SW_ISM>(h, q, ecs, ta, tb, add);
+                                    }
+                                }
+                            });
+                    }
                 }
                 if (hist)
                     for (int m = 0; m < nm; m++)

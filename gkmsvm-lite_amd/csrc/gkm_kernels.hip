@@ -650,8 +650,10 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
     const int sv0 = blockIdx.y * a.sv_per_block;
     const int sv1 = min(a.S, sv0 + a.sv_per_block);
     const uint8_t *xc = MODE == SW_ISM ? a.xcodes + (int64_t)b * lmax : nullptr;
-    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const uint8_t *wtp = a.wtab ? wt : nullptr;
+    const int wx = lx - l + 1;
 
     auto add = [&](int kind, int yb, int pos, double v) {
         if (kind == 0)
@@ -661,11 +663,16 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
         else
             atomicAdd(&da[yb * lmax + pos], v);
     };
+    // Hits per distribution round: lane = (group, position), groups of l lanes.
+    const int G = min(WAVE / l, 32);
+    const int grp = lane / l, qpos = lane - grp * l;
+    const uint64_t below = (1ull << lane) - 1ull;
 
     for (int s = sv0; s < sv1; s++) {
         const int ly = a.ylen[s];
         if (ly < l)
             continue;
+        const int wy = ly - l + 1;
         for (int i = threadIdx.x; i < norient * nwy; i += SWEEP_THREADS) {
             int o = i / nwy, k = i % nwy;
             ys[o * nwy + k] = (o ? a.yplanes_rc : a.yplanes)[(int64_t)s * nwy + k];
@@ -677,13 +684,77 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
         for (int m = 0; m < NMC; m++)
             cnt[m] = 0;
         const int nd = lx + ly - 2 * l + 1;
-        for (int d = threadIdx.x; d < nd * norient; d += SWEEP_THREADS) {
-            const int o = d / nd;
-            const int delta = d % nd - (lx - l);
+        const int nblk = (nd + WAVE - 1) / WAVE;
+        // waves take 64 consecutive diagonals of one strand; lanes stay converged
+        for (int blk = wave; blk < nblk * norient; blk += SWEEP_THREADS / WAVE) {
+            const int o = blk / nblk;
+            const int d0 = (blk % nblk) * WAVE;
+            const int d = d0 + lane;
+            const bool lane_ok = d < nd;
+            const int delta = (lane_ok ? d : nd - 1) - (lx - l);
+            const int ty_lo = max(0, delta);
+            const int ty_hi = min(ly, lx + delta) - l;
+            int U0, U1;
+            wave_word_range(lx, ly, l, d0, nd, U0, U1);
             const P2 *yo = ys + o * nwy;
-            sweep_lane<LT, NMC, MODE>([&](int k) { return xs[k]; }, [&](int u) { return yo[u]; }, lx,
-                                      ly, l, a.nm, a.cmax, delta, o, wtp, ecs, stab, stab + 16, cnt,
-                                      add);
+            walk_converged<LT>(
+                [&](int k) { return xs[k]; }, [&](int u) { return yo[u]; }, nwx, nwy, l, delta,
+                lane_ok, ty_lo, ty_hi, U0, U1,
+                [&](int base, bool, uint32_t ep, uint32_t ec, P2 yp, P2 yc, uint32_t b0,
+                    uint32_t b1, uint32_t b2, uint32_t b3, uint32_t valid) {
+                    uint32_t mk[NMC];
+                    levels<NMC>(b0, b1, b2, b3, valid, a.nm, mk);
+                    uint32_t hits = 0;
+#pragma unroll
+                    for (int m = 0; m < NMC; m++) {
+                        if (MODE == SW_ISM && !wtp)
+                            cnt[m] += popc(mk[m]);
+                        if (m <= a.cmax)
+                            hits |= mk[m];
+                    }
+                    uint64_t any = __ballot(hits != 0u);
+                    while (any) {  // wave-uniform
+                        const int rank = __popcll(any & below);
+                        const bool chosen = hits != 0u && rank < G;
+                        const int nchosen = min(__popcll(any), G);
+                        const int j = hits ? __builtin_ctz(hits) : 0;
+                        // (lanes without a hit build a record nobody reads; no weights
+                        // for them, whose window indexes may lie outside the rows)
+                        const HitRec h = make_hit(l, j, base, delta, o, ep, ec, yp, yc, b0, b1,
+                                                  b2, b3, hits ? wtp : nullptr, wx, wy);
+                        if (MODE == SW_ISM && wtp && chosen) {
+#pragma unroll
+                            for (int m = 0; m < NMC; m++)
+                                cnt[m] += (h.c == m) ? (unsigned long long)h.w : 0ull;
+                        }
+                        // compact the chosen lanes' records to lanes 0..nchosen-1
+                        const int dst = (chosen ? rank : WAVE - 1) * 4;
+                        const uint32_t r0 = (uint32_t)h.tx | ((uint32_t)h.c << 16);
+                        const uint32_t r1 = h.mis | (h.y0 << 15);
+                        const uint32_t r2 = h.y1 | (h.w << 15);
+                        const uint32_t c0 = __builtin_amdgcn_ds_permute(dst, (int)r0);
+                        const uint32_t c1 = __builtin_amdgcn_ds_permute(dst, (int)r1);
+                        const uint32_t c2 = __builtin_amdgcn_ds_permute(dst, (int)r2);
+                        // each lane takes one position of one compacted hit
+                        const uint32_t s0 = __builtin_amdgcn_ds_bpermute(grp * 4, (int)c0);
+                        const uint32_t s1 = __builtin_amdgcn_ds_bpermute(grp * 4, (int)c1);
+                        const uint32_t s2 = __builtin_amdgcn_ds_bpermute(grp * 4, (int)c2);
+                        if (grp < nchosen) {
+                            HitRec g;
+                            g.tx = (int)(s0 & 0xFFFFu);
+                            g.c = (int)(s0 >> 16);
+                            g.mis = s1 & 0x7FFFu;
+                            g.y0 = (s1 >> 15) & 0x7FFFu;
+                            g.y1 = s2 & 0x7FFFu;
+                            g.w = s2 >> 15;
+                            if (g.tx + qpos < lx)
+                                apply_hit<MODE>(g, qpos, ecs, stab, stab + 16, add);
+                        }
+                        if (chosen)
+                            hits &= hits - 1u;
+                        any = __ballot(hits != 0u);
+                    }
+                });
         }
         if (MODE == SW_ISM) {
             // K_raw(x, s) from the block's histogram, then the F-sweep.
