@@ -230,6 +230,89 @@ def kernel_only(args):
     print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
 
 
+def _gc_seqs(rng, n, L, gc=0.42):
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return rng.choice(4, size=(n, L), p=p).astype(np.uint8)
+
+
+def config_runs(args):
+    """Bounded timings of the other configurations (one JSON line each): plain
+    -t 2 scoring, the 10k x 10k Gram, batched uneven -t 4 scoring (a sample of
+    the 100,000 queries), GkmExplain + ISM under -t 5 (a sample of the 2,000
+    queries)."""
+    import torch
+
+    from gkmsvm_amd import GkmSVM, compute_gram, gkmexplain, ism
+    from gkmsvm_amd.batching import BatchScorer
+    from gkmsvm_amd.kernels import build_kernel
+
+    def timed(f, reps=1):
+        f()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps, r
+
+    p = {"L": 11, "k": 7, "d": 3}
+    # plain scoring: 500 SVs x 300 bp, 1,000 queries x 300 bp
+    rng = np.random.default_rng(1)
+    sv = rng.integers(0, 4, (500, 300)).astype(np.uint8)
+    coef = np.abs(np.random.default_rng(2).normal(size=500))
+    coef[250:] *= -1
+    q = np.random.default_rng(3).integers(0, 4, (1000, 300)).astype(np.uint8)
+    m = GkmSVM(sv, coef, 0.0, 2, p)
+    t, _ = timed(lambda: m(q).cpu(), 5)
+    print(json.dumps({"config": "plain -t 2, 500 SVs x 300 bp, 1,000 x 300 bp queries",
+                      "ms": t * 1e3, "pairs_per_s": 500 * 1000 / t}), flush=True)
+    # Gram: 5,000 pos (planted 12-bp PWM at 1-3 sites) + 5,000 neg x 500 bp, GC 0.42
+    rng = np.random.default_rng(4)
+    X = _gc_seqs(rng, 10_000, 500)
+    pwm = rng.dirichlet(0.3 * np.ones(4), size=12)
+    for i in range(5000):
+        for site in rng.choice(np.arange(0, 488, 13), size=int(rng.integers(1, 4)), replace=False):
+            X[i, site:site + 12] = [rng.choice(4, p=pwm[c]) for c in range(12)]
+    kern = build_kernel(2, p)
+    t, g = timed(lambda: compute_gram(kern, X))
+    print(json.dumps({"config": "Gram -t 2, 10,000 x 10,000 x 500 bp (5k with a planted motif)",
+                      "s": t, "unique_pairs_per_s": 10_000 * 10_001 / 2 / t,
+                      "symmetric": bool((g == g.T).all())}), flush=True)
+    del g
+    # batched uneven -t 4: 20,000 SVs 100-1000 bp; a sample of the 100k queries
+    rng = np.random.default_rng(6)
+    svl = rng.integers(100, 1001, 20_000)
+    svc = np.zeros((20_000, 1000), np.uint8)
+    for i, n in enumerate(svl):
+        svc[i, :n] = rng.integers(0, 4, n)
+    c3 = rng.normal(size=20_000)
+    m3 = GkmSVM(svc, c3, 0.0, 4, {**p, "M": 50, "H": 50.0}, sv_lengths=svl)
+    nq = args.c3_queries
+    ql = rng.integers(100, 1001, nq)
+    qs = ["".join("ACGTN"[c] for c in rng.choice(5, n, p=[.2475] * 4 + [.01])) for n in ql]
+    sc = BatchScorer(m3, rows=256, bucket=128)
+    sc(qs[:512])  # tune + capture every bucket shape
+    t, _ = timed(lambda: sc(qs).cpu())
+    print(json.dumps({"config": f"batched -t 4, 20,000 SVs 100-1000 bp, {nq} of the 100,000 "
+                                "queries 100-1000 bp (1% N)",
+                      "s": t, "pairs_per_s": nq * 20_000 / t,
+                      "plans": {str(k): repr(v) for k, v in sc.plans.items()}}), flush=True)
+    # GkmExplain + ISM, -t 5 gamma 2: 50,000 SVs x 500 bp, Laplace(0, 0.5) coefficients
+    rng = np.random.default_rng(8)
+    sv4 = _gc_seqs(rng, 50_000, 500, 0.5)
+    m4 = GkmSVM(sv4, rng.laplace(0, 0.5, 50_000), 0.0, 5,
+                {**p, "gamma": 2.0, "M": 50, "H": 50.0})
+    q4 = ["".join("ACGT"[c] for c in rng.integers(0, 4, 1000)) for _ in range(args.c4_queries)]
+    t, _ = timed(lambda: gkmexplain(m4, q4, mode="hypothetical").cpu())
+    print(json.dumps({"config": f"gkmexplain -t 5, 50,000 SVs x 500 bp, {len(q4)} of the 2,000 "
+                                "x 1 kb queries", "ms_per_query": t / len(q4) * 1e3}), flush=True)
+    t, _ = timed(lambda: ism(m4, q4[:2]).cpu())
+    print(json.dumps({"config": "ISM -t 5, 50,000 SVs x 500 bp, 2 of the 2,000 x 1 kb queries "
+                                "(3,000 mutants each)",
+                      "ms_per_query": t / 2 * 1e3,
+                      "mutant_sv_evals_per_s": 2 * 3000 * 50_000 / t}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,8 +327,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--mode", default="serve", choices=["serve", "cross", "index", "copy"],
-                    help="serve: the benchmark; cross/index/copy: one kernel, for profiling")
+    ap.add_argument("--mode", default="serve",
+                    choices=["serve", "cross", "index", "copy", "configs"],
+                    help="serve: the benchmark; cross/index/copy: one kernel, for profiling; "
+                         "configs: bounded timings of the other configurations")
+    ap.add_argument("--c3-queries", type=int, default=4096)
+    ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -253,6 +340,8 @@ def main():
               "note, not part of this build (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "configs":
+        return config_runs(args)
     if args.mode != "serve":
         return kernel_only(args)
 

@@ -817,6 +817,7 @@ constexpr int INDEX_THREADS = 256;
 
 struct IndexArgs {
     const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
+    const int32_t *xpos;     // [B][Wx] position of each window in its row, or NULL
     const int32_t *xlen;     // [B]
     const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
     const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
@@ -836,6 +837,10 @@ struct IndexArgs {
 // weight times the query window's) to N[level] of that pair -- the pairs
 // kernels/lmer_index_cpu.py:23-37 (_add_window) visits.  The tile's counts go
 // out with plain stores to the chunk's slice of `partial`.
+// PACK16: two 16-bit counts per 32-bit LDS word (unweighted; the host caps the
+// window chunk so no count of a block can reach 2^16), which doubles the SVs a
+// tile holds and halves the probes.
+template <bool PACK16>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -845,7 +850,8 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     const int b = blockIdx.y;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int ncnt = a.tile * a.nm;
+    const int wps = PACK16 ? (a.nm + 1) / 2 : a.nm;  // LDS words per SV
+    const int ncnt = min(a.tile, a.S - tile_id * a.tile) * wps;  // this tile's SVs
     for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS)
         cnt[i] = 0u;
     __syncthreads();
@@ -859,7 +865,8 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         const uint32_t w = a.xwin[(int64_t)b * a.Wx + i];
         if (w >= nl)
             continue;  // PAD
-        const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, i) : 1u;
+        const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
+        const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
         for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
             const int t = t0 + lane;
             if (t >= a.n_masks)
@@ -872,16 +879,26 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 const int sid = a.ids[e];
                 if (sid < lo_col)
                     continue;
-                const uint32_t add = a.vals ? aw * a.vals[e] : 1u;
-                atomicAdd(&cnt[sid * a.nm + lv], add);
+                if (PACK16) {
+                    atomicAdd(&cnt[sid * wps + (lv >> 1)], 1u << (16 * (lv & 1)));
+                } else {
+                    const uint32_t add = a.vals ? aw * a.vals[e] : 1u;
+                    atomicAdd(&cnt[sid * a.nm + lv], add);
+                }
             }
         }
     }
     __syncthreads();
     const int ns = min(a.tile, a.S - s_base);
     uint32_t *dst = a.partial + (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
-    for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS)
-        dst[i] = cnt[i];
+    for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+        if (PACK16) {
+            const int sid = i / a.nm, lv = i - sid * a.nm;
+            dst[i] = (cnt[sid * wps + (lv >> 1)] >> (16 * (lv & 1))) & 0xFFFFu;
+        } else {
+            dst[i] = cnt[i];
+        }
+    }
 }
 
 // hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m].
@@ -897,12 +914,12 @@ __global__ void k_index_reduce(const uint32_t *__restrict__ partial, int64_t n, 
     hist[i] += v;
 }
 
-GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xlen, int B, int Wx,
+GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                            const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
                            const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
                            int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
-                           int chunk_windows, int64_t col_min, int64_t row_base, void *partial,
-                           void *hist, void *stream)
+                           int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
+                           void *partial, void *hist, void *stream)
 {
     int rc = check_l_nm(l, nm);
     if (rc)
@@ -916,11 +933,13 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xlen, int B, int
         return fail(-1, "gkm_index_hist: at most 65535 rows per call, got %d", B);
     if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
         return 0;
-    size_t lds = (size_t)tile * nm * sizeof(uint32_t);
+    const bool pack16 = vals == nullptr && pack_counts;
+    size_t lds = (size_t)tile * (pack16 ? (nm + 1) / 2 : nm) * sizeof(uint32_t);
     if (lds > 160 * 1024)
         return fail(-2, "gkm_index_hist: a tile of %d SVs x %d levels exceeds LDS", tile, nm);
     IndexArgs a;
     a.xwin = xwin;
+    a.xpos = xpos;
     a.xlen = xlen;
     a.offsets = offsets;
     a.ids = ids;
@@ -941,12 +960,15 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xlen, int B, int
     a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     a.col_min = col_min;
     a.row_base = row_base;
+    const void *kfn = pack16 ? (const void *)k_index_tile<true> : (const void *)k_index_tile<false>;
     if (lds > 64 * 1024 &&
-        hipFuncSetAttribute((const void *)k_index_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
+        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
     dim3 grid((unsigned)(n_tiles * a.nchunks), (unsigned)B);
-    hipLaunchKernelGGL(k_index_tile, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    if (pack16)
+        hipLaunchKernelGGL(k_index_tile<true>, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    else
+        hipLaunchKernelGGL(k_index_tile<false>, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_hist");
     int64_t n = (int64_t)B * S * nm;
     hipLaunchKernelGGL(k_index_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S_(stream),

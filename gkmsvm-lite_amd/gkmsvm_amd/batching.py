@@ -1,0 +1,181 @@
+"""Batched scoring of many queries of uneven lengths.
+
+Queries are sorted by length and cut into length buckets (each a multiple of
+32 bases); a bucket's rows go to the GPU in steps of ``rows`` rows padded to the
+bucket's length -- rows inside a step keep their own lengths, so padding costs
+memory only.  For each bucket shape the autotuner times the candidate
+configurations once (the l-mer index with a few block targets, the
+brute-force kernel with a few launch shapes) on the bucket's first step and
+keeps the fastest; every configuration computes the same integers, so the
+choice never changes a result.  With ``graphs`` each bucket shape is captured
+once as a HIP graph (packing, self-kernels, histograms, combine) over static
+input buffers and replayed for every later step of that shape.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import engine as E
+from .inputs import as_batch, full_lengths, nan_rows, short_rows
+
+
+class _Plan:
+    __slots__ = ("use_index", "shape", "target_blocks", "ms")
+
+    def __init__(self, use_index, shape=None, target_blocks=2048, ms=float("inf")):
+        self.use_index = use_index
+        self.shape = shape
+        self.target_blocks = target_blocks
+        self.ms = ms
+
+    def __repr__(self):
+        if self.use_index:
+            return f"index(target_blocks={self.target_blocks}, {self.ms:.2f} ms)"
+        return f"brute(shape={self.shape}, {self.ms:.2f} ms)"
+
+
+def _score_step(sv, kp, codes_t, len_t, lengths, out, plan: _Plan):
+    """One step on device inputs (codes_t [R, Lb] uint8, len_t [R] int32)."""
+    xs = E.DeviceSeqs.__new__(E.DeviceSeqs)
+    xs.n, xs.lmax = codes_t.shape
+    xs.nw = xs.lmax // 32 + 4
+    xs.lengths = lengths
+    xs.device = kp.device
+    xs.codes = codes_t
+    xs.len = len_t
+    xs.planes = torch.empty((xs.n, xs.nw, 2), dtype=torch.int32, device=kp.device)
+    xs.planes_rc = torch.empty_like(xs.planes) if kp.rc else None
+    xs._windows = {}
+    _lib.call("gkm_pack_planes", E._p(xs.codes), E._p(xs.len), xs.n, xs.lmax, xs.nw,
+              E._p(xs.planes), E._p(xs.planes_rc), E.stream_handle())
+    dx = E.self_kraw(kp, xs) if kp.normalize else None
+    hist = torch.empty((xs.n, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
+    if plan.use_index:
+        E.index_hist(kp, sv.index(), xs, 0, xs.n, sv.S, out=hist,
+                     target_blocks=plan.target_blocks)
+    else:
+        E.cross_hist(kp, xs, 0, xs.n, sv.seqs, sv.groups, out=hist, shape=plan.shape)
+    _lib.call("gkm_combine_score", E._p(hist), xs.n, sv.S, kp.nm, E._p(kp.tab),
+              E._p(dx) if dx is not None else None, E._p(sv.ds), E._p(sv.coef),
+              int(kp.normalize), int(kp.rbf), kp.gamma, 0.0, E._p(out), E.stream_handle())
+    return out
+
+
+class BatchScorer:
+    """Score many sequences of uneven lengths against one resident model."""
+
+    def __init__(self, model, *, rows: int = 256, bucket: int = 128, graphs: bool = True,
+                 autotune: bool = True):
+        model.rocm()
+        self.model = model
+        self.sv = model._svs()
+        self.kp = model._params()
+        self.rows = int(rows)
+        self.bucket = max(32, int(bucket) // 32 * 32)
+        self.graphs = graphs
+        self.autotune = autotune
+        self.plans: dict = {}
+        self._graphs: dict = {}
+        # the bucket histogram must fit the hist budget of one step
+        self.rows = max(1, min(self.rows, E.rows_per_chunk(self.sv.S, self.kp.nm)))
+
+    # -- autotuning ----------------------------------------------------------------
+
+    def _candidates(self, Lb):
+        cands = []
+        if self.kp.l <= E.INDEX_MAX_L and self.kp.dmax >= 0:
+            cands += [_Plan(True, target_blocks=t) for t in (1024, 2048, 4096)]
+        norient = 2 if self.kp.rc else 1
+        cands += [_Plan(False, shape=E.launch_shape(self.rows, self.sv.S, norient))]
+        cands += [_Plan(False, shape=s) for s in ((1, 1), (2, 1), (1, 2), (4, 1))]
+        return cands
+
+    def _plan_for(self, Lb, codes_t, len_t, lengths, out):
+        key = (Lb, codes_t.shape[0])
+        if key in self.plans:
+            return self.plans[key]
+        if not self.autotune:
+            plan = _Plan(self.sv.use_index_for(lengths, self.model.use_lmer_index),
+                         target_blocks=2048)
+            self.plans[key] = plan
+            return plan
+        best = None
+        for cand in self._candidates(Lb):
+            if self.model.use_lmer_index is False and cand.use_index:
+                continue
+            if self.model.use_lmer_index is True and not cand.use_index:
+                continue
+            _score_step(self.sv, self.kp, codes_t, len_t, lengths, out, cand)  # warm-up
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _score_step(self.sv, self.kp, codes_t, len_t, lengths, out, cand)
+            torch.cuda.synchronize()
+            cand.ms = (time.perf_counter() - t) * 1e3
+            if best is None or cand.ms < best.ms:
+                best = cand
+        self.plans[key] = best
+        return best
+
+    # -- steps ----------------------------------------------------------------------------
+
+    def _run_step(self, Lb, codes: np.ndarray, lengths: np.ndarray) -> torch.Tensor:
+        dev = self.kp.device
+        R = self.rows
+        n = codes.shape[0]
+        pad = np.zeros((R, Lb), dtype=np.uint8)
+        pad[:n, : codes.shape[1]] = codes
+        lens = np.zeros(R, dtype=np.int64)
+        lens[:n] = lengths
+        key = (Lb, R)
+        g = self._graphs.get(key)
+        if g is not None:
+            g["codes"].copy_(torch.from_numpy(pad))
+            g["len"].copy_(torch.from_numpy(lens.astype(np.int32)))
+            g["graph"].replay()
+            return g["out"][:n]
+        codes_t = torch.from_numpy(pad).to(dev)
+        len_t = torch.from_numpy(lens.astype(np.int32)).to(dev)
+        out = torch.empty(R, dtype=torch.float64, device=dev)
+        plan = self._plan_for(Lb, codes_t, len_t, lens, out)
+        if not self.graphs:
+            return _score_step(self.sv, self.kp, codes_t, len_t, lens, out, plan)[:n]
+        # capture this shape: lengths live on the device, so any rows up to Lb
+        # bases replay correctly (host-side launch sizing uses the bucket's
+        # longest row, which bounds every later step's)
+        full = np.full(R, Lb, dtype=np.int64)
+        if self.sv.index() is not None and plan.use_index:
+            self.sv.index()
+        _score_step(self.sv, self.kp, codes_t, len_t, full, out, plan)  # warm-up (LDS limits)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            _score_step(self.sv, self.kp, codes_t, len_t, full, out, plan)
+        self._graphs[key] = {"graph": graph, "codes": codes_t, "len": len_t, "out": out}
+        graph.replay()
+        return out[:n]
+
+    def __call__(self, x, *, lengths=None) -> torch.Tensor:
+        """[B, 1] float64 decision values (NaN for rows shorter than l)."""
+        codes, lengths = as_batch(x, lengths)
+        l = self.kp.l
+        short = short_rows(lengths, l)
+        lens = full_lengths(codes, lengths)
+        order = np.argsort(lens, kind="stable")
+        out = torch.empty(len(lens), dtype=torch.float64, device=self.kp.device)
+        i = 0
+        while i < len(order):
+            # bucket of the next row's length; take up to `rows` rows that fit it
+            Lb = max(l, -(-int(lens[order[i]]) // self.bucket) * self.bucket)
+            j = i
+            while j < len(order) and j - i < self.rows and lens[order[j]] <= Lb:
+                j += 1
+            idx = order[i:j]
+            res = self._run_step(Lb, codes[idx][:, :Lb], lens[idx])
+            out[torch.from_numpy(idx).to(out.device)] = res + self.model.bias
+            i = j
+        return nan_rows(out[:, None], short)

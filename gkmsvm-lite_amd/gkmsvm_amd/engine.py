@@ -202,9 +202,8 @@ def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeq
     B = x1 - x0
     S = ys.n
     if out is None:
-        out = torch.zeros((B, S, kp.nm), dtype=torch.int64, device=kp.device)
-    else:
-        _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
+        out = torch.empty((B, S, kp.nm), dtype=torch.int64, device=kp.device)
+    _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
     norient = 2 if (kp.rc and ys.planes_rc is not None) else 1
     rpb, nsplit = shape if shape is not None else launch_shape(B, S, norient)
     wt, wn = kp.wtab(max(xs.lmax, ys.lmax))
@@ -220,7 +219,8 @@ def cross_hist(kp: KernelParams, xs: DeviceSeqs, x0: int, x1: int, ys: DeviceSeq
 
 def self_kraw(kp: KernelParams, xs: DeviceSeqs, stream=None) -> torch.Tensor:
     """K_raw(x, x) [n] float64 of every sequence (forward x forward and RC)."""
-    hist = torch.zeros((xs.n, kp.nm), dtype=torch.int64, device=kp.device)
+    hist = torch.empty((xs.n, kp.nm), dtype=torch.int64, device=kp.device)
+    _lib.call("gkm_zero_u64", _p(hist), hist.numel(), stream_handle(stream))
     wt, wn = kp.wtab(xs.lmax)
     longest = xs.max_windows
     # enough diagonal chunks that a lone long sequence still spreads over CUs
@@ -278,7 +278,12 @@ class DeviceIndex:
         nl = 1 << (2 * l)
         dev = kp.device
         S = seqs.n
-        self.tile = tile or index_tile(kp.nm)
+        # 16-bit packed LDS counts (unweighted, and a chunk of at least one window
+        # keeps every count below 2^16)
+        nb = 2 if kp.rc else 1
+        sv_windows = int(np.clip(seqs.lengths - l + 1, 0, None).max()) * nb if S else 0
+        self.pack16 = not kp.weighted and 0 < sv_windows < 65536
+        self.tile = tile or min(index_tile(kp.nm, self.pack16), max(256, -(-S // 256) * 256))
         self.n_tiles = (S + self.tile - 1) // self.tile
         fwd, rcw = seqs.windows(l, stream)
         blocks = [fwd] + ([rcw] if kp.rc else [])
@@ -325,20 +330,41 @@ class DeviceIndex:
         return sum(x.numel() * x.element_size() for x in t)
 
 
-def index_tile(nm: int) -> int:
-    """SVs per index tile: the tile's [tile, nm] uint32 counts take 128 KiB of
-    LDS (the index is probe-bound, so the largest tile that fits wins)."""
-    return max(256, (32768 // nm) // 256 * 256)
+def index_tile(nm: int, pack16: bool = False) -> int:
+    """SVs per index tile: the tile's counts ([tile, nm] uint32, or two 16-bit
+    counts per word) take 128 KiB of LDS -- the index is probe-bound, so the
+    largest tile that fits wins; at most 65,536 (uint16 ids)."""
+    words = (nm + 1) // 2 if pack16 else nm
+    return min(65536, max(256, (32768 // words) // 256 * 256))
+
+
+MAX_PARTIAL_BYTES = 2 << 30
 
 
 def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: int, S: int, *,
                col_min: int = -1, out=None, stream=None, target_blocks: int = 2048) -> torch.Tensor:
-    """[x1 - x0, S, nm] int64 histograms of rows x0..x1 through the index."""
+    """[x1 - x0, S, nm] int64 histograms of rows x0..x1 through the index (in
+    row groups whose per-chunk partial counts stay under MAX_PARTIAL_BYTES)."""
     B = x1 - x0
     if out is None:
-        out = torch.zeros((B, S, kp.nm), dtype=torch.int64, device=kp.device)
-    else:
-        _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
+        out = torch.empty((B, S, kp.nm), dtype=torch.int64, device=kp.device)
+    _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
+    fwd, _ = xs.windows(kp.l, stream)
+    Wx = fwd.shape[1]
+    # rows per launch: the per-chunk partials of the smallest chunk (16 windows,
+    # or the 16-bit cap) must fit the budget
+    min_cw = 16 if not idx.pack16 else max(1, min(16, 65535 // max(1, idx.max_sv_windows)))
+    per_row = -(-Wx // min_cw) * S * kp.nm * 4
+    step = max(1, min(B, MAX_PARTIAL_BYTES // max(1, per_row), 65535))
+    for r0 in range(0, B, step):
+        r1 = min(B, r0 + step)
+        _index_launch(kp, idx, xs, x0 + r0, x0 + r1, S, col_min, out[r0:r1], stream,
+                      target_blocks)
+    return out
+
+
+def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
+    B = x1 - x0
     fwd, _ = xs.windows(kp.l, stream)
     Wx = fwd.shape[1]
     wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
@@ -347,15 +373,23 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     if idx.vals is not None:  # per-block uint32 counts must not overflow
         cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
         cw = max(1, min(cw, cap))
+    elif idx.pack16:  # nor the 16-bit ones: a window meets an SV window at most once
+        cw = max(1, min(cw, 65535 // max(1, idx.max_sv_windows)))
     nchunks = -(-Wx // cw)
     partial = torch.empty((nchunks, B, S, kp.nm), dtype=torch.int32, device=kp.device)
     wt, _ = kp.wtab(xs.lmax)
+    # each row's windows in l-mer order (PAD last): a block's windows then share
+    # high digits, and so do their neighbours' offsets -- cache lines reused
+    keys = fwd[x0:x1].to(torch.int64) & 0xFFFFFFFF
+    keys, pos = torch.sort(keys, dim=1)
+    swin = keys.to(torch.int32).contiguous()
+    spos = pos.to(torch.int32).contiguous() if wt is not None else None
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
            lambda: _lib.call(
-               "gkm_index_hist", _p(fwd[x0:x1]), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
+               "gkm_index_hist", _p(swin), _p(spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
                _p(idx.ids), _p(idx.vals), _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l,
-               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, col_min, x0, _p(partial), _p(out),
-               stream_handle(stream)))
+               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, int(idx.pack16), col_min, x0,
+               _p(partial), _p(out), stream_handle(stream)))
     return out
 
 
@@ -412,7 +446,7 @@ class SVModel:
         per_lmer = norient * ws.sum() / 4**self.kp.l
         wx = np.clip(np.asarray(x_lengths) - self.kp.l + 1, 0, None).sum()
         probes = float(wx) * n_masks
-        n_tiles = -(-self.S // index_tile(self.kp.nm))
+        n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted))
         index_s = probes * n_tiles / INDEX_PROBES_PER_S + probes * per_lmer / INDEX_ENTRIES_PER_S
         brute_s = float(wx) * norient * ws.sum() / BRUTE_COMPARISONS_PER_S
         return index_s < brute_s

@@ -1,0 +1,116 @@
+"""Batched scoring (length buckets, autotuned launch plans, HIP-graph replays)
+and the continuous-batching server: every result equals the plain call's
+(scores bit for bit; GkmExplain, whose FP64 atomics sum in any order, within
+1e-10), alone or in any mix."""
+
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_model(S=700, lo=150, hi=500, kt=2, seed=0, **extra):
+    from gkmsvm_amd import GkmSVM
+
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, S)
+    codes = np.zeros((S, hi), np.uint8)
+    for i in range(S):
+        codes[i, : lens[i]] = rng.integers(0, 4, lens[i])
+    p = {"L": 11, "k": 7, "d": 3, **extra}
+    return GkmSVM(codes, rng.normal(size=S), 0.37, kt, p, sv_lengths=lens)
+
+
+def _queries(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    return ["".join("ACGTN"[c] for c in rng.choice(5, int(L), p=[.2475, .2475, .2475, .2475, .01]))
+            for L in rng.integers(lo, hi + 1, n)]
+
+
+@pytest.mark.parametrize("kt,extra", [(2, {}), (4, {"M": 50, "H": 50.0}), (3, {"gamma": 2.0})])
+def test_batch_scorer_matches_plain_calls(gpu, kt, extra):
+    """Uneven queries (8-700 bp, 1% N, some shorter than l) through buckets,
+    the autotuner and graph replays equal model(x) exactly."""
+    from gkmsvm_amd.batching import BatchScorer
+
+    m = _random_model(kt=kt, **extra)
+    qs = _queries(300, 8, 700, 1)
+    want = m(qs).cpu().numpy()
+    sc = BatchScorer(m, rows=64, bucket=128, graphs=True, autotune=True)
+    for rep in range(3):  # replays of captured graphs, eager work in between
+        got = sc(qs).cpu().numpy()
+        np.testing.assert_array_equal(got, want, err_msg=f"replay {rep}")
+        m(qs[:5])
+        torch.cuda.synchronize()
+    assert sc.plans and sc._graphs
+    got = BatchScorer(m, rows=32, graphs=False, autotune=False)(qs).cpu().numpy()
+    np.testing.assert_array_equal(got, want)
+
+
+def test_server_results_equal_alone_and_in_the_mix(gpu):
+    """Requests submitted concurrently from several threads (scoring of many
+    lengths, variants, explain), served in overlapping steps on two streams,
+    return what each gives alone."""
+    from gkmsvm_amd import gkmexplain
+    from gkmsvm_amd.serving import GkmServer
+
+    m = _random_model(S=900, seed=3)
+    rng = np.random.default_rng(4)
+    reqs = []
+    for i in range(60):
+        u = rng.random()
+        if u < 0.6:
+            reqs.append(("score", _queries(int(rng.integers(1, 4)), 5, 900, 100 + i)))
+        elif u < 0.85:
+            ref = _queries(1, 200, 300, 200 + i)[0]
+            alt = ref[:100] + ("A" if ref[100] != "A" else "C") + ref[101:]
+            reqs.append(("variant", [(ref, alt)]))
+        else:
+            reqs.append(("explain", _queries(1, 300, 600, 300 + i)))
+    alone = []
+    for kind, payload in reqs:
+        if kind == "score":
+            alone.append(m(payload).cpu().numpy().ravel())
+        elif kind == "variant":
+            alone.append(m.score_variants([p[0] for p in payload],
+                                          [p[1] for p in payload]).cpu().numpy().ravel())
+        else:
+            alone.append(gkmexplain(m, payload, mode="hypothetical").cpu().numpy())
+    futs = [None] * len(reqs)
+    with GkmServer(m, hbm_budget=2 << 30, streams=2) as srv:
+        def client(k):
+            for i in range(k, len(reqs), 3):
+                kind, payload = reqs[i]
+                futs[i] = srv.submit(kind, payload)
+        ths = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        got = [f.result(timeout=120) for f in futs]
+    for (kind, _), g, a in zip(reqs, got, alone):
+        if kind == "explain":
+            H.assert_close(g, a, 1e-10, "explain")
+        else:
+            np.testing.assert_array_equal(g, a)
+
+
+def test_server_admission_rejects_oversized_and_small_budget_still_exact(gpu):
+    from gkmsvm_amd.serving import GkmServer
+
+    m = _random_model(S=300, seed=5)
+    qs = _queries(20, 50, 400, 9)
+    want = m(qs).cpu().numpy().ravel()
+    one = m._svs().S * m._params().nm * 8 + 4096
+    with GkmServer(m, hbm_budget=one * 3, streams=2) as srv:
+        big = srv.submit("score", qs)  # 20 rows: over the budget
+        with pytest.raises(MemoryError):
+            big.result(timeout=60)
+        futs = [srv.submit("score", [q]) for q in qs]  # one row each: admitted 3 per step
+        got = np.array([f.result(timeout=120)[0] for f in futs])
+    np.testing.assert_array_equal(got, want)
