@@ -33,12 +33,10 @@
 #include <math.h>
 
 #include "gkm_core.h"
+#include "gkm_internal.h"
 #include "gkmhip.h"
 
-#define GKM_API extern "C" __attribute__((visibility("default")))
-
 static int check_l_nm(int l, int nm);
-static inline hipStream_t S_(void *stream);
 
 using namespace gkm;
 
@@ -48,7 +46,7 @@ using namespace gkm;
 
 static thread_local char g_err[1024];
 
-static int fail(int code, const char *fmt, ...)
+int fail(int code, const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -56,13 +54,6 @@ static int fail(int code, const char *fmt, ...)
     va_end(ap);
     return code;
 }
-
-#define GKM_CHECK_LAUNCH(what)                                                  \
-    do {                                                                        \
-        hipError_t e_ = hipGetLastError();                                      \
-        if (e_ != hipSuccess)                                                   \
-            return fail(-3, "%s: launch failed: %s", what, hipGetErrorString(e_)); \
-    } while (0)
 
 GKM_API const char *gkm_last_error(void) { return g_err; }
 
@@ -1053,8 +1044,6 @@ static sweep_fn pick_sweep(int l, int nm, int mode)
     default: return pick_sweep_n<0>(nm, mode);
     }
 }
-
-static inline hipStream_t S_(void *stream) { return (hipStream_t)stream; }
 
 static int check_l_nm(int l, int nm)
 {

@@ -1,11 +1,13 @@
 """gkmsvm_amd: an MI355X (gfx950) engine for gapped k-mer SVM kernels.
 
-Drop-in for gkmsvm-lite's scoring, variant scoring, GkmExplain, ISM and the
-Gram build behind train_gkmsvm; the work runs in hand-written HIP kernels
+Drop-in for gkmsvm-lite's scoring, variant scoring, GkmExplain, ISM, the
+Gram build behind train_gkmsvm, the exact l-mer tables and to_deltasvm; the
+work runs in hand-written HIP kernels
 (libgkmhip.so, C-ABI in include/gkmhip.h) called through ctypes.
 """
 
 from .codec import codes_to_one_hot, encode_codes, one_hot_encode, to_codes
+from .deltasvm import DeltaSVM
 from .explain import gkmexplain, mutation_impact
 from .gram import compute_gram
 from .importers import load_lsgkm_model, read_fasta
@@ -29,5 +31,5 @@ __all__ = [
     "load_lsgkm_model", "read_fasta", "one_hot_encode", "encode_codes", "to_codes",
     "codes_to_one_hot", "to_cpu", "resolve_kernel_type", "KERNEL_BUILDERS", "KERNEL_ALIASES",
     "DirectGkmKernel", "EstTruncGkmKernel", "CenterWeightedGkmKernel", "RbfKernel",
-    "RbfGkmKernel", "CenterWeightedRbfGkmKernel",
+    "RbfGkmKernel", "CenterWeightedRbfGkmKernel", "DeltaSVM",
 ]

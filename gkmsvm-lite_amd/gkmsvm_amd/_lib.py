@@ -39,6 +39,15 @@ SIGNATURES = {
                        P, P],
     "gkm_ism": [P, P, P, I, I, I, P, P, P, I, I, I, I, P, I, I, P, P, P, P, P, I, I, D, P,
                 P],
+    "gkm_lmer_weights": [P, P, P, P, I, P, P],
+    "gkm_wht4": [P, I, I, P],
+    "gkm_table_spectrum": [P, I, P, D, P],
+    "gkm_explain_fill": [P, I, P, P, D, P, P],
+    "gkm_table_score": [P, P, I, I, I, P, P, P, I, D, P, P],
+    "gkm_table_explain": [P, P, I, I, I, I, P, P, P, I, I, P, P],
+    "gkm_table_ism": [P, P, P, I, I, I, I, P, P, P, P, P, I, P, P],
+    "gkm_deltasvm_weights": [P, I, D, P, I, I, P, P],
+    "gkm_deltasvm_score": [P, P, I, I, I, I, P, I, P, I, D, P, P],
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p}
 

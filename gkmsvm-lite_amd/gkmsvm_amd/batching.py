@@ -7,7 +7,9 @@ memory only.  For each bucket shape the autotuner times the candidate
 configurations once (the l-mer index with a few block targets, the
 brute-force kernel with a few launch shapes) on the bucket's first step and
 keeps the fastest; every configuration computes the same integers, so the
-choice never changes a result.  With ``graphs`` each bucket shape is captured
+choice never changes a result.  Models with an exact l-mer table (non-RBF;
+built under "auto" when the call's per-SV cost exceeds the build) score every
+bucket through the table instead: one lookup per window.  With ``graphs`` each bucket shape is captured
 once as a HIP graph (packing, self-kernels, histograms, combine) over static
 input buffers and replayed for every later step of that shape.
 """
@@ -25,15 +27,18 @@ from .inputs import as_batch, full_lengths, nan_rows, short_rows
 
 
 class _Plan:
-    __slots__ = ("use_index", "shape", "target_blocks", "ms")
+    __slots__ = ("use_index", "shape", "target_blocks", "ms", "table")
 
-    def __init__(self, use_index, shape=None, target_blocks=2048, ms=float("inf")):
+    def __init__(self, use_index, shape=None, target_blocks=2048, ms=float("inf"), table=None):
         self.use_index = use_index
         self.shape = shape
         self.target_blocks = target_blocks
         self.ms = ms
+        self.table = table
 
     def __repr__(self):
+        if self.table is not None:
+            return f"table({self.ms:.2f} ms)"
         if self.use_index:
             return f"index(target_blocks={self.target_blocks}, {self.ms:.2f} ms)"
         return f"brute(shape={self.shape}, {self.ms:.2f} ms)"
@@ -53,6 +58,8 @@ def _score_step(sv, kp, codes_t, len_t, lengths, out, plan: _Plan):
     xs._windows = {}
     _lib.call("gkm_pack_planes", E._p(xs.codes), E._p(xs.len), xs.n, xs.lmax, xs.nw,
               E._p(xs.planes), E._p(xs.planes_rc), E.stream_handle())
+    if plan.table is not None:
+        return E.table_rows(kp, xs, plan.table, 0.0, out)
     dx = E.self_kraw(kp, xs) if kp.normalize else None
     hist = torch.empty((xs.n, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
     if plan.use_index:
@@ -87,6 +94,9 @@ class BatchScorer:
     # -- autotuning ----------------------------------------------------------------
 
     def _candidates(self, Lb):
+        table = self.model._lmer_table_for()
+        if table is not None:  # one lookup per window: no per-SV plan comes close
+            return [_Plan(False, table=table)]
         cands = []
         if self.kp.l <= E.INDEX_MAX_L and self.kp.dmax >= 0:
             cands += [_Plan(True, target_blocks=t) for t in (1024, 2048, 4096)]
@@ -101,14 +111,15 @@ class BatchScorer:
             return self.plans[key]
         if not self.autotune:
             plan = _Plan(self.sv.use_index_for(lengths, self.model.use_lmer_index),
-                         target_blocks=2048)
+                         target_blocks=2048, table=self.model._lmer_table_for())
             self.plans[key] = plan
             return plan
         best = None
         for cand in self._candidates(Lb):
             if self.model.use_lmer_index is False and cand.use_index:
                 continue
-            if self.model.use_lmer_index is True and not cand.use_index:
+            if (self.model.use_lmer_index is True and not cand.use_index
+                    and cand.table is None):
                 continue
             _score_step(self.sv, self.kp, codes_t, len_t, lengths, out, cand)  # warm-up
             torch.cuda.synchronize()
@@ -165,6 +176,12 @@ class BatchScorer:
         l = self.kp.l
         short = short_rows(lengths, l)
         lens = full_lengths(codes, lengths)
+        # under "auto" the whole call decides whether the l-mer table pays
+        self.model._plan_lmer_table(float(np.clip(lens - l + 1, 0, None).sum()))
+        if self.model._lmer_table_for() is not None:
+            # plans and graphs made before the table existed are stale
+            self.plans = {k: v for k, v in self.plans.items() if v.table is not None}
+            self._graphs = {k: g for k, g in self._graphs.items() if k in self.plans}
         order = np.argsort(lens, kind="stable")
         out = torch.empty(len(lens), dtype=torch.float64, device=self.kp.device)
         i = 0

@@ -1,6 +1,8 @@
 """Build the engine's native libraries in-tree.
 
-libgkmhip.so   the HIP kernels and their C-ABI (hipcc, gfx950)
+libgkmhip.so   the HIP kernels and their C-ABI (hipcc, gfx950): the pairwise,
+               explain and ISM kernels and the l-mer index (gkm_kernels.hip),
+               the exact l-mer tables and DeltaSVM (gkm_tables.hip)
 libgkm_emu.so  a CPU build of the same lane code, used by the CPU test suite
                to check the bit-sliced algorithm without a GPU
 
@@ -27,6 +29,8 @@ HIP_FLAGS = [
     f"-I{CSRC}", f"-I{INCLUDE}",
 ]
 
+HIP_SOURCES = ["gkm_kernels.hip", "gkm_tables.hip"]
+
 
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -39,7 +43,7 @@ def build_hip() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libgkmhip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gkm_kernels.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *(os.path.join(CSRC, f) for f in HIP_SOURCES)])
     os.replace(tmp, out)
     return out
 

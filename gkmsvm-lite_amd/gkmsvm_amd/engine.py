@@ -422,6 +422,9 @@ class SVModel:
             self.ds = torch.ones(self.S, dtype=torch.float64, device=kp.device)
 
         self._index = None
+        # exact l-mer tables of non-RBF models (tables.py), built on demand:
+        # "V", "score", ("explain", perturbation)
+        self.tables: dict = {}
 
     def index(self):
         """The l-mer index of the SVs (built on first use), or None when l is
@@ -430,6 +433,30 @@ class SVModel:
             self._index = DeviceIndex(self.kp, self.seqs)
         return self._index
 
+    def _path_seconds(self, n_windows: float):
+        """(index, brute-force) estimated GPU seconds for n_windows query
+        windows against every SV (index None when it does not apply)."""
+        norient = 2 if self.kp.rc else 1
+        ws = float(np.clip(self.seqs.lengths - self.kp.l + 1, 0, None).sum())
+        brute_s = n_windows * norient * ws / BRUTE_COMPARISONS_PER_S
+        if self.kp.l > INDEX_MAX_L or self.kp.dmax < 0:
+            return None, brute_s
+        from math import comb
+
+        n_masks = sum(comb(self.kp.l, m) * 3**m for m in range(self.kp.dmax + 1)
+                      if self.kp.h[m] != 0)
+        per_lmer = norient * ws / 4**self.kp.l
+        probes = n_windows * n_masks
+        n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted))
+        index_s = probes * n_tiles / INDEX_PROBES_PER_S + probes * per_lmer / INDEX_ENTRIES_PER_S
+        return index_s, brute_s
+
+    def query_seconds(self, n_windows: float) -> float:
+        """Estimated GPU seconds of the cheaper per-SV path for n_windows query
+        windows."""
+        index_s, brute_s = self._path_seconds(float(n_windows))
+        return brute_s if index_s is None else min(index_s, brute_s)
+
     def use_index_for(self, x_lengths, mode="auto") -> bool:
         """Whether scoring rows of these lengths goes through the index: forced
         by mode True/False, else the cheaper of the two paths by the cost model."""
@@ -437,18 +464,8 @@ class SVModel:
             return False
         if mode is True:
             return True
-        from math import comb
-
-        n_masks = sum(comb(self.kp.l, m) * 3**m for m in range(self.kp.dmax + 1)
-                      if self.kp.h[m] != 0)
-        norient = 2 if self.kp.rc else 1
-        ws = np.clip(self.seqs.lengths - self.kp.l + 1, 0, None)
-        per_lmer = norient * ws.sum() / 4**self.kp.l
         wx = np.clip(np.asarray(x_lengths) - self.kp.l + 1, 0, None).sum()
-        probes = float(wx) * n_masks
-        n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted))
-        index_s = probes * n_tiles / INDEX_PROBES_PER_S + probes * per_lmer / INDEX_ENTRIES_PER_S
-        brute_s = float(wx) * norient * ws.sum() / BRUTE_COMPARISONS_PER_S
+        index_s, brute_s = self._path_seconds(float(wx))
         return index_s < brute_s
 
     def diag_original_order(self) -> torch.Tensor:
@@ -476,10 +493,29 @@ def sv_hist(sv: SVModel, xs: DeviceSeqs, r0: int, r1: int, *, out=None, stream=N
     return cross_hist(sv.kp, xs, r0, r1, sv.seqs, sv.groups, out=out, stream=stream, shape=shape)
 
 
+def table_rows(kp: KernelParams, xs: DeviceSeqs, table: torch.Tensor, bias: float,
+               out: torch.Tensor, *, normalize: bool | None = None, stream=None):
+    """out[b] = sum_i wt_i table[x_b,i] / sqrt(K_raw(x_b, x_b)) + bias through
+    the exact l-mer table (svm.py:559-582); normalize=False gives the raw
+    window sums (bias then 0 in the callers that need them)."""
+    norm = kp.normalize if normalize is None else normalize
+    dx = self_kraw(kp, xs, stream) if norm else None
+    fwd, _ = xs.windows(kp.l, stream)
+    wt, _ = kp.wtab(xs.lmax)
+    _timed("k_table_score", lambda: float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * 12.0,
+           stream, lambda: _lib.call(
+               "gkm_table_score", _p(fwd), _p(xs.len), xs.n, fwd.shape[1], kp.l, _p(wt),
+               _p(table), _p(dx), int(norm), float(bias), _p(out), stream_handle(stream)))
+    return out
+
+
 def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stream=None,
-               shape=None, use_index="auto"):
-    """out[b] = sum_s coef_s F(K(x_b, s)) + bias for every row of xs."""
+               shape=None, use_index="auto", table: torch.Tensor | None = None):
+    """out[b] = sum_s coef_s F(K(x_b, s)) + bias for every row of xs (through
+    the exact l-mer table when one is given)."""
     kp = sv.kp
+    if table is not None:
+        return table_rows(kp, xs, table, bias, out, stream=stream)
     dx = self_kraw(kp, xs, stream) if kp.normalize else None
     step = rows_per_chunk(sv.S, kp.nm)
     hist = None

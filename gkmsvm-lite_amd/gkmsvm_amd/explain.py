@@ -59,6 +59,11 @@ def explain_tables(base, perturbation: bool):
     return a, b, (max(nz) if nz else -1)
 
 
+# Window-pair comparisons per second of the explain sweep (k_sweep, MI355X):
+# the per-SV cost that an explain table has to beat
+SWEEP_COMPARISONS_PER_S = 5e12
+
+
 def _sv_chunk(B: int, S: int) -> int:
     """SVs per block of the sweep kernel: enough blocks to fill the chip."""
     spb = max(1, (B * S) // 8192)
@@ -85,7 +90,11 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
     B, L = codes.shape
     out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
     if L >= base.l:
+        A = model._explain_table_for(B, lens, resolved == "perturbation")
         bs = max(1, int(batch_size)) if batch_size else B
+        if A is not None:
+            bs = max(bs, 4096)  # a table row costs l lookups per position
+        bs = min(bs, 65535)
         starts = range(0, B, bs)
         if verbose:
             from tqdm import tqdm
@@ -93,7 +102,11 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
             starts = tqdm(starts, desc="GkmExplain")
         for s in starts:
             e = min(B, s + bs)
-            _explain_rows(model, codes[s:e], lens[s:e], resolved, out[s:e])
+            if A is not None:
+                _explain_rows_table(model, codes[s:e], lens[s:e], A,
+                                    resolved == "perturbation", out[s:e])
+            else:
+                _explain_rows(model, codes[s:e], lens[s:e], resolved, out[s:e])
     if resolved == "importance":
         out = out * torch.from_numpy(codes_to_one_hot(codes)).to(dev)
     return per_position(out, lengths, short)
@@ -132,6 +145,21 @@ def _explain_rows(model, codes, lens, resolved, out):
     out += buf
 
 
+def _explain_rows_table(model, codes, lens, A, pert, out):
+    """GkmExplain of rows from the explain table (explain/table.py:172-196)."""
+    kp = model._params()
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    dx = E.self_kraw(kp, xs) if kp.normalize else None
+    fwd, _ = xs.windows(kp.l)
+    wt, _ = kp.wtab(xs.lmax)
+    W = np.clip(xs.lengths - kp.l + 1, 0, None)
+    E._timed("k_table_explain", lambda: float(W.sum()) * (4 + 32 * kp.l) + xs.n * 32.0 * xs.lmax,
+             None, lambda: _lib.call(
+                 "gkm_table_explain", E._p(fwd), E._p(xs.len), xs.n, fwd.shape[1], xs.lmax,
+                 kp.l, E._p(wt), E._p(A), E._p(dx), int(kp.normalize), int(pert), E._p(out),
+                 E.stream_handle()))
+
+
 def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
                     batch_size: int | None = None, verbose: bool = False) -> torch.Tensor:
     """[B, 1] GkmExplain mutation impact (lsgkm gkmexplain -m 5): sum_j coef_j
@@ -153,6 +181,10 @@ def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
     out = torch.zeros(B, dtype=torch.float64, device=dev)
     bs = B if batch_size is None else max(1, int(batch_size))
     st = E.stream_handle()
+    # the per-SV paths visit every window of both alleles (explain/impact.py:110-120)
+    W = np.clip(np.maximum(rl, al) - kp.l + 1, 0, None)
+    model._plan_lmer_table(float(2 * W.sum()))
+    table = model._lmer_table_for()
     for s in range(0, B, bs):
         e = min(B, s + bs)
         n = e - s
@@ -164,6 +196,16 @@ def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
             dref = E.self_kraw(kp, rs)
         else:
             dref = torch.ones(n, dtype=torch.float64, device=dev)
+        if table is not None:
+            # linear in the SVs: the table's alt minus ref sums over the
+            # reference allele's norm (explain/impact.py:139-148)
+            raw = torch.empty(2 * n, dtype=torch.float64, device=dev)
+            E.table_rows(kp, xs, table, 0.0, raw, normalize=False)
+            dk = raw[n:] - raw[:n]
+            if kp.normalize:
+                dk = dk / torch.sqrt(torch.clamp(dref, min=1e-10))
+            out[s:e] = dk
+            continue
         hist = E.sv_hist(sv, xs, 0, 2 * n, use_index=model.use_lmer_index)
         _lib.call("gkm_combine_impact", E._p(hist), n, sv.S, kp.nm, E._p(kp.tab), E._p(dref),
                   E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma,

@@ -36,6 +36,10 @@ def ism(model, x, *, lengths=None, batch_size: int | None = None,
     B, L = codes.shape
     out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
     if L >= kp.l and kp.dmax >= 0:
+        # without the table each (position, sequence) queries the l windows
+        # covering it before and after each substitution (ism.py:49-52)
+        model._plan_lmer_table(5.0 * kp.l * B * L)
+        table = model._lmer_table_for()
         bs = B if batch_size is None else max(1, int(batch_size))
         bs = min(bs, 65535)
         starts = range(0, B, bs)
@@ -45,8 +49,41 @@ def ism(model, x, *, lengths=None, batch_size: int | None = None,
             starts = tqdm(starts, desc="ISM")
         for s in starts:
             e = min(B, s + bs)
-            _ism_rows(model, codes[s:e], lens[s:e], out[s:e])
+            if table is not None:
+                _ism_rows_table(model, codes[s:e], lens[s:e], table, out[s:e])
+            else:
+                _ism_rows(model, codes[s:e], lens[s:e], out[s:e])
     return per_position(out, lengths, short)
+
+
+def _mutant_diag(kp, xs, dx):
+    """K_raw of every single-base mutant [4, lmax, B] (gkm_mutant_diag)."""
+    fwd, rcw = xs.windows(kp.l)
+    dmut = torch.empty((4, xs.lmax, xs.n), dtype=torch.float64, device=kp.device)
+    wt, _ = kp.wtab(xs.lmax)
+    _lib.call("gkm_mutant_diag", E._p(fwd), E._p(rcw) if kp.rc else None, E._p(xs.len),
+              E._p(xs.codes), xs.lmax, xs.n, kp.l, E._p(kp.tab), E._p(wt), E._p(dx),
+              E._p(dmut), E.stream_handle())
+    return dmut
+
+
+def _ism_rows_table(model, codes, lens, table, out):
+    """ISM from the l-mer table (ism.py:118-144): per position the table
+    weights of the l windows covering it, before and after each substitution."""
+    kp = model._params()
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    B = xs.n
+    raw = torch.empty(B, dtype=torch.float64, device=kp.device)
+    E.table_rows(kp, xs, table, 0.0, raw, normalize=False)
+    dx = dmut = None
+    if kp.normalize:
+        dx = E.self_kraw(kp, xs)
+        dmut = _mutant_diag(kp, xs, dx)
+    fwd, _ = xs.windows(kp.l)
+    wt, _ = kp.wtab(xs.lmax)
+    _lib.call("gkm_table_ism", E._p(fwd), E._p(xs.len), E._p(xs.codes), B, fwd.shape[1],
+              xs.lmax, kp.l, E._p(wt), E._p(table), E._p(raw), E._p(dx), E._p(dmut),
+              int(kp.normalize), E._p(out), E.stream_handle())
 
 
 def _ism_rows(model, codes, lens, out):
@@ -57,12 +94,7 @@ def _ism_rows(model, codes, lens, out):
     B = xs.n
     if kp.normalize:
         dx = E.self_kraw(kp, xs)
-        fwd, rcw = xs.windows(kp.l)
-        dmut = torch.empty((4, xs.lmax, B), dtype=torch.float64, device=kp.device)
-        wt, _ = kp.wtab(xs.lmax)
-        _lib.call("gkm_mutant_diag", E._p(fwd), E._p(rcw) if kp.rc else None, E._p(xs.len),
-                  E._p(xs.codes), xs.lmax, B, kp.l, E._p(kp.tab), E._p(wt), E._p(dx),
-                  E._p(dmut), st)
+        dmut = _mutant_diag(kp, xs, dx)
     else:
         dx = torch.ones(B, dtype=torch.float64, device=kp.device)
         dmut = torch.ones((4, xs.lmax, B), dtype=torch.float64, device=kp.device)

@@ -14,7 +14,10 @@ would hold (histograms, planes, outputs) stay within ``hbm_budget``; a
 request larger than the whole budget is rejected with MemoryError.
 
 Each row's result depends only on that row (the per-row reduction runs in a
-fixed order), so a request scores identically alone and in any mix.
+fixed order), so a request scores identically alone and in any mix.  For
+non-RBF models the server builds the exact l-mer tables at start (``tables``):
+scoring and variant rows then cost one table lookup per window and explain
+rows l lookups per position, independent of the number of SVs.
 """
 
 from __future__ import annotations
@@ -29,6 +32,7 @@ import numpy as np
 import torch
 
 from . import engine as E
+from . import tables as T
 from .codec import encode_codes
 from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
 
@@ -67,11 +71,25 @@ class GkmServer:
     """Serve one model; see the module docstring."""
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
-                 streams: int = 2, max_explain: int = 8):
+                 streams: int = 2, max_explain: int = 8, tables: bool = True):
         model.rocm()
         self.model = model
         self.sv = model._svs()
         self.kp = model._params()
+        # A server amortises the exact l-mer tables over its whole stream: with
+        # ``tables`` it builds them up front wherever the model allows them
+        # (non-RBF; use_lmer_table / use_explain_table not False), so scoring
+        # and variant rows cost one lookup per window and explain rows l per
+        # position, whatever the number of SVs.
+        self.table = None
+        self.explain_tables = False
+        if tables and model.use_lmer_table is not False:
+            self.table = model._build_lmer_table()
+        if (tables and model.use_explain_table is not False and self.table is not None
+                and T.explain_table_bytes(self.kp.l) <= T.EXPLAIN_TABLE_BYTES):
+            for pert in (False, True):
+                self._build_explain(pert)
+            self.explain_tables = True
         self.hbm_budget = int(hbm_budget)
         self.max_rows = int(max_rows)
         self.max_explain = int(max_explain)
@@ -83,6 +101,16 @@ class GkmServer:
         self._stop = False
         self._thread = None
         self.completed: list = []
+
+    def _build_explain(self, pert: bool):
+        from .explain import explain_tables
+
+        key = ("explain", pert)
+        if key not in self.sv.tables:
+            ta, tb, _ = explain_tables(self.kp.base, pert)
+            if "V" not in self.sv.tables:
+                self.sv.tables["V"] = T.lmer_weights(self.sv)
+            self.sv.tables[key] = T.explain_table(self.sv, ta, tb, self.sv.tables["V"])
 
     # -- client side ----------------------------------------------------------------
 
@@ -127,10 +155,12 @@ class GkmServer:
     def _bytes(self, req: Request) -> int:
         S, nm = self.sv.S, self.kp.nm
         L = req.max_len()
-        planes = 2 * (L // 32 + 4) * 8 + L
+        planes = 2 * (L // 32 + 4) * 8 + L + 8 * L  # planes, codes, packed windows
         if req.kind == "explain":
-            return len(req.seqs) * (S * (8 + nm * 8) + 4 * L * 8 * 2 + planes)
-        return req.rows() * (S * nm * 8 + planes + 16)
+            per_sv = 0 if self.explain_tables else S * (8 + nm * 8)
+            return len(req.seqs) * (per_sv + 4 * L * 8 * 2 + planes)
+        per_sv = 0 if self.table is not None else S * nm * 8
+        return req.rows() * (per_sv + planes + 16)
 
     def _admit(self):
         """Pop the requests of the next step (at least one if any is pending)."""
@@ -226,7 +256,7 @@ class GkmServer:
                     codes[i, : len(r)] = r
                 xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
                 out = torch.empty(len(codes_rows), dtype=torch.float64, device=kp.device)
-                E.score_rows(sv, xs, model.bias, out)
+                E.score_rows(sv, xs, model.bias, out, table=self.table)
                 short = torch.from_numpy(lens < kp.l).to(kp.device)
                 out = torch.where(short, torch.full_like(out, float("nan")), out)
                 score_host = torch.empty(out.shape, dtype=torch.float64, pin_memory=True)

@@ -11,7 +11,9 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import _lib
 from . import engine as E
+from . import tables as T
 from .codec import to_codes
 from .inputs import (
     allele_arrays,
@@ -67,6 +69,16 @@ class GkmSVM:
         # Score through the l-mer index of the SVs: "auto" picks the cheaper of
         # the index and the brute-force kernel per call; True/False force it
         self.use_lmer_index = "auto"
+        # Exact l-mer weight table for non-RBF kernels (tables.py): "auto"
+        # builds it when a call queries enough windows to pay for building it
+        # (_plan_lmer_table) and uses it once built; True/False force it on or
+        # off (svm.py:216-220)
+        self.use_lmer_table: bool | str = "auto"
+        # GkmExplain attributions of every l-mer window for non-RBF kernels,
+        # one table per mode (hypothetical/importance, perturbation): "auto"
+        # builds one when a call explains enough sequences and it fits
+        # EXPLAIN_TABLE_BYTES; True/False force it on or off (svm.py:224-228)
+        self.use_explain_table: bool | str = "auto"
         self._device = None
         self._kp = None
         self._sv = None
@@ -99,7 +111,8 @@ class GkmSVM:
         return "cpu" if self._device is None else "rocm"
 
     def clear_caches(self) -> None:
-        """Drop the resident SV tables (rebuilt on the next call)."""
+        """Drop the resident SV tables, l-mer index, l-mer table and explain
+        tables (rebuilt on the next call; svm.py:241-251)."""
         self._sv = None
 
     def _require_device(self):
@@ -132,6 +145,137 @@ class GkmSVM:
     def kernel_params(self) -> dict:
         return dict(self._kernel_params)
 
+    # -- exact l-mer tables ---------------------------------------------------------
+
+    def _build_lmer_table(self) -> torch.Tensor | None:
+        """The score table w(u) = sum_sv c_sv K_raw(u, sv) on the GPU (cached),
+        or None when the kernel has none (RBF, l > MAX_TABLE_L); svm.py:485-506."""
+        sv = self._svs()
+        if not T.applies(sv.kp):
+            return None
+        if "score" not in sv.tables:
+            if "V" not in sv.tables:
+                sv.tables["V"] = T.lmer_weights(sv)
+            sv.tables["score"] = T.score_table(sv, sv.tables["V"])
+        return sv.tables["score"]
+
+    def _plan_lmer_table(self, n_windows: float) -> None:
+        """Under "auto", build the l-mer table if querying n_windows windows
+        through the per-SV paths would take longer than building it
+        (svm.py:508-536, with MI355X costs)."""
+        if self.use_lmer_table != "auto":
+            return
+        sv = self._svs()
+        if "score" in sv.tables or not T.applies(sv.kp):
+            return
+        if sv.query_seconds(n_windows) >= T.build_seconds(sv):
+            self._build_lmer_table()
+
+    def _lmer_table_for(self) -> torch.Tensor | None:
+        """The l-mer table if it applies and is on: built now when forced,
+        used under "auto" only once built (svm.py:538-557)."""
+        use = self.use_lmer_table
+        if use is False:
+            return None
+        sv = self._svs()
+        if use == "auto":
+            return sv.tables.get("score")
+        return self._build_lmer_table()
+
+    def _explain_table_for(self, n_rows: int, x_lengths, perturbation: bool):
+        """The explain table of this mode, built if use_explain_table calls for
+        it (explain/table.py:150-169), or None."""
+        use = self.use_explain_table
+        sv = self._svs()
+        kp = sv.kp
+        if use is False or not T.explain_applies(kp):
+            return None
+        key = ("explain", bool(perturbation))
+        if key not in sv.tables:
+            if use == "auto":
+                if T.explain_table_bytes(kp.l) > T.EXPLAIN_TABLE_BYTES:
+                    return None
+                from .explain import SWEEP_COMPARISONS_PER_S, explain_tables
+
+                norient = 2 if kp.rc else 1
+                sweep_s = E.comparisons(kp, x_lengths, sv.seqs.lengths, norient) \
+                    / SWEEP_COMPARISONS_PER_S
+                if sweep_s < T.explain_build_seconds(sv):
+                    return None
+            from .explain import explain_tables
+
+            ta, tb, _ = explain_tables(kp.base, perturbation)
+            if "V" not in sv.tables:
+                sv.tables["V"] = T.lmer_weights(sv)
+            sv.tables[key] = T.explain_table(sv, ta, tb, sv.tables["V"])
+        return sv.tables[key]
+
+    def to_deltasvm(self, *, device: str = "rocm", query_norm: bool = False,
+                    verbose: bool = False):
+        """A DeltaSVM with one weight per l-mer: the model's kernel sum on that
+        l-mer alone (svm.py:707-753).  Kernels with an l-mer table read it off
+        the table; RBF kernels and l > MAX_TABLE_L score every l-mer through
+        the kernel.  Without query_norm the weights drop the l-mer's own
+        normalisation 1 / sqrt(K(u, u)); RBF kernels always keep it."""
+        from .deltasvm import DeltaSVM
+
+        kp = self._params()
+        l = kp.l
+        if l > self._MAX_DELTASVM_L:
+            raise ValueError(
+                f"to_deltasvm() requires l <= {self._MAX_DELTASVM_L} "
+                f"(4^{l} = {4**l:,} weights would use {4**l * 4 / 1e9:.1f} GB). Got l={l}."
+            )
+        table = self._build_lmer_table()
+        if table is not None:
+            wt = kp.base.weight_table(1)
+            wt0 = 1.0 if wt is None else float(wt[0])
+            out = torch.empty(1 << (2 * l), dtype=torch.float64, device=kp.device)
+            h = torch.tensor(kp.h[: l + 1], dtype=torch.float64, device=kp.device)
+            _lib.call("gkm_deltasvm_weights", E._p(table), l, wt0, E._p(h), int(kp.rc),
+                      int(query_norm and kp.normalize), E._p(out), E.stream_handle())
+            weights = out.cpu().numpy()
+        else:
+            weights = self._deltasvm_weights_from_kernel(query_norm, verbose)
+        return DeltaSVM(weights, l, l, include_rc=False, bias=self.bias, device=device)
+
+    _MAX_DELTASVM_L = 14
+    _DELTASVM_CHUNK = 65536
+
+    def _deltasvm_weights_from_kernel(self, query_norm: bool, verbose: bool) -> np.ndarray:
+        """Weights by scoring every l-mer (DeltaSVM index order); with RC only
+        canonical l-mers are scored (svm.py:758-785)."""
+        kp = self._params()
+        unnorm = not query_norm and kp.normalize and not kp.rbf
+        l = kp.l
+        idx_all = np.arange(4**l, dtype=np.int64)
+        powers = 4 ** np.arange(l - 1, -1, -1)
+        digits = (idx_all[:, None] // powers) % 4
+        rc = ((3 - digits[:, ::-1]) * powers).sum(axis=1)
+        todo = idx_all[idx_all <= rc] if kp.rc else idx_all
+        weights = np.empty(4**l, dtype=np.float64)
+        starts = range(0, len(todo), self._DELTASVM_CHUNK)
+        if verbose:
+            from tqdm import tqdm
+
+            starts = tqdm(starts, desc="to_deltasvm")
+        sv = self._svs()
+        for start in starts:
+            idx = todo[start:start + self._DELTASVM_CHUNK]
+            codes = np.ascontiguousarray(digits[idx].astype(np.uint8))
+            lens = np.full(len(idx), l, dtype=np.int64)
+            xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+            out = torch.empty(len(idx), dtype=torch.float64, device=kp.device)
+            E.score_rows(sv, xs, 0.0, out, use_index=self.use_lmer_index)
+            w = out.cpu().numpy()
+            if unnorm:
+                diag = E.self_kraw(kp, xs).cpu().numpy()
+                w = w * np.sqrt(np.clip(diag, 1e-10, None))
+            weights[idx] = w
+        if kp.rc:
+            weights[rc[todo]] = weights[todo]
+        return weights
+
     # -- scoring ------------------------------------------------------------------
 
     def _score_codes(self, codes: np.ndarray, lengths: np.ndarray) -> torch.Tensor:
@@ -139,7 +283,8 @@ class GkmSVM:
         sv = self._svs()
         xs = E.DeviceSeqs(codes, lengths, self._device, rc=self._params().rc)
         out = torch.empty(codes.shape[0], dtype=torch.float64, device=self._device)
-        return E.score_rows(sv, xs, self.bias, out, use_index=self.use_lmer_index)
+        return E.score_rows(sv, xs, self.bias, out, use_index=self.use_lmer_index,
+                            table=self._lmer_table_for())
 
     def __call__(self, x, *, lengths=None, batch_size: int | None = None,
                  verbose: bool = False) -> torch.Tensor:
@@ -158,6 +303,7 @@ class GkmSVM:
             return nan_rows(torch.zeros((B, 1), dtype=torch.float64, device=self._device), short)
         if codes.shape[1] < l:
             raise ValueError(f"Sequence length {codes.shape[1]} is shorter than window length {l}")
+        self._plan_lmer_table(float(np.clip(lens - l + 1, 0, None).sum()))
         bs = B if batch_size is None else max(1, int(batch_size))
         parts = []
         starts = range(0, B, bs)

@@ -1,6 +1,7 @@
 /*
  * gkmhip.h -- C-ABI of the MI355X (gfx950) gapped k-mer engine
- * (libgkmhip.so, built from gkmsvm-lite_amd/csrc/gkm_kernels.hip).
+ * (libgkmhip.so, built from gkmsvm-lite_amd/csrc/gkm_kernels.hip and
+ * gkm_tables.hip).
  *
  * Every pointer argument is a DEVICE pointer owned by the caller (in the
  * Python package: torch tensors on the ROCm device), except where noted.
@@ -157,6 +158,79 @@ int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xle
                    const uint8_t *wtab, int S, int tile, int n_tiles, int chunk_windows,
                    int pack_counts, int64_t col_min, int64_t row_base, void *partial, void *hist,
                    void *stream);
+
+/* ---- Exact l-mer tables of non-RBF models (gkm_tables.hip) ----------------
+ * V, table  [4^l] float64 indexed by the packed l-mer (base k in bits 2k).
+ * A         [4^l][l][4] float64 explain table: window u, digit k, delta d.
+ * l <= 15 for every table entry point (memory: 8 * 4^l bytes per table,
+ * 32 * l * 4^l for the explain table). */
+
+/* V[v] = sum over entries e of the CSR bucket of l-mer v (offsets [4^l + 1]
+ * int32, svid [n] int32 SV of each entry, vals [n] uint8 window weights or
+ * NULL) of c[svid[e]] * vals[e].  Buckets hold their entries in SV order, so
+ * V is deterministic.  sv_lmer_weights (lmer_table.py:35-57),
+ * _explain_V (explain/common.py:87-96). */
+int gkm_lmer_weights(const int32_t *offsets, const int32_t *svid, const uint8_t *vals,
+                     const double *c, int l, double *V, void *stream);
+
+/* In-place unnormalised radix-4 Walsh-Hadamard transform down the rows of a
+ * [4^l][C] float64 array (C columns at once).  _wht4, _wht4_rows
+ * (explain/table.py:41-75). */
+int gkm_wht4(double *a, int l, int C, void *stream);
+
+/* f[y] *= lam[m(y)] * scale, m(y) the number of non-zero 2-bit digits of y
+ * (lam [l + 1]): the spectrum of the pair weight h[m].  With two gkm_wht4
+ * calls around it this builds the score table w(u) = sum_v V[v] h[m(u, v)]
+ * (build_lmer_table, lmer_table.py:60-98, by transforms instead of masked
+ * marginals). */
+int gkm_table_spectrum(double *f, int l, const double *lam, double scale, void *stream);
+
+/* A[y][k][d] = fv[y] * scale * chi(y_k, d) * lam_d(m(y) - [y_k != 0]) with
+ * chi(a, b) = (-1)^popcount(a & b), lam_0 = lamA, lam_1..3 = lamB ([l]
+ * each); gkm_wht4 over the 4l columns then gives the explain table.
+ * _fill_products, _build_explain_table (explain/table.py:77-113). */
+int gkm_explain_fill(const double *fv, int l, const double *lamA, const double *lamB,
+                     double scale, double *A, void *stream);
+
+/* out[b] = sum_i wt_i table[xwin[b][i]] (over the row's xlen[b] - l + 1
+ * windows; wt_i from wtab or 1), divided by sqrt(max(dx[b], 1e-10)) when
+ * normalize, plus bias.  _table_raw, _score_lmer_table (svm.py:559-582). */
+int gkm_table_score(const uint32_t *xwin, const int32_t *xlen, int B, int Wx, int l,
+                    const uint8_t *wtab, const double *table, const double *dx, int normalize,
+                    double bias, double *out, void *stream);
+
+/* GkmExplain from the explain table into out [B][4][lmax] (every entry
+ * written; 0 past a row): hypothetical (perturbation = 0) or perturbation
+ * attributions, scaled by 1 / sqrt(max(dx[b], 1e-20)) when normalize.
+ * B <= 65535.  _explain_table_cpu (explain/table.py:116-147). */
+int gkm_table_explain(const uint32_t *xwin, const int32_t *xlen, int B, int Wx, int lmax, int l,
+                      const uint8_t *wtab, const double *A, const double *dx, int normalize,
+                      int perturbation, double *out, void *stream);
+
+/* ISM deltas from the score table into out [B][4][lmax] (every entry
+ * written): raw [B] = unnormalised table sums of the rows (gkm_table_score
+ * with normalize 0, bias 0), codes [B][lmax] the reference bases, dx [B] and
+ * dmut [4][lmax][B] (gkm_mutant_diag) the self-kernels when normalize.
+ * B <= 65535.  _ism_table (ism.py:118-144). */
+int gkm_table_ism(const uint32_t *xwin, const int32_t *xlen, const uint8_t *codes, int B, int Wx,
+                  int lmax, int l, const uint8_t *wtab, const double *table, const double *raw,
+                  const double *dx, const double *dmut, int normalize, double *out, void *stream);
+
+/* DeltaSVM weights [4^l] (first base most significant) from the score table:
+ * table[u] * wt0, divided by sqrt(max((h[0] + rc * h[m(u, rc u)]) wt0^2,
+ * 1e-10)) when query_norm (h [l + 1]).  _deltasvm_weights_from_table,
+ * _lmer_self_kernels (svm.py:667-705). */
+int gkm_deltasvm_weights(const double *table, int l, double wt0, const double *h, int rc,
+                         int query_norm, double *out, void *stream);
+
+/* DeltaSVM scores out[b] = sum over the row's windows and the gapped k-mer
+ * combos [n_combos][k] (int32 positions within a window of l) of
+ * weights[k-mer] (first base most significant), plus the same over the
+ * reverse complement when rc, plus bias.  k <= 15.
+ * _score_strand_numba, DeltaSVM.__call__ (deltasvm.py:22-44, 107-147). */
+int gkm_deltasvm_score(const uint8_t *codes, const int32_t *xlen, int B, int lmax, int l, int k,
+                       const int32_t *combos, int n_combos, const double *weights, int rc,
+                       double bias, double *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -449,6 +449,164 @@ def impact(model: Model, ref_codes, ref_len, alt_codes, alt_len) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# Exact l-mer tables (lmer_table.py, explain/table.py) and DeltaSVM
+# ---------------------------------------------------------------------------
+
+
+def lmer_weights(model: Model) -> np.ndarray:
+    """V [4^l] (lmer_table.py:35-57, sv_lmer_weights): per l-mer, the sum over
+    SV windows (forward and RC, each with its window weight) of
+    coef / sqrt(max(K_raw(sv, sv), 1e-10)) (svm.py:497-502)."""
+    kern = model.kern
+    n = 4 ** kern.l
+    c = model.coef / np.sqrt(np.clip(model.sv_diag, 1e-10, None))
+    sv = model.sv
+    V = np.zeros(n)
+    for packed, w in ((sv.fwd, sv.w), (sv.rc, sv.w_rc)):
+        if packed is None:
+            continue
+        wt = c[:, None] if w is None else c[:, None] * w.astype(np.float64)
+        keys = packed.ravel().astype(np.int64)
+        wt = np.broadcast_to(wt, packed.shape).ravel()
+        keep = keys < n
+        V += np.bincount(keys[keep], weights=wt[keep], minlength=n)
+    return V
+
+
+def mask_plan(kern: Kern):
+    """kernels/masked.py:44-74 (mask_plan): [(j, a_j, positions)] with
+    h(m) = sum_t a_t C(l - m, l - dmax + t), or None when it does not apply."""
+    from itertools import combinations
+
+    l, dmax = kern.l, kern.dmax
+    min_matches = l - dmax
+    if dmax < 0 or min_matches < 1:
+        return None
+    js = list(range(l - dmax, l + 1))
+    A = np.array([[comb(l - m, j) for j in js] for m in range(dmax + 1)], dtype=np.float64)
+    a = np.linalg.solve(A, kern.table[: dmax + 1])
+    tol = 1e-12 * np.abs(a).max()
+    return [(j, float(c), list(combinations(range(l), j))) for j, c in zip(js, a)
+            if abs(c) > tol]
+
+
+def lmer_table(model: Model) -> np.ndarray:
+    """w [4^l], w(u) = sum_sv c_sv K_raw(u, sv), by the masked marginals of
+    lmer_table.py:60-98 (build_lmer_table; without the RC pairing shortcut)."""
+    kern = model.kern
+    l = kern.l
+    plan = mask_plan(kern)
+    if plan is None:
+        raise ValueError("no mask plan for this kernel")
+    V = lmer_weights(model).reshape((4,) * l)  # axis a holds position l - 1 - a
+    w = np.zeros_like(V)
+    for _, coef, positions in plan:
+        for S in positions:
+            keep = {l - 1 - int(p) for p in S}
+            drop = tuple(a for a in range(l) if a not in keep)
+            w += coef * (V.sum(axis=drop, keepdims=True) if drop else V)
+    return w.ravel()
+
+
+def digit_mismatches(l: int) -> np.ndarray:
+    """[4^l, 4^l] m(u, v) for every pair of packed l-mers (small l only)."""
+    z = np.arange(4 ** l, dtype=np.int64)
+    x = z[:, None] ^ z[None, :]
+    m = np.zeros(x.shape, dtype=np.int64)
+    for k in range(l):
+        m += ((x >> (2 * k)) & 3) != 0
+    return m
+
+
+def lmer_table_brute(model: Model) -> np.ndarray:
+    """w(u) = sum_v V[v] h[m(u, v)] over all pairs of l-mers (l <= 6)."""
+    l = model.kern.l
+    h = np.concatenate([model.kern.table, np.zeros(2)])
+    return h[digit_mismatches(l)] @ lmer_weights(model)
+
+
+def explain_table_brute(model: Model, perturbation: bool) -> np.ndarray:
+    """A [4^l, l, 4]: A[u, k, d] = sum_v V[v] [(u ^ v)_k == d] t_d(m(u, v)),
+    t_0 = table_a, t_1..3 = table_b (the definition explain/table.py:14-26
+    states; l <= 5)."""
+    l = model.kern.l
+    ta, tb, _ = explain_tables(model.kern, perturbation)
+    V = lmer_weights(model)
+    m = digit_mismatches(l)
+    z = np.arange(4 ** l, dtype=np.int64)
+    x = z[:, None] ^ z[None, :]
+    A = np.zeros((4 ** l, l, 4))
+    for k in range(l):
+        xk = (x >> (2 * k)) & 3
+        for d in range(4):
+            t = ta if d == 0 else tb
+            A[:, k, d] = np.where(xk == d, t[m], 0.0) @ V
+    return A
+
+
+def score_with_table(model: Model, w: np.ndarray, codes, lengths) -> np.ndarray:
+    """[B] f(x) = sum_i wt_i w(x_i) / sqrt(max(K_raw(x, x), 1e-10)) + b
+    (svm.py:559-582, _table_raw / _score_lmer_table); NaN for short rows."""
+    kern = model.kern
+    px = pack_both(kern, codes, lengths)
+    dx = combine(kern, self_hist(kern, px))
+    keys = px.fwd.astype(np.int64)
+    pad = keys >= 4 ** kern.l
+    vals = np.where(pad, 0.0, w[np.where(pad, 0, keys)])
+    if px.w is not None:
+        vals = vals * px.w
+    raw_ = vals.sum(axis=1)
+    out = raw_ / np.sqrt(np.clip(dx, 1e-10, None)) + model.bias
+    return np.where(np.asarray(lengths) < kern.l, np.nan, out)
+
+
+def deltasvm_weights(model: Model, w: np.ndarray, query_norm: bool) -> np.ndarray:
+    """svm.py:667-705: the table in DeltaSVM order (first base most
+    significant), times the one-window weight, divided by sqrt(K_raw(u, u))
+    with query_norm."""
+    kern = model.kern
+    l = kern.l
+    out = w.reshape((4,) * l).transpose(tuple(range(l - 1, -1, -1))).ravel().copy()
+    wt0 = 1.0 if not kern.weighted else float(window_weights(1, kern.M, kern.H)[0])
+    out *= wt0
+    if query_norm:
+        idx = np.arange(4 ** l)
+        digits = (idx[:, None] // 4 ** np.arange(l - 1, -1, -1)) % 4
+        agree = digits == 3 - digits[:, ::-1]
+        h = kern.table
+        selfk = np.full(len(idx), h[0])
+        if kern.rc:
+            selfk = selfk + h[(~agree).sum(axis=1)]
+        out = out / np.sqrt(np.clip(selfk * wt0 ** 2, 1e-10, None))
+    return out
+
+
+def deltasvm_score(weights, l: int, k: int, codes, lengths, include_rc=True,
+                   bias=0.0) -> np.ndarray:
+    """[B] DeltaSVM scores, deltasvm.py:22-44 (_score_strand_numba) per strand
+    at each row's own length, the RC strand 3 - x reversed (deltasvm.py:107-147)."""
+    from itertools import combinations
+
+    combos = list(combinations(range(l), k))
+    out = np.zeros(len(lengths))
+    for b, n in enumerate(np.asarray(lengths)):
+        x = np.asarray(codes[b, :n], dtype=np.int64)
+        strands = [x] + ([3 - x[::-1]] if include_rc else [])
+        tot = 0.0
+        for s in strands:
+            acc = 0.0
+            for i in range(len(s) - l + 1):
+                for c in combos:
+                    idx = 0
+                    for j in c:
+                        idx = idx * 4 + int(s[i + j])
+                    acc += weights[idx]
+            tot += acc
+        out[b] = tot + bias
+    return out
+
+
+# ---------------------------------------------------------------------------
 # The reference's CPU scoring algorithm (l-mer index + fused reduce)
 # ---------------------------------------------------------------------------
 

@@ -33,12 +33,15 @@ def _queries(n, lo, hi, seed):
 
 
 @pytest.mark.parametrize("kt,extra", [(2, {}), (4, {"M": 50, "H": 50.0}), (3, {"gamma": 2.0})])
-def test_batch_scorer_matches_plain_calls(gpu, kt, extra):
+@pytest.mark.parametrize("table", [False, True])
+def test_batch_scorer_matches_plain_calls(gpu, kt, extra, table):
     """Uneven queries (8-700 bp, 1% N, some shorter than l) through buckets,
-    the autotuner and graph replays equal model(x) exactly."""
+    the autotuner (or the l-mer table) and graph replays equal model(x)
+    exactly."""
     from gkmsvm_amd.batching import BatchScorer
 
     m = _random_model(kt=kt, **extra)
+    m.use_lmer_table = table
     qs = _queries(300, 8, 700, 1)
     want = m(qs).cpu().numpy()
     sc = BatchScorer(m, rows=64, bucket=128, graphs=True, autotune=True)
@@ -52,14 +55,20 @@ def test_batch_scorer_matches_plain_calls(gpu, kt, extra):
     np.testing.assert_array_equal(got, want)
 
 
-def test_server_results_equal_alone_and_in_the_mix(gpu):
+@pytest.mark.parametrize("tables", [False, True])
+def test_server_results_equal_alone_and_in_the_mix(gpu, tables):
     """Requests submitted concurrently from several threads (scoring of many
     lengths, variants, explain), served in overlapping steps on two streams,
-    return what each gives alone."""
+    return what each gives alone (per-SV kernels, or the l-mer tables the
+    server builds at start)."""
     from gkmsvm_amd import gkmexplain
     from gkmsvm_amd.serving import GkmServer
 
     m = _random_model(S=900, seed=3)
+    if not tables:
+        m.use_lmer_table = m.use_explain_table = False
+    srv = GkmServer(m, hbm_budget=2 << 30, streams=2, tables=tables)
+    assert (srv.table is not None) == tables and srv.explain_tables == tables
     rng = np.random.default_rng(4)
     reqs = []
     for i in range(60):
@@ -82,7 +91,7 @@ def test_server_results_equal_alone_and_in_the_mix(gpu):
         else:
             alone.append(gkmexplain(m, payload, mode="hypothetical").cpu().numpy())
     futs = [None] * len(reqs)
-    with GkmServer(m, hbm_budget=2 << 30, streams=2) as srv:
+    with srv:
         def client(k):
             for i in range(k, len(reqs), 3):
                 kind, payload = reqs[i]
@@ -104,10 +113,11 @@ def test_server_admission_rejects_oversized_and_small_budget_still_exact(gpu):
     from gkmsvm_amd.serving import GkmServer
 
     m = _random_model(S=300, seed=5)
+    m.use_lmer_table = m.use_explain_table = False
     qs = _queries(20, 50, 400, 9)
     want = m(qs).cpu().numpy().ravel()
-    one = m._svs().S * m._params().nm * 8 + 4096
-    with GkmServer(m, hbm_budget=one * 3, streams=2) as srv:
+    one = m._svs().S * m._params().nm * 8 + 8192
+    with GkmServer(m, hbm_budget=one * 3, streams=2, tables=False) as srv:
         big = srv.submit("score", qs)  # 20 rows: over the budget
         with pytest.raises(MemoryError):
             big.result(timeout=60)

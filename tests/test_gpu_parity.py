@@ -33,11 +33,25 @@ def _params(kt, l, k, rng):
     return p
 
 
+def _per_sv(m):
+    """The model with its exact l-mer tables off: these tests hold the per-SV
+    kernels (the table paths have tests of their own)."""
+    m.use_lmer_table = False
+    m.use_explain_table = False
+    return m
+
+
+def _load(name):
+    from gkmsvm_amd import load_lsgkm_model
+
+    return _per_sv(load_lsgkm_model(H.path(name)))
+
+
 def _model(kt, p, codes, lens, coef, bias=0.0):
     from gkmsvm_amd import GkmSVM
 
     sv_l = None if (lens == codes.shape[1]).all() else lens
-    return GkmSVM(codes, coef, bias, kt, p, sv_lengths=sv_l)
+    return _per_sv(GkmSVM(codes, coef, bias, kt, p, sv_lengths=sv_l))
 
 
 def _oracle_model(m):
@@ -116,9 +130,7 @@ def test_upper_triangle_columns(gpu, seed):
 def test_scores_lsgkm_fixtures(gpu, kind, n):
     """-t 0..6 fixture models: scores equal the oracle to 1e-10 and gkmpredict
     to its printed digits."""
-    from gkmsvm_amd import load_lsgkm_model
-
-    m = load_lsgkm_model(H.path(f"lsgkm_wgkm_{kind}.model.txt.gz"))
+    m = _load(f"lsgkm_wgkm_{kind}.model.txt.gz")
     recs = O.read_fasta(H.path(f"lsgkm_wgkm_test{n}.fa"))
     seqs = [s for _, s in recs]
     got = m(seqs).cpu().numpy().ravel()
@@ -133,9 +145,7 @@ def test_scores_lsgkm_fixtures(gpu, kind, n):
 @pytest.mark.parametrize("kind", ("t2", "t4", "t5"))
 def test_ragged_models_gkmpredict(gpu, kind):
     """Models trained on 60-140 bp sequences (ragged SVs) on 30-200 bp queries."""
-    from gkmsvm_amd import load_lsgkm_model
-
-    m = load_lsgkm_model(H.path(f"lsgkm_ragged_{kind}.model.txt.gz"))
+    m = _load(f"lsgkm_ragged_{kind}.model.txt.gz")
     recs = O.read_fasta(H.path("lsgkm_ragged_test.fa"))
     seqs = [s for _, s in recs]
     got = m(seqs).cpu().numpy().ravel()
@@ -152,9 +162,7 @@ def test_encode_10sv_gkmpredict(gpu):
     import json
     import random
 
-    from gkmsvm_amd import load_lsgkm_model
-
-    m = load_lsgkm_model(H.path("encode_ENCFF579AOX_10sv.model.txt.gz"))
+    m = _load("encode_ENCFF579AOX_10sv.model.txt.gz")
     gold = json.load(open(H.path("encode_10sv_gkmpredict.json")))["scores"]
     random.seed(42)
     rand = "".join(random.choice("ACGT") for _ in range(300))
@@ -173,10 +181,10 @@ def test_encode_10sv_gkmpredict(gpu):
 def test_gkmexplain_fixtures(gpu, name, mode, lm):
     """GkmExplain equals the oracle's per-SV loop to 1e-10 and the kundajelab
     lsgkm gkmexplain output at its printed precision."""
-    from gkmsvm_amd import gkmexplain, load_lsgkm_model
+    from gkmsvm_amd import gkmexplain
 
     mf, fa, pre = H.EXPLAIN_SETS[name]
-    m = load_lsgkm_model(H.path(mf))
+    m = _load(mf)
     seqs = [s for _, s in O.read_fasta(H.path(fa))]
     got = gkmexplain(m, seqs, mode=mode, batch_size=8).cpu().numpy()
     om = _oracle_model(m)
@@ -191,10 +199,8 @@ def test_gkmexplain_fixtures(gpu, name, mode, lm):
 @pytest.mark.parametrize("name", sorted(k for k in H.EXPLAIN_SETS if not k.endswith("_51")))
 def test_mutation_impact_fixtures(gpu, name):
     """score_variants(method='gkmexplain') equals the oracle and gkmexplain -m 5."""
-    from gkmsvm_amd import load_lsgkm_model
-
     mf, fa, pre = H.EXPLAIN_SETS[name]
-    m = load_lsgkm_model(H.path(mf))
+    m = _load(mf)
     seqs = [s for _, s in O.read_fasta(H.path(fa))]
     ref, alt = H.mode5_pairs(seqs, fa)
     got = m.score_variants(ref, alt, method="gkmexplain", batch_size=16).cpu().numpy().ravel()
@@ -213,9 +219,9 @@ def test_mutation_impact_fixtures(gpu, name):
 def test_ism_matches_rescoring(gpu, mf):
     """ISM equals f(mutant) - f(x) with every mutant scored from scratch by the
     oracle (ragged queries, N bases, lengths with odd and even window counts)."""
-    from gkmsvm_amd import ism, load_lsgkm_model
+    from gkmsvm_amd import ism
 
-    m = load_lsgkm_model(H.path(mf))
+    m = _load(mf)
     seqs = [s for _, s in O.read_fasta(H.path("lsgkm_ragged_query.fa"))][:5]
     got = ism(m, seqs).cpu().numpy()
     om = _oracle_model(m)
@@ -229,9 +235,7 @@ def test_ism_matches_rescoring(gpu, mf):
 
 
 def test_score_variants_kernel(gpu):
-    from gkmsvm_amd import load_lsgkm_model
-
-    m = load_lsgkm_model(H.path("lsgkm_mode5_rbf.model.txt.gz"))
+    m = _load("lsgkm_mode5_rbf.model.txt.gz")
     seqs = [s for _, s in O.read_fasta(H.path("lsgkm_mode5_test51.fa"))]
     ref, alt = H.mode5_pairs(seqs, "x")
     got = m.score_variants(ref, alt).cpu().numpy().ravel()
@@ -282,9 +286,7 @@ def test_plain_scoring_config_against_index_oracle(gpu):
 
 def test_short_and_n_rows(gpu):
     """Rows shorter than l score NaN; N bases score as A (RC: T)."""
-    from gkmsvm_amd import load_lsgkm_model
-
-    m = load_lsgkm_model(H.path("lsgkm_mode5_est.model.txt.gz"))
+    m = _load("lsgkm_mode5_est.model.txt.gz")
     seqs = ["ACGT", "ACGTNNNNNACGTACGGATCGATTTACGA" * 3, "NNNNNNNNNNNNNNNNNNNN"]
     got = m(seqs).cpu().numpy().ravel()
     assert np.isnan(got[0]) and np.isfinite(got[1:]).all()
