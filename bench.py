@@ -4,27 +4,36 @@ Workload (the serving configuration): a 72,000-SV -t 2 model (l=11 k=7 d=3)
 of 500 bp uniform-iid ACGT support vectors with sign-balanced N(0,1)
 coefficients, resident in HBM; a Poisson request stream (seeded) mixing
 scoring requests (one query, 50-2000 bp), SNV ref/alt pairs (300 bp,
-score_variants) and GkmExplain calls (1 kb), admitted by an HBM budget.
+score_variants) and GkmExplain calls (1 kb, hypothetical), admitted by an
+HBM budget.
 
 A step is one burst of --requests requests served through the continuous-
 batching server (gkmsvm_amd.serving.GkmServer) as they arrive.  value is
-query x SV kernel evaluations per second over the timed steps (scoring row =
-S evaluations, an SNV pair = 2 S, an explain call = S); p50/p99 are request
-latencies from arrival to result.  The plain path -- the same requests, one
-API call each (model(x), model.score_variants, gkmexplain) -- is timed
-beside it.
+query x SV kernel evaluations per second over the timed steps (a scoring row
+counts S evaluations, an SNV pair 2 S, an explain call S); p50/p99 are request
+latencies from arrival to result.  The server builds the model's exact l-mer
+tables when it starts (non-RBF model: scoring and variant rows cost one table
+lookup per window, explain rows l lookups per position; the build time is
+reported).  Beside it: the same stream served through the per-SV kernels
+(pairwise kernel / l-mer index / explain sweep, tables off), and the plain path
+-- the same requests, one API call each (model(x), model.score_variants,
+gkmexplain) -- on the per-SV kernels (what the model's "auto" planning picks
+for one request at a time) and with the tables prebuilt.
 
-roofline: every launch of the hot kernels (k_cross_hist, k_index_tile,
-k_sweep) is timed with HIP events on the stream it runs on during the timed
-steps; the object describes the kernel with the most GPU time (the others
-under "others").  Brute-force kernels: achieved = window-pair comparisons x 6
-integer operations (XOR, shift, OR, AND, popcount, accumulate: the direct
-method's count per comparison) / launch time, against the int32 VALU peak.
-The l-mer index: algorithmic bytes (8 per probe, 2-3 per entry) / launch
-time, against HBM.
-cpu_baseline: the reference's CPU scoring algorithm (l-mer index + fused
-reduce), restated in C with OpenMP (the oracle), on a bounded sample of the
-stream's scoring queries.
+roofline: every launch of the hot kernels is timed with HIP events on the
+stream it runs on during the timed steps; the object describes the kernel
+with the most GPU time in the value run (the others under "others"; the
+per-SV run's under "kernel_path").  Pairwise kernels (k_cross_hist,
+k_self_hist, k_sweep): window-pair comparisons x 6 integer operations (XOR,
+shift, OR, AND, popcount, accumulate) / launch time, against the int32 VALU
+peak.  HBM-bound kernels (k_index_tile: 8 B offsets per probe + 2-3 B per
+entry; k_table_score: 4 B window + 8 B table entry per window;
+k_table_explain: 4 B window + 32 l B table row per window + 32 B output per
+position) / launch time, against 8 TB/s.
+cpu_baseline: the reference's CPU algorithm on the host cores (the oracle, C
+with OpenMP + numpy) on a bounded sample of the stream's scoring queries: the
+l-mer table (svm.py:544-582) when the GPU value used it, with the l-mer index
++ fused reduce rate beside it.
 """
 
 from __future__ import annotations
@@ -131,41 +140,59 @@ def run_served(server, reqs, rate, rng):
 
 
 def cpu_baseline(sv, coef, reqs, seconds):
-    """The reference's CPU algorithm (oracle: C + OpenMP) on a sample of the
-    stream's scoring queries: query x SV evaluations per second."""
+    """The reference's CPU algorithm (oracle: C + OpenMP, numpy) on a sample of
+    the stream's scoring queries: query x SV evaluations per second through the
+    exact l-mer table, and through the l-mer index + fused reduce."""
     from oracle import oracle as O
 
-    t = time.perf_counter()
     om = O.Model(O.Kern.make(2, PARAMS), sv, np.full(len(coef), SV_LEN), coef, 0.0)
-    index = O.LmerIndex(om)
-    build_s = time.perf_counter() - t
     queries = [p[0] for k, p in reqs if k == "score"]
     nt = O.threads()
-    done, t0 = 0, time.perf_counter()
-    lens = []
-    # one query per thread per call (the reduce runs a row per thread), until
-    # the time budget is spent
-    for i in range(0, len(queries), nt):
-        chunk = queries[i:i + nt]
-        codes, ln = O.batch(chunk)
-        index.score(codes, ln)
-        done += len(chunk)
-        lens += [len(q) for q in chunk]
-        if time.perf_counter() - t0 > seconds:
-            break
-    el = time.perf_counter() - t0
+
+    def run(score_fn, budget):
+        done, lens, t0 = 0, [], time.perf_counter()
+        for i in range(0, len(queries), nt):
+            chunk = queries[i:i + nt]
+            codes, ln = O.batch(chunk)
+            score_fn(codes, ln)
+            done += len(chunk)
+            lens += [len(q) for q in chunk]
+            if time.perf_counter() - t0 > budget:
+                break
+        return done * len(coef) / (time.perf_counter() - t0), done, float(np.mean(lens))
+
+    t = time.perf_counter()
+    w = O.lmer_table(om)
+    table_build_s = time.perf_counter() - t
+    v_table, n_table, mean_table = run(lambda c, ln: O.score_with_table(om, w, c, ln),
+                                       seconds / 2)
+    t = time.perf_counter()
+    index = O.LmerIndex(om)
+    index_build_s = time.perf_counter() - t
+    v_index, n_index, mean_index = run(index.score, seconds / 2)
     return {
-        "value": done * len(coef) / el,
+        "value": v_table,
         "unit": "query x SV kernel evals/s",
-        "cores": O.threads(),
+        "cores": nt,
         "kind": "port",
-        "sample": (f"{done} scoring queries of the request stream (mean {np.mean(lens):.0f} bp) "
-                   f"against the 72,000-SV index, l-mer index + fused reduce in C/OpenMP; "
-                   f"index build {build_s:.1f} s not counted"),
+        "sample": (f"{n_table} scoring queries of the request stream (mean {mean_table:.0f} bp) "
+                   f"scored through the exact l-mer table (lookups in numpy, query self-kernel "
+                   f"in C/OpenMP; table build {table_build_s:.1f} s by masked marginals, not "
+                   f"counted); the l-mer index + fused reduce (C/OpenMP) scores "
+                   f"{v_index:.3g} evals/s on {n_index} of them (index build "
+                   f"{index_build_s:.1f} s, not counted)"),
+        "index_path_value": v_index,
     }
 
 
 HBM_PEAK = 8.0e12  # bytes/s
+
+
+HBM_KINDS = {
+    "k_index_tile": "8 B offsets per (window, mask, SV tile) probe + 2 B id (+1 B weight) per entry",
+    "k_table_score": "4 B packed window + 8 B table entry per query window",
+    "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",
+}
 
 
 def roofline_of(summary: dict) -> dict:
@@ -176,16 +203,16 @@ def roofline_of(summary: dict) -> dict:
     rows = {}
     for kind, d in summary.items():
         per_s = d["units_per_launch"] / (d["mean_ms"] * 1e-3)
-        if kind == "k_index_tile":
-            r = {"bound": "hbm", "achieved": per_s, "peak": HBM_PEAK, "unit": "B/s",
-                 "frac": per_s / HBM_PEAK,
-                 "work": "8 B offsets per (window, mask, SV tile) probe + 2 B id per entry"}
+        if kind in HBM_KINDS:
+            r = {"bound": "hbm", "achieved": per_s / 1e9, "peak": HBM_PEAK / 1e9,
+                 "unit": "GB/s", "frac": per_s / HBM_PEAK, "work": HBM_KINDS[kind],
+                 "bytes_per_launch": d["units_per_launch"]}
         else:
             ops = per_s * OPS_PER_COMPARISON
-            r = {"bound": "valu-int32", "achieved": ops, "peak": INT_VALU_PEAK,
-                 "unit": "int32 op/s", "frac": ops / INT_VALU_PEAK,
-                 "comparisons_per_s": per_s,
-                 "work": "6 int ops per window-pair comparison (XOR, SHR, OR, AND, POPC, add)"}
+            r = {"bound": "valu-int32", "achieved": ops / 1e12, "peak": INT_VALU_PEAK / 1e12,
+                 "unit": "TOP/s", "frac": ops / INT_VALU_PEAK, "comparisons_per_s": per_s,
+                 "comparisons_per_launch": d["units_per_launch"],
+                 "work": "6 int32 ops per window-pair comparison (XOR, SHR, OR, AND, POPC, add)"}
         r.update({"kernel": kind, "launches": d["launches"], "mean_launch_ms": d["mean_ms"],
                   "total_ms": d["total_ms"], "traffic": None})
         rows[kind] = r
@@ -262,10 +289,17 @@ def config_runs(args):
     coef = np.abs(np.random.default_rng(2).normal(size=500))
     coef[250:] *= -1
     q = np.random.default_rng(3).integers(0, 4, (1000, 300)).astype(np.uint8)
-    m = GkmSVM(sv, coef, 0.0, 2, p)
-    t, _ = timed(lambda: m(q).cpu(), 5)
-    print(json.dumps({"config": "plain -t 2, 500 SVs x 300 bp, 1,000 x 300 bp queries",
-                      "ms": t * 1e3, "pairs_per_s": 500 * 1000 / t}), flush=True)
+    for use_table in (False, True):
+        m = GkmSVM(sv, coef, 0.0, 2, p)
+        m.use_lmer_table = use_table
+        t_first = time.perf_counter()
+        m(q).cpu()
+        t_first = time.perf_counter() - t_first
+        t, _ = timed(lambda: m(q).cpu(), 5)
+        print(json.dumps({"config": "plain -t 2, 500 SVs x 300 bp, 1,000 x 300 bp queries",
+                          "path": "l-mer table" if use_table else "pairwise kernel",
+                          "ms": t * 1e3, "first_call_ms": t_first * 1e3,
+                          "pairs_per_s": 500 * 1000 / t}), flush=True)
     # Gram: 5,000 pos (planted 12-bp PWM at 1-3 sites) + 5,000 neg x 500 bp, GC 0.42
     rng = np.random.default_rng(4)
     X = _gc_seqs(rng, 10_000, 500)
@@ -290,13 +324,23 @@ def config_runs(args):
     nq = args.c3_queries
     ql = rng.integers(100, 1001, nq)
     qs = ["".join("ACGTN"[c] for c in rng.choice(5, n, p=[.2475] * 4 + [.01])) for n in ql]
-    sc = BatchScorer(m3, rows=256, bucket=128)
-    sc(qs[:512])  # tune + capture every bucket shape
-    t, _ = timed(lambda: sc(qs).cpu())
-    print(json.dumps({"config": f"batched -t 4, 20,000 SVs 100-1000 bp, {nq} of the 100,000 "
-                                "queries 100-1000 bp (1% N)",
-                      "s": t, "pairs_per_s": nq * 20_000 / t,
-                      "plans": {str(k): repr(v) for k, v in sc.plans.items()}}), flush=True)
+    for use_table in (False, True):
+        m3.use_lmer_table = use_table
+        sc = BatchScorer(m3, rows=256, bucket=128)
+        sc(qs[:512])  # tune + capture every bucket shape
+        t, _ = timed(lambda: sc(qs).cpu())
+        print(json.dumps({"config": f"batched -t 4, 20,000 SVs 100-1000 bp, {nq} of the "
+                                    "100,000 queries 100-1000 bp (1% N)",
+                          "path": "l-mer table" if use_table else "pairwise kernel / index",
+                          "s": t, "pairs_per_s": nq * 20_000 / t,
+                          "plans": {str(k): repr(v) for k, v in sc.plans.items()}}), flush=True)
+    if True:  # all 100,000 queries through the table
+        qa = ["".join("ACGTN"[c] for c in rng.choice(5, n, p=[.2475] * 4 + [.01]))
+              for n in rng.integers(100, 1001, 100_000)]
+        t, _ = timed(lambda: sc(qa).cpu())
+        print(json.dumps({"config": "batched -t 4, 20,000 SVs 100-1000 bp, all 100,000 queries "
+                                    "100-1000 bp (1% N)", "path": "l-mer table",
+                          "s": t, "pairs_per_s": 100_000 * 20_000 / t}), flush=True)
     # GkmExplain + ISM, -t 5 gamma 2: 50,000 SVs x 500 bp, Laplace(0, 0.5) coefficients
     rng = np.random.default_rng(8)
     sv4 = _gc_seqs(rng, 50_000, 500, 0.5)
@@ -316,10 +360,12 @@ def config_runs(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel-steps", type=int, default=6,
+                    help="timed steps of the per-SV kernel runs (served and plain)")
     ap.add_argument("--requests", type=int, default=96, help="requests per step")
-    ap.add_argument("--rate", type=float, default=4000.0,
+    ap.add_argument("--rate", type=float, default=1e6,
                     help="Poisson arrivals per second in the throughput steps (above capacity)")
     ap.add_argument("--load", type=float, default=0.5,
                     help="arrival rate of the latency steps, as a fraction of the measured capacity")
@@ -355,46 +401,81 @@ def main():
     model.rocm()
     model._svs()
     torch.cuda.synchronize()
-    steps = [make_requests(args.requests, 100 + i) for i in range(args.warmup + args.steps)]
+    n_steps = args.warmup + max(args.steps, args.kernel_steps)
+    steps = [make_requests(args.requests, 100 + i) for i in range(n_steps)]
     rng = np.random.default_rng(5)
 
-    server = GkmServer(model, hbm_budget=16 << 30).start()
-    for i in range(args.warmup):
-        run_served(server, steps[i], args.rate, rng)
-    server.completed.clear()
-    # Throughput: requests arrive (Poisson) faster than they are served.
-    timer = E.KernelTimer()
-    E.TIMER = timer
+    def served_run(server, timed_steps):
+        for i in range(args.warmup):
+            run_served(server, steps[i], args.rate, rng)
+        server.completed.clear()
+        timer = E.KernelTimer()
+        E.TIMER = timer
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in timed_steps:
+            run_served(server, s_, args.rate, rng)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        E.TIMER = None
+        lat_sat = latency_stats(server.completed)
+        server.completed.clear()
+        return el, timer.summary(), lat_sat
+
+    # 1. The server as it runs by default: exact l-mer tables built at start.
+    t = time.perf_counter()
+    server = GkmServer(model, hbm_budget=16 << 30)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.warmup, args.warmup + args.steps):
-        run_served(server, steps[i], args.rate, rng)
-    torch.cuda.synchronize()
-    served_s = time.perf_counter() - t0
-    E.TIMER = None
-    lat_sat = latency_stats(server.completed)
-    server.completed.clear()
-    timed = steps[args.warmup:]
-    n_evals = sum(evals(s) for s in timed)
+    build_s = time.perf_counter() - t
+    server.start()
+    timed = steps[args.warmup:args.warmup + args.steps]
+    served_s, summary, lat_sat = served_run(server, timed)
+    n_evals = sum(evals(s_) for s_ in timed)
     served_rate = n_evals / served_s
-    # Latency at a nominal load: arrivals at a fraction of the measured capacity.
-    cap = args.requests * args.steps / served_s
+    # latency at a nominal load: arrivals at a fraction of the measured capacity
+    cap = args.requests * len(timed) / served_s
     load_rate = args.load * cap
-    for i in range(args.warmup, args.warmup + args.lat_steps):
-        run_served(server, steps[i], load_rate, rng)
+    for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
+        run_served(server, s_, load_rate, rng)
     lat = latency_stats(server.completed)
     server.stop()
 
-    plain = None
+    # 2. The same stream through the per-SV kernels (tables off).
+    model_k = make_model()[0]
+    model_k.use_lmer_table = False
+    model_k.use_explain_table = False
+    model_k.rocm()
+    model_k._svs()
+    ktimed = steps[args.warmup:args.warmup + args.kernel_steps]
+    k_evals = sum(evals(s_) for s_ in ktimed)
+    server_k = GkmServer(model_k, hbm_budget=16 << 30, tables=False).start()
+    served_k_s, summary_k, lat_k = served_run(server_k, ktimed)
+    server_k.stop()
+    served_k = {"value": k_evals / served_k_s, "ms_per_step": served_k_s / len(ktimed) * 1e3,
+                "steps": len(ktimed), "latency_ms_at_saturation": lat_k}
+
+    plain = plain_t = None
     if not args.no_plain:
+        # 3. Plain: one API call per request on the per-SV kernels (what "auto"
+        # planning picks for single requests of these sizes).
+        run_plain(model_k, steps[0][:8])
+        t = time.perf_counter()
+        for s_ in ktimed:
+            run_plain(model_k, s_)
+        plain_s = time.perf_counter() - t
+        plain = {"value": k_evals / plain_s, "ms_per_step": plain_s / len(ktimed) * 1e3,
+                 "steps": len(ktimed)}
+        # 4. Plain calls with the tables already built.
         run_plain(model, steps[0][:8])
         t = time.perf_counter()
-        for s in timed:
-            run_plain(model, s)
-        plain_s = time.perf_counter() - t
-        plain = {"value": n_evals / plain_s, "ms_per_step": plain_s / len(timed) * 1e3}
+        for s_ in timed:
+            run_plain(model, s_)
+        plain_t_s = time.perf_counter() - t
+        plain_t = {"value": n_evals / plain_t_s, "ms_per_step": plain_t_s / len(timed) * 1e3,
+                   "steps": len(timed)}
 
-    roofline = roofline_of(timer.summary())
+    roofline = roofline_of(summary)
+    roofline["kernel_path"] = roofline_of(summary_k)
     line = {
         "metric": "query x SV kernel evals/sec (served), with p50/p99 request latency and the plain path",
         "value": served_rate,
@@ -406,7 +487,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 bit-plane popcount histograms, f64 combine",
+        "dtype": "f64 table lookups and sums; u32 bit-plane popcount histograms (self-kernels)",
         "data": "synthetic (uniform iid ACGT, seeded)",
         "config": {
             "workload": "serving: 72,000-SV -t 2 l=11 k=7 d=3 model of 500 bp SVs resident in "
@@ -414,11 +495,16 @@ def main():
                         "and gkmexplain (1 kb)",
             "requests_per_step": args.requests,
             "arrival_rate_per_s": args.rate,
+            "table_build_s": build_s,
             "latency_ms_at_saturation": lat_sat,
             "latency_load_rate_per_s": load_rate,
             "latency_ms": lat,
+            "served_kernel_path": served_k,
             "plain": plain,
+            "plain_tables_prebuilt": plain_t,
             "served_over_plain": (served_rate / plain["value"]) if plain else None,
+            "served_over_plain_tables": (served_rate / plain_t["value"]) if plain_t else None,
+            "kernel_path_served_over_plain": (served_k["value"] / plain["value"]) if plain else None,
         },
         "roofline": roofline,
     }

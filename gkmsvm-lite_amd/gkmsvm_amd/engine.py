@@ -147,7 +147,8 @@ def launch_shape(B: int, S: int, norient: int, n_cu: int = 256):
 class KernelTimer:
     """HIP-event timings of the hot kernels' launches (events recorded on the
     stream each launch goes to) with each launch's algorithmic work: window-pair
-    comparisons (k_cross_hist, k_sweep) or bytes (k_index_tile)."""
+    comparisons (k_cross_hist, k_self_hist, k_sweep) or bytes (k_index_tile,
+    k_table_score, k_table_explain)."""
 
     def __init__(self):
         self.records = []
@@ -226,8 +227,12 @@ def self_kraw(kp: KernelParams, xs: DeviceSeqs, stream=None) -> torch.Tensor:
     # enough diagonal chunks that a lone long sequence still spreads over CUs
     nd = 2 * (2 * max(longest - kp.l, 0) + 1)
     nchunks = max(1, min(64, math.ceil(nd / 256), math.ceil(4096 / max(1, xs.n))))
-    _lib.call("gkm_self_hist", _p(xs.planes), _p(xs.planes_rc) if kp.rc else None, _p(xs.len),
-              xs.n, xs.nw, kp.l, kp.nm, _p(wt), wn, nchunks, _p(hist), stream_handle(stream))
+    norient = 2 if kp.rc else 1
+    _timed("k_self_hist",
+           lambda: float(norient * (np.clip(xs.lengths - kp.l + 1, 0, None) ** 2).sum()), stream,
+           lambda: _lib.call(
+               "gkm_self_hist", _p(xs.planes), _p(xs.planes_rc) if kp.rc else None, _p(xs.len),
+               xs.n, xs.nw, kp.l, kp.nm, _p(wt), wn, nchunks, _p(hist), stream_handle(stream)))
     out = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
     _lib.call("gkm_combine_kraw", _p(hist), xs.n, kp.nm, _p(kp.tab), _p(out),
               stream_handle(stream))
