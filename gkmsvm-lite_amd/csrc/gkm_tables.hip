@@ -212,30 +212,28 @@ __global__ __launch_bounds__(TAB_THREADS) void k_table_score(
 // row b gathers the windows covering it; window u at digit k adds A[u][k][d]
 // to base u_k ^ d (d = 1..3) and A[u][k][0] to u_k (hypothetical) or to the
 // three other bases (perturbation), times its window weight; the sum is scaled
-// by 1 / sqrt(max(K_raw(x, x), 1e-20)).  Every output entry is written.
-__global__ __launch_bounds__(TAB_THREADS) void k_table_explain(
+// by 1 / sqrt(max(K_raw(x, x), 1e-20)).  A block covers EXPLAIN_TILE positions
+// of one row with one thread per (position, digit k): every table row is
+// fetched at once, and the l partial sums of a position are added in k order
+// through LDS.  Every output entry is written.
+constexpr int EXPLAIN_TILE = 64;
+
+__global__ __launch_bounds__(EXPLAIN_TILE * 16) void k_table_explain(
     const uint32_t *__restrict__ xwin, const int32_t *__restrict__ xlen, int Wx, int lmax, int l,
     const uint8_t *__restrict__ wtab, const double *__restrict__ A,
     const double *__restrict__ dx, int normalize, int pert, double *__restrict__ out)
 {
+    __shared__ double part[16][EXPLAIN_TILE][4];
     const int b = blockIdx.y;
-    const int p = blockIdx.x * TAB_THREADS + threadIdx.x;
-    if (p >= lmax)
-        return;
+    const int pl = threadIdx.x, k = threadIdx.y;
+    const int p = blockIdx.x * EXPLAIN_TILE + pl;
     const int L = xlen[b], W = L - l + 1;
     const uint32_t nl = 1u << (2 * l);
-    const uint32_t *row = xwin + (int64_t)b * Wx;
     double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-    if (p < L) {
-        for (int k = 0; k < l; k++) {
-            const int i = p - k;
-            if (i < 0)
-                break;
-            if (i >= W)
-                continue;
-            const uint32_t u = row[i];
-            if (u >= nl)
-                continue;
+    const int i = p - k;
+    if (k < l && p < L && i >= 0 && i < W) {
+        const uint32_t u = xwin[(int64_t)b * Wx + i];
+        if (u < nl) {
             const double a = wtab ? (double)window_weight(wtab, W, i) : 1.0;
             const uint32_t uk = (u >> (2 * k)) & 3u;
             const double2 *r = reinterpret_cast<const double2 *>(A + ((int64_t)u * l + k) * 4);
@@ -254,12 +252,20 @@ __global__ __launch_bounds__(TAB_THREADS) void k_table_explain(
             }
         }
     }
-    const double inv = normalize ? 1.0 / sqrt(fmax(dx[b], 1e-20)) : 1.0;
-    double *o = out + (int64_t)b * 4 * lmax + p;
-    o[0] = c0 * inv;
-    o[(int64_t)lmax] = c1 * inv;
-    o[2 * (int64_t)lmax] = c2 * inv;
-    o[3 * (int64_t)lmax] = c3 * inv;
+    part[k][pl][0] = c0;
+    part[k][pl][1] = c1;
+    part[k][pl][2] = c2;
+    part[k][pl][3] = c3;
+    __syncthreads();
+    // thread (pl, j < 4) sums base j of position pl over the digits in order
+    // (blockDim.y = max(l, 4); rows k >= l hold zeros)
+    if (k < 4 && p < lmax) {
+        double t = 0.0;
+        for (int kk = 0; kk < l; kk++)
+            t += part[kk][pl][k];
+        const double inv = normalize ? 1.0 / sqrt(fmax(dx[b], 1e-20)) : 1.0;
+        out[((int64_t)b * 4 + k) * lmax + p] = p < L ? t * inv : 0.0;
+    }
 }
 
 // ISM from the score table (ism.py:118-144): substituting base v at p changes
@@ -480,8 +486,9 @@ GKM_API int gkm_table_explain(const uint32_t *xwin, const int32_t *xlen, int B, 
                     "and dx when normalising (B=%d lmax=%d Wx=%d)", B, lmax, Wx);
     if (B == 0)
         return 0;
-    hipLaunchKernelGGL(k_table_explain, dim3(blocks_for(lmax, TAB_THREADS), (unsigned)B),
-                       dim3(TAB_THREADS), 0, S_(stream), xwin, xlen, Wx, lmax, l, wtab, A, dx,
+    // one thread per (position, digit); at least 4 digit rows for the 4 bases
+    hipLaunchKernelGGL(k_table_explain, dim3(blocks_for(lmax, EXPLAIN_TILE), (unsigned)B),
+                       dim3(EXPLAIN_TILE, l < 4 ? 4 : l), 0, S_(stream), xwin, xlen, Wx, lmax, l, wtab, A, dx,
                        normalize, perturbation, out);
     GKM_CHECK_LAUNCH("gkm_table_explain");
     return 0;

@@ -88,9 +88,11 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
     short = short_rows(lengths, base.l, "attributions")
     lens = full_lengths(codes, lengths)
     B, L = codes.shape
-    out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
+    A = model._explain_table_for(B, lens, resolved == "perturbation") if L >= base.l else None
+    # the table kernel writes every entry (0 past each row's length)
+    out = (torch.empty if A is not None else torch.zeros)((B, 4, L), dtype=torch.float64,
+                                                          device=dev)
     if L >= base.l:
-        A = model._explain_table_for(B, lens, resolved == "perturbation")
         bs = max(1, int(batch_size)) if batch_size else B
         if A is not None:
             bs = max(bs, 4096)  # a table row costs l lookups per position
@@ -109,6 +111,8 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
                 _explain_rows(model, codes[s:e], lens[s:e], resolved, out[s:e])
     if resolved == "importance":
         out = out * torch.from_numpy(codes_to_one_hot(codes)).to(dev)
+    if A is not None:
+        return nan_rows(out, short)
     return per_position(out, lengths, short)
 
 

@@ -6,7 +6,8 @@ requests from any thread and get a Future.  One worker thread admits pending
 requests into GPU steps while earlier steps run: each step packs every
 admitted scoring/variant row into one batch (rows of any length: each row's
 diagonal walk uses its own length) and launches the cross kernel once for all
-of them on its own HIP stream; explain requests follow in the same step.  Up
+of them on its own HIP stream; the step's explain requests follow as one
+batch per mode.  Up
 to ``streams`` steps are in flight, so a request that arrives while others
 are on the GPU is admitted as soon as a stream frees up and its work overlaps
 theirs.  Admission is by memory: a step takes requests while the bytes it
@@ -63,7 +64,7 @@ class _Step:
     done: torch.cuda.Event
     score_host: torch.Tensor | None
     spans: list               # (request, start row, n rows) for scoring/variant rows
-    explain_out: list         # (request, host tensor [n, 4, L])
+    explain_out: list         # (request, host tensor [rows, 4, L], first row, rows, length)
     keep: list                # device tensors kept alive until the step completes
 
 
@@ -71,7 +72,7 @@ class GkmServer:
     """Serve one model; see the module docstring."""
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
-                 streams: int = 2, max_explain: int = 8, tables: bool = True):
+                 streams: int = 2, max_explain: int = 64, tables: bool = True):
         model.rocm()
         self.model = model
         self.sv = model._svs()
@@ -224,9 +225,9 @@ class GkmServer:
                 req.t_done = now
                 req.future.set_result(res)
                 self.completed.append(req)
-            for req, host in st.explain_out:
+            for req, host, r0, n, L in st.explain_out:
                 req.t_done = now
-                req.future.set_result(host.numpy())
+                req.future.set_result(host[r0:r0 + n, :, :L].numpy().copy())
                 self.completed.append(req)
             self._free.append(st.stream)
         self._inflight = still
@@ -263,14 +264,23 @@ class GkmServer:
                 score_host.copy_(out, non_blocking=True)
                 keep += [xs, out]
             ex_out = []
+            by_mode: dict = {}
             for req in explains:
+                by_mode.setdefault(req.mode, []).append(req)
+            for mode, reqs in by_mode.items():
+                # every explain request of this mode in one call (rows of any
+                # length; each row's attributions depend only on that row)
                 from .explain import gkmexplain
 
-                res = gkmexplain(model, req.seqs, mode=req.mode, batch_size=len(req.seqs))
+                seqs = [q for req in reqs for q in req.seqs]
+                res = gkmexplain(model, seqs, mode=mode, batch_size=len(seqs))
                 host = torch.empty(res.shape, dtype=torch.float64, pin_memory=True)
                 host.copy_(res, non_blocking=True)
                 keep.append(res)
-                ex_out.append((req, host))
+                r0 = 0
+                for req in reqs:
+                    ex_out.append((req, host, r0, len(req.seqs), req.max_len()))
+                    r0 += len(req.seqs)
             ev = torch.cuda.Event()
             ev.record(stream)
         return _Step(batch, stream, ev, score_host, spans, ex_out, keep)

@@ -79,13 +79,17 @@ def test_tables_equal_pairwise_definitions(gpu, seed, kt, l, k, extra):
     got = m._build_lmer_table().cpu().numpy()
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
     if l <= 5:
-        from gkmsvm_amd.explain import explain_tables
+        from gkmsvm_amd import gkmexplain
 
+        rng = np.random.default_rng(seed + 100)
+        codes, lens = _rand_queries(rng, 5, l, 40)
         for pert in (False, True):
             A = m._explain_table_for(1, np.array([l]), pert).cpu().numpy()
             Aw = O.explain_table_brute(om, pert)
             np.testing.assert_allclose(A, Aw, rtol=0, atol=1e-12 * np.abs(Aw).max())
-            assert explain_tables(m.kernel.base, pert)[2] >= 0
+            mode = "perturbation" if pert else "hypothetical"
+            got = gkmexplain(m, codes, mode=mode, lengths=lens).cpu().numpy()
+            H.assert_close(got, O.explain(om, codes, lens, mode), 1e-10, f"l={l} {mode}")
 
 
 @pytest.mark.parametrize("kind", TABLE_TYPES)
