@@ -229,10 +229,11 @@ def kernel_only(args):
 
     from gkmsvm_amd import engine as E
 
-    if args.mode == "copy":
+    if args.mode == "copy":  # device-to-device copies of 256 MiB: the counter calibration
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
         for _ in range(args.launches):
-            src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
-            torch.mul(src, 2.0)
+            dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
         return
@@ -240,6 +241,23 @@ def kernel_only(args):
     sv = model._svs()
     kp = model._params()
     rng = np.random.default_rng(7)
+    if args.mode == "table":
+        # the table path on one serving step's worth of rows: 64 x 1 kb queries
+        # scored through the l-mer table (k_self_hist, k_table_score) and
+        # explained through the explain table (k_table_explain)
+        from gkmsvm_amd import gkmexplain
+
+        model.use_lmer_table = model.use_explain_table = True
+        q = rng.integers(0, 4, (64, 1000)).astype(np.uint8)
+        model(q)
+        gkmexplain(model, q, mode="hypothetical")
+        torch.cuda.synchronize()
+        for _ in range(args.launches):
+            model(q)
+            gkmexplain(model, q, mode="hypothetical")
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
+        return
     q = rng.integers(0, 4, (16, 1000)).astype(np.uint8)
     xs = E.DeviceSeqs(q, np.full(16, 1000), kp.device)
     out = torch.empty((16, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
@@ -250,9 +268,8 @@ def kernel_only(args):
             E.index_hist(kp, sv.index(), xs, 0, 16, sv.S, out=out)
         elif args.mode == "cross":
             E.cross_hist(kp, xs, 0, 16, sv.seqs, sv.groups, out=out)
-        else:  # an elementwise pass over 256 MiB (reads 256 MiB, writes 256 MiB)
-            src = torch.ones(32 << 20, dtype=torch.float64, device=kp.device)
-            torch.mul(src, 2.0)
+        else:
+            raise ValueError(args.mode)
     torch.cuda.synchronize()
     print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
 
@@ -374,8 +391,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--mode", default="serve",
-                    choices=["serve", "cross", "index", "copy", "configs"],
-                    help="serve: the benchmark; cross/index/copy: one kernel, for profiling; "
+                    choices=["serve", "cross", "index", "table", "copy", "configs"],
+                    help="serve: the benchmark; cross/index/table/copy: one kernel family on "
+                         "the serving shape, for profiling; "
                          "configs: bounded timings of the other configurations")
     ap.add_argument("--c3-queries", type=int, default=4096)
     ap.add_argument("--c4-queries", type=int, default=8)

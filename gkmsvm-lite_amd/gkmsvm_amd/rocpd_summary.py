@@ -38,9 +38,12 @@ def pmc(dir_path: str, match: str = "") -> str:
                 if match and match not in name:
                     continue
                 acc[(name.split("(")[0][:58], r["Counter_Name"])].append(float(r["Counter_Value"]))
-    lines = [f"{'kernel':58s} {'counter':>12s} {'dispatches':>10s} {'mean/dispatch':>16s}"]
+    lines = [f"{'kernel':58s} {'counter':>12s} {'dispatches':>10s} {'mean/dispatch':>16s} "
+             f"{'median':>14s}"]
     for (name, ctr), vals in sorted(acc.items()):
-        lines.append(f"{name:58s} {ctr:>12s} {len(vals):10d} {sum(vals) / len(vals):16.1f}")
+        med = sorted(vals)[len(vals) // 2]
+        lines.append(f"{name:58s} {ctr:>12s} {len(vals):10d} {sum(vals) / len(vals):16.1f} "
+                     f"{med:14.1f}")
     return "\n".join(lines)
 
 
