@@ -10,7 +10,7 @@ from .codec import codes_to_one_hot, encode_codes, one_hot_encode, to_codes
 from .deltasvm import DeltaSVM
 from .explain import gkmexplain, mutation_impact
 from .gram import compute_gram
-from .importers import load_lsgkm_model, read_fasta
+from .importers import load_classic_model, load_lsgkm_model, load_r_gkmsvm_model, read_fasta
 from .ism import ism
 from .kernels import (
     KERNEL_ALIASES,
@@ -23,6 +23,15 @@ from .kernels import (
     RbfKernel,
     resolve_kernel_type,
 )
+from .serialization import (
+    load_deltasvm_model,
+    load_deltasvm_npz,
+    load_model,
+    load_npz,
+    save_deltasvm_npz,
+    save_lsgkm,
+    save_npz,
+)
 from .svm import GkmSVM, to_cpu
 from .train import train_gkmsvm
 
@@ -31,5 +40,7 @@ __all__ = [
     "load_lsgkm_model", "read_fasta", "one_hot_encode", "encode_codes", "to_codes",
     "codes_to_one_hot", "to_cpu", "resolve_kernel_type", "KERNEL_BUILDERS", "KERNEL_ALIASES",
     "DirectGkmKernel", "EstTruncGkmKernel", "CenterWeightedGkmKernel", "RbfKernel",
-    "RbfGkmKernel", "CenterWeightedRbfGkmKernel", "DeltaSVM",
+    "RbfGkmKernel", "CenterWeightedRbfGkmKernel", "DeltaSVM", "load_classic_model",
+    "load_r_gkmsvm_model", "load_model", "load_npz", "save_npz", "save_lsgkm",
+    "load_deltasvm_npz", "save_deltasvm_npz", "load_deltasvm_model",
 ]

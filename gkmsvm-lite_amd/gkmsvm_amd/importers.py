@@ -1,5 +1,6 @@
-"""LS-GKM model files and FASTA (the formats the fixtures and users hold;
-gkmsvm-lite importers/lsgkm.py:23-97, importers/__init__.py:15-113,
+"""LS-GKM model files, classic / R gkmSVM model files and FASTA (the formats
+the fixtures and users hold; gkmsvm-lite importers/lsgkm.py:23-97,
+importers/__init__.py:15-113, importers/classic.py, importers/r_gkmsvm.py,
 fasta.py:19-42)."""
 
 from __future__ import annotations
@@ -111,3 +112,71 @@ def load_lsgkm_model(path, *, sv_chunk_size: int | None = None, device: str = "r
     return GkmSVM(codes, np.array(coefs, dtype=np.float64), -header["rho"][0],
                   header["kernel_type"], params, sv_lengths=lengths,
                   sv_chunk_size=sv_chunk_size, device=device)
+
+
+# gkmsvm_kernel / gkmsvm_classify -t (R useTgkm) as kernel types: full filter,
+# truncated filter (the tools' default) and gkm counts, each cut at d
+# mismatches (importers/classic.py:24-27); 3 and 4 have no equivalent.
+GKMSVM_FILTERS = {0: "gkm_estfull", 1: "gkm_esttrunc", 2: "gkm_cnt"}
+
+
+def load_classic_model(alpha_path, svseq_path=None, *, L: int = 10, k: int = 6, d: int = 3,
+                       filter_type: int = 1, include_rc: bool = True,
+                       sv_chunk_size: int | None = None, device: str = "rocm"):
+    """GkmSVM from the two files classic (C++) gkmsvm_train writes
+    (importers/classic.py:30-127): PREFIX_svalpha.out (``id <TAB> alpha``) and
+    PREFIX_svseq.fa (default: next to it).  The files hold no bias and no
+    kernel parameters, so L, k, d, filter_type (-t) and include_rc are
+    arguments; SV letters other than ACGT are dropped, as the tools read them
+    (with a warning).  Bias 0."""
+    import re
+
+    from .svm import GkmSVM
+
+    alpha_path = Path(alpha_path)
+    if svseq_path is None:
+        name = alpha_path.name
+        if not name.endswith("svalpha.out"):
+            raise ValueError(f"give svseq_path for {alpha_path}")
+        svseq_path = alpha_path.with_name(name[: -len("svalpha.out")] + "svseq.fa")
+    ids, coefs = [], []
+    with _open(alpha_path) as f:
+        for line in f:
+            line = line.strip()
+            if not line:
+                continue
+            parts = line.split()
+            if len(parts) != 2:
+                raise ValueError(f"Expected 'id <TAB> alpha', got: {line[:80]}")
+            ids.append(parts[0])
+            coefs.append(float(parts[1]))
+    by_id = {name.split()[0]: seq for name, seq in read_fasta(svseq_path)}
+    missing = [i for i in ids if i not in by_id]
+    if missing:
+        raise ValueError(f"{len(missing)} SV ids of {alpha_path} are not in {svseq_path}, "
+                         f"e.g. {missing[0]}")
+    seqs = [re.sub("[^ACGTacgt]", "", by_id[i]).upper() for i in ids]
+    dropped = sum(len(by_id[i]) != len(s) for i, s in zip(ids, seqs))
+    if dropped:
+        warnings.warn(f"{dropped} support vectors hold letters other than ACGT, dropped as "
+                      "gkmSVM reads them (queries with N score as in gkmsvm_classify only "
+                      "with those letters dropped too)", stacklevel=2)
+    if filter_type not in GKMSVM_FILTERS:
+        raise NotImplementedError(
+            f"gkmSVM filter type {filter_type} (wildcard/mismatch kernels) is not supported; "
+            f"use {sorted(GKMSVM_FILTERS)}")
+    if not seqs:
+        raise ValueError("No support vector sequences found")
+    codes, lengths = as_batch([encode_codes(s) for s in seqs])
+    params = {"L": L, "k": k, "d": d, "include_rc": bool(include_rc)}
+    return GkmSVM(codes, np.array(coefs, dtype=np.float64), 0.0, GKMSVM_FILTERS[filter_type],
+                  params, sv_lengths=lengths, sv_chunk_size=sv_chunk_size, device=device)
+
+
+def load_r_gkmsvm_model(path, svseq_path=None, *, L: int = 10, K: int = 6, maxnmm: int = 3,
+                        useTgkm: int = 1, addRC: bool = True, sv_chunk_size: int | None = None,
+                        device: str = "rocm"):
+    """GkmSVM from the files R gkmSVM's gkmsvm_train writes (the classic
+    format, importers/r_gkmsvm.py:17-56) with the R package's argument names."""
+    return load_classic_model(path, svseq_path, L=L, k=K, d=maxnmm, filter_type=useTgkm,
+                              include_rc=addRC, sv_chunk_size=sv_chunk_size, device=device)
