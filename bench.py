@@ -10,30 +10,32 @@ HBM budget.
 A step is one burst of --requests requests served through the continuous-
 batching server (gkmsvm_amd.serving.GkmServer) as they arrive.  value is
 query x SV kernel evaluations per second over the timed steps (a scoring row
-counts S evaluations, an SNV pair 2 S, an explain call S); p50/p99 are request
-latencies from arrival to result.  The server builds the model's exact l-mer
-tables when it starts (non-RBF model: scoring and variant rows cost one table
-lookup per window, explain rows l lookups per position; the build time is
-reported).  Beside it: the same stream served through the per-SV kernels
-(pairwise kernel / l-mer index / explain sweep, tables off), and the plain path
--- the same requests, one API call each (model(x), model.score_variants,
-gkmexplain) -- on the per-SV kernels (what the model's "auto" planning picks
-for one request at a time) and with the tables prebuilt.
+counts S evaluations, an SNV pair 2 S, an explain call S) with the model's
+l-mer tables off: scoring and variant rows go through the pairwise kernel or
+the l-mer index (whichever the cost model picks; both give the same integer
+histograms), explain rows through the explain sweep -- every query x SV pair
+is evaluated.  p50/p99 are request latencies from arrival to result.  Beside
+it: the plain path (the same requests, one API call each: model(x),
+model.score_variants, gkmexplain) on the same kernels, and the same stream
+through the exact l-mer tables the server builds at start for a non-RBF
+model (S-independent: one lookup per window; no query x SV pair is
+evaluated, and evaluations are counted as the per-SV path's only to compare
+rates), served and plain.
 
 roofline: every launch of the hot kernels is timed with HIP events on the
 stream it runs on during the timed steps; the object describes the kernel
 with the most GPU time in the value run (the others under "others"; the
-per-SV run's under "kernel_path").  Pairwise kernels (k_cross_hist,
-k_self_hist, k_sweep): window-pair comparisons x 6 integer operations (XOR,
-shift, OR, AND, popcount, accumulate) / launch time, against the int32 VALU
-peak.  HBM-bound kernels (k_index_tile: 8 B offsets per probe + 2-3 B per
-entry; k_table_score: 4 B window + 8 B table entry per window;
-k_table_explain: 4 B window + 32 l B table row per window + 32 B output per
-position) / launch time, against 8 TB/s.
+table run's under config.served_lmer_tables.roofline).  Pairwise kernels
+(k_cross_hist, k_self_hist, k_sweep): window-pair comparisons x 6 integer
+operations (XOR, shift, OR, AND, popcount, accumulate) / launch time,
+against the int32 VALU peak.  HBM-bound kernels (k_index_tile: 8 B offsets
+per probe + 2-3 B per entry; k_table_score: 4 B window + 8 B table entry per
+window; k_table_explain: 4 B window + 32 l B table row per window + 32 B
+output per position) / launch time, against 8 TB/s.
 cpu_baseline: the reference's CPU algorithm on the host cores (the oracle, C
 with OpenMP + numpy) on a bounded sample of the stream's scoring queries: the
-l-mer table (svm.py:544-582) when the GPU value used it, with the l-mer index
-+ fused reduce rate beside it.
+l-mer index + fused reduce (kernels/lmer_index_cpu.py:70-120, what the
+reference picks for these calls), with the l-mer table rate beside it.
 """
 
 from __future__ import annotations
@@ -171,17 +173,17 @@ def cpu_baseline(sv, coef, reqs, seconds):
     index_build_s = time.perf_counter() - t
     v_index, n_index, mean_index = run(index.score, seconds / 2)
     return {
-        "value": v_table,
+        "value": v_index,
         "unit": "query x SV kernel evals/s",
         "cores": nt,
         "kind": "port",
-        "sample": (f"{n_table} scoring queries of the request stream (mean {mean_table:.0f} bp) "
-                   f"scored through the exact l-mer table (lookups in numpy, query self-kernel "
-                   f"in C/OpenMP; table build {table_build_s:.1f} s by masked marginals, not "
-                   f"counted); the l-mer index + fused reduce (C/OpenMP) scores "
-                   f"{v_index:.3g} evals/s on {n_index} of them (index build "
-                   f"{index_build_s:.1f} s, not counted)"),
-        "index_path_value": v_index,
+        "sample": (f"{n_index} scoring queries of the request stream (mean {mean_index:.0f} bp) "
+                   f"through the l-mer index + fused reduce (C/OpenMP; index build "
+                   f"{index_build_s:.1f} s, not counted); through the exact l-mer table "
+                   f"(numpy lookups, C/OpenMP self-kernel; table build {table_build_s:.1f} s by "
+                   f"masked marginals, not counted) {n_table} of them (mean {mean_table:.0f} bp) "
+                   f"score {v_table:.3g} evals/s"),
+        "lmer_table_value": v_table,
     }
 
 
@@ -377,10 +379,11 @@ def config_runs(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel-steps", type=int, default=6,
-                    help="timed steps of the per-SV kernel runs (served and plain)")
+    ap.add_argument("--steps", type=int, default=8,
+                    help="timed steps of the headline run (per-SV kernels) and its plain path")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--table-steps", type=int, default=20,
+                    help="timed steps of the l-mer table runs (served and plain)")
     ap.add_argument("--requests", type=int, default=96, help="requests per step")
     ap.add_argument("--rate", type=float, default=1e6,
                     help="Poisson arrivals per second in the throughput steps (above capacity)")
@@ -419,7 +422,7 @@ def main():
     model.rocm()
     model._svs()
     torch.cuda.synchronize()
-    n_steps = args.warmup + max(args.steps, args.kernel_steps)
+    n_steps = args.warmup + max(args.steps, args.table_steps)
     steps = [make_requests(args.requests, 100 + i) for i in range(n_steps)]
     rng = np.random.default_rng(5)
 
@@ -440,37 +443,49 @@ def main():
         server.completed.clear()
         return el, timer.summary(), lat_sat
 
-    # 1. The server as it runs by default: exact l-mer tables built at start.
-    t = time.perf_counter()
-    server = GkmServer(model, hbm_budget=16 << 30)
-    torch.cuda.synchronize()
-    build_s = time.perf_counter() - t
-    server.start()
-    timed = steps[args.warmup:args.warmup + args.steps]
-    served_s, summary, lat_sat = served_run(server, timed)
-    n_evals = sum(evals(s_) for s_ in timed)
-    served_rate = n_evals / served_s
-    # latency at a nominal load: arrivals at a fraction of the measured capacity
-    cap = args.requests * len(timed) / served_s
-    load_rate = args.load * cap
-    for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
-        run_served(server, s_, load_rate, rng)
-    lat = latency_stats(server.completed)
-    server.stop()
-
-    # 2. The same stream through the per-SV kernels (tables off).
+    # 1. The headline: the stream served through the per-SV kernels (pairwise
+    # kernel / l-mer index for scoring, the explain sweep), tables off -- the
+    # path that evaluates every query x SV pair.
     model_k = make_model()[0]
     model_k.use_lmer_table = False
     model_k.use_explain_table = False
     model_k.rocm()
     model_k._svs()
-    ktimed = steps[args.warmup:args.warmup + args.kernel_steps]
+    ktimed = steps[args.warmup:args.warmup + args.steps]
     k_evals = sum(evals(s_) for s_ in ktimed)
     server_k = GkmServer(model_k, hbm_budget=16 << 30, tables=False).start()
-    served_k_s, summary_k, lat_k = served_run(server_k, ktimed)
+    served_s, summary, lat_sat = served_run(server_k, ktimed)
+    served_rate = k_evals / served_s
+    # latency at a nominal load: arrivals at a fraction of the measured capacity
+    cap = args.requests * len(ktimed) / served_s
+    load_rate = args.load * cap
+    for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
+        run_served(server_k, s_, load_rate, rng)
+    lat = latency_stats(server_k.completed)
     server_k.stop()
-    served_k = {"value": k_evals / served_k_s, "ms_per_step": served_k_s / len(ktimed) * 1e3,
-                "steps": len(ktimed), "latency_ms_at_saturation": lat_k}
+
+    # 2. The same stream through the exact l-mer tables, built when the server
+    # starts (non-RBF model): no query x SV pair is evaluated; evaluations
+    # are counted as the per-SV path's for comparison.
+    t = time.perf_counter()
+    server = GkmServer(model, hbm_budget=16 << 30)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t
+    server.start()
+    ttimed = steps[args.warmup:args.warmup + args.table_steps]
+    t_evals = sum(evals(s_) for s_ in ttimed)
+    served_t_s, summary_t, lat_t_sat = served_run(server, ttimed)
+    cap_t = args.requests * len(ttimed) / served_t_s
+    server.completed.clear()
+    for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
+        run_served(server, s_, args.load * cap_t, rng)
+    lat_t = latency_stats(server.completed)
+    server.stop()
+    served_t = {"value": t_evals / served_t_s, "ms_per_step": served_t_s / len(ttimed) * 1e3,
+                "steps": len(ttimed), "table_build_s": build_s,
+                "latency_ms_at_saturation": lat_t_sat,
+                "latency_load_rate_per_s": args.load * cap_t, "latency_ms": lat_t,
+                "roofline": roofline_of(summary_t)}
 
     plain = plain_t = None
     if not args.no_plain:
@@ -486,14 +501,15 @@ def main():
         # 4. Plain calls with the tables already built.
         run_plain(model, steps[0][:8])
         t = time.perf_counter()
-        for s_ in timed:
+        for s_ in ttimed:
             run_plain(model, s_)
         plain_t_s = time.perf_counter() - t
-        plain_t = {"value": n_evals / plain_t_s, "ms_per_step": plain_t_s / len(timed) * 1e3,
-                   "steps": len(timed)}
+        plain_t = {"value": t_evals / plain_t_s, "ms_per_step": plain_t_s / len(ttimed) * 1e3,
+                   "steps": len(ttimed)}
+        served_t["served_over_plain_tables"] = served_t["value"] / plain_t["value"]
+        served_t["served_over_plain"] = served_t["value"] / plain["value"]
 
     roofline = roofline_of(summary)
-    roofline["kernel_path"] = roofline_of(summary_k)
     line = {
         "metric": "query x SV kernel evals/sec (served), with p50/p99 request latency and the plain path",
         "value": served_rate,
@@ -505,24 +521,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 table lookups and sums; u32 bit-plane popcount histograms (self-kernels)",
+        "dtype": "u32 bit-plane popcount histograms and l-mer index counts, f64 combine",
         "data": "synthetic (uniform iid ACGT, seeded)",
         "config": {
             "workload": "serving: 72,000-SV -t 2 l=11 k=7 d=3 model of 500 bp SVs resident in "
                         "HBM; Poisson stream mixing scoring (50-2000 bp), SNV pairs (300 bp) "
-                        "and gkmexplain (1 kb)",
+                        "and gkmexplain (1 kb); served through the per-SV kernels",
             "requests_per_step": args.requests,
             "arrival_rate_per_s": args.rate,
-            "table_build_s": build_s,
             "latency_ms_at_saturation": lat_sat,
             "latency_load_rate_per_s": load_rate,
             "latency_ms": lat,
-            "served_kernel_path": served_k,
             "plain": plain,
             "plain_tables_prebuilt": plain_t,
             "served_over_plain": (served_rate / plain["value"]) if plain else None,
-            "served_over_plain_tables": (served_rate / plain_t["value"]) if plain_t else None,
-            "kernel_path_served_over_plain": (served_k["value"] / plain["value"]) if plain else None,
+            "served_lmer_tables": served_t,
         },
         "roofline": roofline,
     }

@@ -804,7 +804,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
 // k_index_tile: the same histograms through an l-mer index of the SV windows
 // ---------------------------------------------------------------------------
 
-constexpr int INDEX_THREADS = 256;
+constexpr int INDEX_THREADS = 1024;  // 16 waves share one LDS tile (one block per CU)
 
 struct IndexArgs {
     const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
@@ -831,13 +831,21 @@ struct IndexArgs {
 // PACK16: two 16-bit counts per 32-bit LDS word (unweighted; the host caps the
 // window chunk so no count of a block can reach 2^16), which doubles the SVs a
 // tile holds and halves the probes.
+// Block order: the XCDs take blocks round-robin by linear id (gridDim.x is a
+// multiple of 8), so block p runs on XCD p % 8.  The blocks of one window
+// chunk -- one per SV tile, probing the same offsets lines -- are given ids
+// congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
+// chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk exit at once.
 template <bool PACK16>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     uint32_t *cnt = (uint32_t *)smem;
-    const int tile_id = blockIdx.x % a.n_tiles;
-    const int chunk = blockIdx.x / a.n_tiles;
+    const int q = blockIdx.x >> 3;
+    const int tile_id = q % a.n_tiles;
+    const int chunk = (q / a.n_tiles) * 8 + (int)(blockIdx.x & 7);
+    if (chunk >= a.nchunks)
+        return;
     const int b = blockIdx.y;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
@@ -859,22 +867,48 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
         const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
         for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
+            // lane t probes neighbour v = w ^ mask[t]: its bucket [e0, e0 + c)
             const int t = t0 + lane;
-            if (t >= a.n_masks)
-                break;
-            const uint32_t v = w ^ a.masks[t];
-            const int lv = a.levels[t];
-            const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
-            const uint32_t e0 = a.offsets[key], e1 = a.offsets[key + 1];
-            for (uint32_t e = e0; e < e1; e++) {
+            uint32_t e0 = 0;
+            int c = 0, lv = 0;
+            if (t < a.n_masks) {
+                const uint32_t v = w ^ a.masks[t];
+                lv = a.levels[t];
+                const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
+                e0 = a.offsets[key];
+                c = (int)(a.offsets[key + 1] - e0);
+            }
+            // the wave's buckets laid end to end (exclusive prefix sums), then
+            // walked 64 entries at a time: lane j of a pass takes entry base + j,
+            // found in probe p = the last lane with excl[p] <= base + j
+            int incl = c;
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const int u = __shfl_up(incl, off, WAVE);
+                incl += lane >= off ? u : 0;
+            }
+            const int excl = incl - c;
+            const int total = __shfl(incl, WAVE - 1, WAVE);
+            for (int base = 0; base < total; base += WAVE) {
+                const int j = base + lane;
+                int pr = 0;
+                for (int step = WAVE / 2; step > 0; step >>= 1) {
+                    const int ex = __shfl(excl, pr + step, WAVE);
+                    pr += ex <= j ? step : 0;
+                }
+                const uint32_t pe0 = (uint32_t)__shfl((int)e0, pr, WAVE);
+                const int plv = __shfl(lv, pr, WAVE);
+                const int pex = __shfl(excl, pr, WAVE);
+                if (j >= total)
+                    continue;
+                const uint32_t e = pe0 + (uint32_t)(j - pex);
                 const int sid = a.ids[e];
                 if (sid < lo_col)
                     continue;
                 if (PACK16) {
-                    atomicAdd(&cnt[sid * wps + (lv >> 1)], 1u << (16 * (lv & 1)));
+                    atomicAdd(&cnt[sid * wps + (plv >> 1)], 1u << (16 * (plv & 1)));
                 } else {
                     const uint32_t add = a.vals ? aw * a.vals[e] : 1u;
-                    atomicAdd(&cnt[sid * a.nm + lv], add);
+                    atomicAdd(&cnt[sid * a.nm + plv], add);
                 }
             }
         }
@@ -889,6 +923,156 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         } else {
             dst[i] = cnt[i];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_index_explain: GkmExplain through the l-mer index
+// ---------------------------------------------------------------------------
+
+constexpr int IXEXP_THREADS = 256;
+
+struct IndexExplainArgs {
+    const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
+    const int32_t *xlen;     // [B]
+    const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
+    const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
+    const uint8_t *vals;     // [entries] window weight of each entry, or NULL
+    const uint32_t *masks;   // [n_masks] XOR neighbourhood (levels 0..cmax)
+    const uint8_t *levels;   // [n_masks]
+    const uint8_t *wtab;     // query window weight by distance, or NULL
+    const double *ta, *tb;   // explain tables [l + 2]
+    const double *ec;        // [B][S] effective coefficients (SV order of the index)
+    double *out;             // [B][4][lmax], added with FP64 atomics
+    int B, Wx, lmax, n_masks, l, S, tile, n_tiles, chunk_windows, pert;
+};
+
+// Block = (chunk of query windows, query row); a wave takes a window u and its
+// lanes 64 neighbourhood masks at a time.  For v = u ^ mask (mismatch count m)
+// the bucket of l-mer v in every SV tile gives s = sum_e ec[sv_e] * weight_e
+// -- the SV windows equal to v, each a pair with m mismatches -- and every one
+// of them adds the same pattern to the window's l positions: at a matching
+// position ta[m] to base u_k, at a mismatching one tb[m] to base v_k
+// (explain/cpu.py:68-86, 146-207).  A lane sums its probes' patterns in
+// registers (am[k]: matches; ad[k][d - 1]: mismatches with digit delta d);
+// after the window's last mask the wave adds them up and 4 l lanes put them
+// into the block's [4][lmax] LDS sums, which go out with FP64 atomics.
+template <int LT>
+__global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArgs a)
+{
+    constexpr int LM = LT > 0 ? LT : 15;
+    extern __shared__ __align__(16) uint8_t smem[];
+    double *res = (double *)smem;          // [4][span]
+    double *stab = res;                    // set below, after res
+    const int l = LT > 0 ? LT : a.l;
+    const int b = blockIdx.y;
+    const int wx = a.xlen[b] - l + 1;
+    const int i0 = blockIdx.x * a.chunk_windows;
+    const int i1 = min(wx, i0 + a.chunk_windows);
+    const int span = a.chunk_windows + LM - 1;  // positions i0 .. i0 + span - 1
+    stab = res + 4 * (size_t)span;
+    for (int i = threadIdx.x; i < 4 * span; i += IXEXP_THREADS)
+        res[i] = 0.0;
+    if (threadIdx.x < 16) {
+        const int t = threadIdx.x;
+        stab[t] = t < a.l + 2 ? a.ta[t] : 0.0;
+        stab[16 + t] = t < a.l + 2 ? a.tb[t] : 0.0;
+    }
+    __syncthreads();
+    const uint32_t nl = 1u << (2 * l);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const double *ecb = a.ec + (int64_t)b * a.S;
+    const int nt = a.n_tiles;
+    for (int i = i0 + wave; i < i1; i += IXEXP_THREADS / WAVE) {
+        const uint32_t u = a.xwin[(int64_t)b * a.Wx + i];
+        if (u >= nl)
+            continue;  // PAD
+        double am[LM], ad[LM][3];
+#pragma unroll
+        for (int k = 0; k < LM; k++) {
+            am[k] = 0.0;
+            ad[k][0] = ad[k][1] = ad[k][2] = 0.0;
+        }
+        for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
+            const int t = t0 + lane;
+            if (t < a.n_masks) {
+                const uint32_t mask = a.masks[t];
+                const uint32_t v = u ^ mask;
+                const uint32_t *off = a.offsets + (int64_t)v * nt;
+                double sv = 0.0;
+                uint32_t e = off[0];
+                for (int tl = 0; tl < nt; tl++) {
+                    const uint32_t e1 = off[tl + 1];
+                    const int base = tl * a.tile;
+                    for (; e < e1; e++) {
+                        const double c = ecb[base + a.ids[e]];
+                        sv += a.vals ? c * (double)a.vals[e] : c;
+                    }
+                }
+                if (sv != 0.0) {
+                    const int m = a.levels[t];
+                    const double sa = sv * stab[m], sb = sv * stab[16 + m];
+#pragma unroll
+                    for (int k = 0; k < LM; k++) {
+                        if (LT == 0 && k >= l)
+                            break;
+                        const uint32_t d = (mask >> (2 * k)) & 3u;
+                        am[k] += d == 0 ? sa : 0.0;
+                        ad[k][0] += d == 1 ? sb : 0.0;
+                        ad[k][1] += d == 2 ? sb : 0.0;
+                        ad[k][2] += d == 3 ? sb : 0.0;
+                    }
+                }
+            }
+        }
+        // wave totals (every lane ends with them), then lane j < 4 l places
+        // accumulator j: position k = j / 4, slot j % 4 (0: matches, 1..3: delta)
+        const double aw = a.wtab ? (double)window_weight(a.wtab, wx, i) : 1.0;
+#pragma unroll
+        for (int k = 0; k < LM; k++) {
+            if (LT == 0 && k >= l)
+                break;
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                am[k] += __shfl_xor(am[k], off, WAVE);
+                ad[k][0] += __shfl_xor(ad[k][0], off, WAVE);
+                ad[k][1] += __shfl_xor(ad[k][1], off, WAVE);
+                ad[k][2] += __shfl_xor(ad[k][2], off, WAVE);
+            }
+        }
+        if (lane < 4 * l) {
+            const int k = lane >> 2, slot = lane & 3;
+            double val = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < LM; kk++)
+                if (kk == k)
+                    val = slot == 0 ? am[kk] : ad[kk][slot - 1];
+            val *= aw;
+            const uint32_t uk = (u >> (2 * k)) & 3u;
+            const int pos = i - i0 + k;
+            if (val != 0.0) {
+                if (slot > 0) {
+                    atomicAdd(&res[(uk ^ (uint32_t)slot) * span + pos], val);
+                } else if (!a.pert) {
+                    atomicAdd(&res[uk * span + pos], val);
+                } else {
+                    for (uint32_t h = 0; h < 4; h++)
+                        if (h != uk)
+                            atomicAdd(&res[h * span + pos], val);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (i1 <= i0)
+        return;
+    const int npos = i1 - i0 + l - 1;
+    double *ob = a.out + (int64_t)b * 4 * a.lmax + i0;
+    for (int j = threadIdx.x; j < 4 * npos; j += IXEXP_THREADS) {
+        const int v = j / npos, p = j - v * npos;
+        const double r = res[v * span + p];
+        if (r != 0.0)
+            atomicAdd(&ob[(int64_t)v * a.lmax + p], r);
     }
 }
 
@@ -955,7 +1139,7 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
-    dim3 grid((unsigned)(n_tiles * a.nchunks), (unsigned)B);
+    dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
     if (pack16)
         hipLaunchKernelGGL(k_index_tile<true>, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     else
@@ -1359,6 +1543,60 @@ GKM_API int gkm_explain(int perturbation, const void *xplanes, const int32_t *xl
                         yplanes, yplanes_rc, ylen, S, nwy, l, nm, cmax, wtab, wtab_len,
                         sv_per_block, ta, tb, ec, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                         0, 0.0, out, stream);
+}
+
+GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B,
+                              int Wx, int lmax, const uint32_t *offsets, const uint16_t *ids,
+                              const uint8_t *vals, const uint32_t *masks, const uint8_t *levels,
+                              int n_masks, int l, const uint8_t *wtab, int S, int tile,
+                              int n_tiles, int chunk_windows, const double *ta, const double *tb,
+                              const double *ec, double *out, void *stream)
+{
+    if (l < 1 || l > 13)
+        return fail(-1, "gkm_index_explain: the l-mer index needs 1 <= l <= 13, got %d", l);
+    if (tile < 1 || tile > 65536 || n_tiles != (S + tile - 1) / tile || chunk_windows < 1)
+        return fail(-1, "gkm_index_explain: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", tile,
+                    n_tiles, S, chunk_windows);
+    if (B < 0 || B > 65535 || Wx < 0 || lmax < Wx + l - 1)
+        return fail(-1, "gkm_index_explain: need 0 <= B <= 65535 and lmax >= Wx + l - 1");
+    if (B == 0 || Wx == 0 || S == 0 || n_masks <= 0)
+        return 0;
+    IndexExplainArgs a;
+    a.xwin = xwin;
+    a.xlen = xlen;
+    a.offsets = offsets;
+    a.ids = ids;
+    a.vals = vals;
+    a.masks = masks;
+    a.levels = levels;
+    a.wtab = wtab;
+    a.ta = ta;
+    a.tb = tb;
+    a.ec = ec;
+    a.out = out;
+    a.B = B;
+    a.Wx = Wx;
+    a.lmax = lmax;
+    a.n_masks = n_masks;
+    a.l = l;
+    a.S = S;
+    a.tile = tile;
+    a.n_tiles = n_tiles;
+    a.chunk_windows = chunk_windows;
+    a.pert = perturbation != 0;
+    const int lm = (l == 10 || l == 11 || l == 12) ? l : 15;
+    size_t lds = (4 * ((size_t)chunk_windows + lm - 1) + 32) * sizeof(double);
+    if (lds > 64 * 1024)
+        return fail(-2, "gkm_index_explain: chunk_windows %d too large", chunk_windows);
+    dim3 grid((unsigned)((Wx + chunk_windows - 1) / chunk_windows), (unsigned)B);
+    switch (l) {
+    case 10: hipLaunchKernelGGL(k_index_explain<10>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
+    case 11: hipLaunchKernelGGL(k_index_explain<11>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
+    case 12: hipLaunchKernelGGL(k_index_explain<12>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
+    default: hipLaunchKernelGGL(k_index_explain<0>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
+    }
+    GKM_CHECK_LAUNCH("gkm_index_explain");
+    return 0;
 }
 
 GKM_API int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int lmax,

@@ -266,8 +266,8 @@ def neighbor_masks(l: int, table, dmax: int):
 # per second of gkm_cross_hist; (window, mask, SV tile) probes and index
 # entries per second of gkm_index_hist.
 BRUTE_COMPARISONS_PER_S = 1.8e13
-INDEX_PROBES_PER_S = 6.0e10
-INDEX_ENTRIES_PER_S = 7.7e10
+INDEX_PROBES_PER_S = 5.0e10
+INDEX_ENTRIES_PER_S = 1.0e12
 INDEX_MAX_L = 12
 
 
@@ -337,17 +337,18 @@ class DeviceIndex:
 
 def index_tile(nm: int, pack16: bool = False) -> int:
     """SVs per index tile: the tile's counts ([tile, nm] uint32, or two 16-bit
-    counts per word) take 128 KiB of LDS -- the index is probe-bound, so the
-    largest tile that fits wins; at most 65,536 (uint16 ids)."""
+    counts per word) take up to 144 KiB of the CU's 160 KiB of LDS -- the index
+    is probe-bound, so the largest tile that fits wins (fewer tiles, fewer
+    probes); at most 65,536 (uint16 ids)."""
     words = (nm + 1) // 2 if pack16 else nm
-    return min(65536, max(256, (32768 // words) // 256 * 256))
+    return min(65536, max(256, (36864 // words) // 256 * 256))
 
 
 MAX_PARTIAL_BYTES = 2 << 30
 
 
 def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: int, S: int, *,
-               col_min: int = -1, out=None, stream=None, target_blocks: int = 2048) -> torch.Tensor:
+               col_min: int = -1, out=None, stream=None, target_blocks: int = 4096) -> torch.Tensor:
     """[x1 - x0, S, nm] int64 histograms of rows x0..x1 through the index (in
     row groups whose per-chunk partial counts stay under MAX_PARTIAL_BYTES)."""
     B = x1 - x0

@@ -59,9 +59,22 @@ def explain_tables(base, perturbation: bool):
     return a, b, (max(nz) if nz else -1)
 
 
-# Window-pair comparisons per second of the explain sweep (k_sweep, MI355X):
+# Window-pair comparisons per second of the per-SV explain kernels (MI355X):
 # the per-SV cost that an explain table has to beat
 SWEEP_COMPARISONS_PER_S = 5e12
+
+# gkm_index_explain rates (MI355X, l = 11): (window, mask) probes and index
+# entries per second; with the sweep's rate, the cost model that picks the
+# per-SV explain kernel
+INDEX_EXPLAIN_PROBES_PER_S = 2.2e10
+INDEX_EXPLAIN_ENTRIES_PER_S = 7e10
+
+# Per-SV explain kernel: "auto" (the l-mer index when the cost model favours
+# it and the model allows it, else the sweep), "index" (gkm_index_explain:
+# every query window visits its neighbourhood in the SVs' l-mer index) or
+# "sweep" (gkm_explain: lanes = 64 diagonals of one query x SV pair).  Both
+# compute the same attributions.
+EXPLAIN_KERNEL = "auto"
 
 
 def _sv_chunk(B: int, S: int) -> int:
@@ -140,13 +153,85 @@ def _explain_rows(model, codes, lens, resolved, out):
     wt, wn = kp.wtab(max(xs.lmax, sv.seqs.lmax))
     buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
     norient = 2 if kp.rc else 1
-    E._timed("k_sweep", lambda: E.comparisons(kp, xs.lengths, sv.seqs.lengths, norient), None,
-             lambda: _lib.call(
-                 "gkm_explain", int(pert), E._p(xs.planes), E._p(xs.len), xs.lmax, B, xs.nw,
-                 E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
-                 E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
-                 _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
+    units = lambda: E.comparisons(kp, xs.lengths, sv.seqs.lengths, norient)  # noqa: E731
+    kernel = _explain_kernel(model, sv, kp, xs, ta, tb, cmax)
+    if kernel == "index":
+        idx = sv.index()
+        masks, levels = _explain_masks(sv, ta, tb, cmax, pert)
+        fwd, _ = xs.windows(kp.l)
+        Wx = fwd.shape[1]
+        wq, _ = kp.wtab(xs.lmax)
+        chunk = max(4, min(64, -(-Wx * B // 1024)))
+        E._timed("k_index_explain",
+                 lambda: float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks) * 8.0,
+                 None, lambda: _lib.call(
+                     "gkm_index_explain", int(pert), E._p(fwd), E._p(xs.len), B, Wx, xs.lmax,
+                     E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(masks),
+                     E._p(levels), len(masks), kp.l, E._p(wq), sv.S, idx.tile, idx.n_tiles,
+                     chunk, E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
+    else:
+        E._timed("k_sweep", units, None, lambda: _lib.call(
+            "gkm_explain", int(pert), E._p(xs.planes), E._p(xs.len), xs.lmax, B, xs.nw,
+            E._p(sv.seqs.planes), E._p(sv.seqs.planes_rc) if kp.rc else None,
+            E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
+            _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
     out += buf
+
+
+def _explain_masks(sv, ta, tb, cmax, pert):
+    """(masks, levels) on the device: the XOR neighbourhood of the levels
+    0..cmax with a non-zero explain weight (cached per mode)."""
+    key = ("explain_masks", bool(pert))
+    if key not in sv.tables:
+        weight = np.abs(np.asarray(ta)) + np.abs(np.asarray(tb))
+        masks, levels = E.neighbor_masks(sv.kp.l, weight, cmax)
+        sv.tables[key] = (torch.from_numpy(masks.view(np.int32)).to(sv.kp.device),
+                          torch.from_numpy(levels).to(sv.kp.device))
+    return sv.tables[key]
+
+
+def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
+    """(index, sweep) estimated GPU seconds of explaining rows of these
+    lengths through the per-SV kernels (index None when it does not apply)."""
+    norient = 2 if kp.rc else 1
+    pair_s = E.comparisons(kp, x_lengths, sv.seqs.lengths, norient) / SWEEP_COMPARISONS_PER_S
+    if kp.l > E.INDEX_MAX_L or cmax < 0:
+        return None, pair_s
+    from math import comb
+
+    weight = np.abs(np.asarray(ta)) + np.abs(np.asarray(tb))
+    n_masks = sum(comb(kp.l, m) * 3**m for m in range(cmax + 1) if weight[m] != 0)
+    wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
+    per_lmer = norient * float(np.clip(sv.seqs.lengths - kp.l + 1, 0, None).sum()) / 4**kp.l
+    probes = wx * n_masks
+    index_s = probes / INDEX_EXPLAIN_PROBES_PER_S + probes * per_lmer / INDEX_EXPLAIN_ENTRIES_PER_S
+    return index_s, pair_s
+
+
+def per_sv_seconds(model, x_lengths, perturbation: bool) -> float:
+    """Estimated GPU seconds of the cheaper per-SV explain kernel for rows of
+    these lengths (what an explain table has to beat)."""
+    kp = model._params()
+    ta, tb, cmax = explain_tables(kp.base, perturbation)
+    index_s, pair_s = _path_seconds(model._svs(), kp, x_lengths, ta, tb, cmax)
+    if index_s is None or model.use_lmer_index is False:
+        return pair_s
+    return index_s if model.use_lmer_index is True else min(index_s, pair_s)
+
+
+def _explain_kernel(model, sv, kp, xs, ta, tb, cmax) -> str:
+    """The per-SV explain kernel for these rows (EXPLAIN_KERNEL, and under
+    "auto" the cheaper of the index and the sweep by the cost model;
+    model.use_lmer_index = False keeps the sweep, True the index)."""
+    if EXPLAIN_KERNEL != "auto":
+        return EXPLAIN_KERNEL
+    use = model.use_lmer_index
+    if use is False or kp.l > E.INDEX_MAX_L or cmax < 0:
+        return "sweep"
+    if use is True:
+        return "index"
+    index_s, pair_s = _path_seconds(sv, kp, xs.lengths, ta, tb, cmax)
+    return "index" if index_s < pair_s else "sweep"
 
 
 def _explain_rows_table(model, codes, lens, A, pert, out):

@@ -195,12 +195,9 @@ class GkmSVM:
             if use == "auto":
                 if T.explain_table_bytes(kp.l) > T.EXPLAIN_TABLE_BYTES:
                     return None
-                from .explain import SWEEP_COMPARISONS_PER_S, explain_tables
+                from .explain import per_sv_seconds
 
-                norient = 2 if kp.rc else 1
-                sweep_s = E.comparisons(kp, x_lengths, sv.seqs.lengths, norient) \
-                    / SWEEP_COMPARISONS_PER_S
-                if sweep_s < T.explain_build_seconds(sv):
+                if per_sv_seconds(self, x_lengths, perturbation) < T.explain_build_seconds(sv):
                     return None
             from .explain import explain_tables
 

@@ -138,6 +138,23 @@ int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int
             const double *tab, const double *coef, const double *ds, const double *dx,
             const double *dmut, int normalize, int rbf, double gamma, double *out, void *stream);
 
+/* GkmExplain through the l-mer index, ADDED into out [B][4][lmax]: every
+ * query window u (xwin [B][Wx], PAD past each row's windows) visits the l-mers
+ * v = u ^ masks[t] (levels[t] mismatches, the levels 0..cmax whose explain
+ * weight is non-zero); the index buckets of v in every SV tile (the
+ * gkm_index_hist CSR) give s = sum_e ec[b][sv_e] * vals_e, which adds
+ * s * ta[m] at matching positions (to base u_k; perturbation: to the three
+ * other bases) and s * tb[m] at mismatching ones (to base v_k), times the
+ * query window's weight (wtab, or 1).  chunk_windows: windows per block.
+ * l <= 13, B <= 65535.  _explain_index_cpu, _distribute
+ * (explain/cpu.py:68-86, 146-207); explain_lmer (explain/cuda.py:174-351). */
+int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B, int Wx,
+                      int lmax, const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                      const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                      const uint8_t *wtab, int S, int tile, int n_tiles, int chunk_windows,
+                      const double *ta, const double *tb, const double *ec, double *out,
+                      void *stream);
+
 /* The same histograms through an l-mer index of the SV windows, ADDED into
  * hist [B][S][nm]: xwin [B][Wx] packed query windows (gkm_pack_windows), in any
  * order within a row (PAD last), with xpos [B][Wx] their positions in the row

@@ -196,6 +196,39 @@ def test_gkmexplain_fixtures(gpu, name, mode, lm):
         np.testing.assert_allclose(got[i, :, : e.shape[1]], e, rtol=2e-5, atol=5e-6 * scale)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_explain_kernels_agree_with_oracle(gpu, seed):
+    """Both per-SV explain kernels (the l-mer index; lanes = diagonals of one
+    pair) against the oracle's per-SV loop on random models: l from 3 to 13
+    (the index to 12), all non-RBF and RBF types, weights, RC on/off, ragged
+    SVs and queries (some shorter than l), long queries."""
+    from gkmsvm_amd import explain as X
+    from gkmsvm_amd import gkmexplain
+
+    rng = np.random.default_rng(100 + seed)
+    l = int(rng.choice([3, 6, 10, 11, 12, 13]))
+    kt = int(rng.integers(0, 7))
+    k = int(rng.integers(max(1, l - 5), l))
+    p = _params(kt, l, k, rng)
+    if kt != 0:
+        p["d"] = int(rng.integers(1, min(4, l)))
+    yc, yl = _rand_codes(rng, int(rng.integers(70, 200)), 1, 260)
+    m = _model(kt, p, yc, yl, rng.normal(size=len(yl)), 0.3)
+    xc, xl = _rand_codes(rng, 5, max(1, l - 2), 700)
+    om = _oracle_model(m)
+    for mode in ("hypothetical", "perturbation"):
+        want = O.explain(om, xc, xl, mode)
+        for kernel in ("index", "sweep"):
+            if kernel == "index" and l > 12:
+                continue
+            X.EXPLAIN_KERNEL = kernel
+            try:
+                got = gkmexplain(m, xc, mode=mode, lengths=xl).cpu().numpy()
+            finally:
+                X.EXPLAIN_KERNEL = "auto"
+            H.assert_close(got, want, 1e-10, f"{kernel} {mode} l={l} t={kt}")
+
+
 @pytest.mark.parametrize("name", sorted(k for k in H.EXPLAIN_SETS if not k.endswith("_51")))
 def test_mutation_impact_fixtures(gpu, name):
     """score_variants(method='gkmexplain') equals the oracle and gkmexplain -m 5."""
