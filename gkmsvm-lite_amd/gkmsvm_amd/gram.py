@@ -10,17 +10,44 @@ from . import engine as E
 from .inputs import as_batch, full_lengths
 
 
+def _gram_index(kp, x_lengths, ys, force: bool = False):
+    """An l-mer index of the columns when it is the cheaper path (the cost
+    model of SVModel.use_index_for, the columns as SVs) or forced, else None
+    (also when the index does not apply)."""
+    if kp.l > E.INDEX_MAX_L or kp.dmax < 0:
+        return None
+    if force:
+        return E.DeviceIndex(kp, ys)
+    from math import comb
+
+    norient = 2 if kp.rc else 1
+    wy = float(np.clip(ys.lengths - kp.l + 1, 0, None).sum())
+    wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
+    n_masks = sum(comb(kp.l, m) * 3**m for m in range(kp.dmax + 1) if kp.h[m] != 0)
+    n_tiles = -(-ys.n // E.index_tile(kp.nm, not kp.weighted))
+    probes = wx * n_masks
+    index_s = (probes * n_tiles / E.INDEX_PROBES_PER_S
+               + probes * norient * wy / 4**kp.l / E.INDEX_ENTRIES_PER_S)
+    # the pairwise kernel skips the lower triangle; the index probes every row
+    brute_s = 0.5 * wx * norient * wy / E.BRUTE_COMPARISONS_PER_S
+    return E.DeviceIndex(kp, ys) if index_s < brute_s else None
+
+
 def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = False,
                  out: np.ndarray | None = None, lengths=None, y_lengths=None,
-                 device=None) -> np.ndarray:
+                 device=None, use_index="auto") -> np.ndarray:
     """[N, M] (or [N, N]) normalised, finished kernel matrix in host memory.
 
     Without Y, only the upper triangle (with the diagonal) is computed and
     mirrored, so the matrix is exactly symmetric.  Rows go to the GPU in blocks
     whose [rows, N] histograms stay under engine.MAX_HIST_BYTES; sequences are
     sorted by length on the device (64-column groups of one length) and the
-    result is put back in input order.  chunk_size is accepted for API
-    compatibility; the block size follows from memory.
+    result is put back in input order.  The histograms come from the pairwise
+    kernel or an l-mer index of the columns (use_index "auto": the cheaper by
+    the cost model; True/False force it) -- the same integers either way, as
+    the reference builds the Gram through its index (gram.py:82-106).
+    chunk_size is accepted for API compatibility; the block size follows from
+    memory.
     """
     dev = E.device_of(device)
     kp = E.KernelParams(kernel, dev)
@@ -43,6 +70,9 @@ def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = F
         oy = np.argsort(yl, kind="stable")
         ys = E.DeviceSeqs(yc[oy], yl[oy], dev, rc=kp.rc)
     groups = torch.from_numpy(E.group_bounds(yl[oy])).to(dev)
+    index = _gram_index(kp, xl, ys, use_index is True) if use_index is not False else None
+    if use_index is True and index is None:
+        raise ValueError("the l-mer index needs l <= 12 and a mismatch cutoff")
     ones_x = torch.ones(N, dtype=torch.float64, device=dev)
     dx = E.self_kraw(kp, xs) if kp.normalize else ones_x
     dy = dx if symmetric else (E.self_kraw(kp, ys) if kp.normalize
@@ -57,7 +87,10 @@ def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = F
         rows = tqdm(rows, desc="Gram matrix")
     for r0 in rows:
         r1 = min(N, r0 + step)
-        hist = E.cross_hist(kp, xs, r0, r1, ys, groups, col_min=0 if symmetric else -1)
+        if index is not None:
+            hist = E.index_hist(kp, index, xs, r0, r1, M, col_min=0 if symmetric else -1)
+        else:
+            hist = E.cross_hist(kp, xs, r0, r1, ys, groups, col_min=0 if symmetric else -1)
         if tile is None or tile.shape[0] != r1 - r0:
             tile = torch.empty((r1 - r0, M), dtype=torch.float64, device=dev)
         _lib.call("gkm_combine_gram", E._p(hist), r1 - r0, M, kp.nm, E._p(kp.tab),

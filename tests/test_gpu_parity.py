@@ -281,7 +281,8 @@ def test_score_variants_kernel(gpu):
 @pytest.mark.parametrize("kt", [2, 3, 4, 0])
 def test_gram_matches_oracle(gpu, kt):
     """Symmetric Gram (upper triangle computed, mirrored, exactly symmetric) and
-    a rectangular one, ragged lengths, against the oracle."""
+    a rectangular one, ragged lengths, through the pairwise kernel and the
+    l-mer index, against the oracle."""
     from gkmsvm_amd import compute_gram
     from gkmsvm_amd.kernels import build_kernel
 
@@ -290,12 +291,15 @@ def test_gram_matches_oracle(gpu, kt):
     kern = build_kernel(kt, p)
     okern = O.Kern.make(kt, p)
     xc, xl = _rand_codes(rng, 140, 40, 160)
-    g = compute_gram(kern, xc, lengths=xl)
-    assert (g == g.T).all()
-    H.assert_close(g, O.gram(okern, xc, xl), 1e-10, "symmetric")
     yc, yl = _rand_codes(rng, 70, 30, 90)
-    g2 = compute_gram(kern, xc, yc, lengths=xl, y_lengths=yl)
-    H.assert_close(g2, O.gram(okern, xc, xl, yc, yl), 1e-10, "rectangular")
+    want = O.gram(okern, xc, xl)
+    want2 = O.gram(okern, xc, xl, yc, yl)
+    for use_index in (False, True):
+        g = compute_gram(kern, xc, lengths=xl, use_index=use_index)
+        assert (g == g.T).all()
+        H.assert_close(g, want, 1e-10, f"symmetric, index={use_index}")
+        g2 = compute_gram(kern, xc, yc, lengths=xl, y_lengths=yl, use_index=use_index)
+        H.assert_close(g2, want2, 1e-10, f"rectangular, index={use_index}")
 
 
 def test_plain_scoring_config_against_index_oracle(gpu):
