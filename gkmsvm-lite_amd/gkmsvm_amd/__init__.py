@@ -33,10 +33,11 @@ from .serialization import (
     save_npz,
 )
 from .svm import GkmSVM, to_cpu
-from .train import train_gkmsvm
+from .train import train_gkmsvm, train_gkmsvr
 
 __all__ = [
     "GkmSVM", "gkmexplain", "mutation_impact", "ism", "compute_gram", "train_gkmsvm",
+    "train_gkmsvr",
     "load_lsgkm_model", "read_fasta", "one_hot_encode", "encode_codes", "to_codes",
     "codes_to_one_hot", "to_cpu", "resolve_kernel_type", "KERNEL_BUILDERS", "KERNEL_ALIASES",
     "DirectGkmKernel", "EstTruncGkmKernel", "CenterWeightedGkmKernel", "RbfKernel",

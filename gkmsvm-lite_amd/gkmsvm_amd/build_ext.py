@@ -5,6 +5,7 @@ libgkmhip.so   the HIP kernels and their C-ABI (hipcc, gfx950): the pairwise,
                the exact l-mer tables and DeltaSVM (gkm_tables.hip)
 libgkm_emu.so  a CPU build of the same lane code, used by the CPU test suite
                to check the bit-sliced algorithm without a GPU
+libgkm_smo.so  the SMO training solver (host C, kernel columns by callback)
 
 Run: python gkmsvm-lite_amd/gkmsvm_amd/build_ext.py
 """
@@ -58,8 +59,18 @@ def build_emu() -> str:
     return out
 
 
+def build_smo() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libgkm_smo.so")
+    tmp = out + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-o", tmp,
+          os.path.join(CSRC, "gkm_smo.c"), "-lm"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> list[str]:
-    return [build_hip(), build_emu()]
+    return [build_hip(), build_emu(), build_smo()]
 
 
 if __name__ == "__main__":

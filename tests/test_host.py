@@ -178,6 +178,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.gkm_planes_words(300) == 300 // 32 + 4
 
 
+def test_smo_library_exports_its_declared_symbol():
+    """libgkm_smo.so exports what include/gkm_smo.h declares."""
+    from gkmsvm_amd.train import smo_lib
+
+    src = open(os.path.join(ROOT, "include", "gkm_smo.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(gkm_[a-z0-9_]+)\s*\(", src)))
+    assert names == ["gkm_smo_solve"]
+    lib = smo_lib()
+    for n in names:
+        assert hasattr(lib, n), n
+
+
 def test_library_rejects_bad_arguments_without_gpu_work():
     """Argument checks fail before any launch (l out of range)."""
     from gkmsvm_amd import _lib
