@@ -1015,8 +1015,8 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                     const double sa = sv * stab[m], sb = sv * stab[16 + m];
 #pragma unroll
                     for (int k = 0; k < LM; k++) {
-                        if (LT == 0 && k >= l)
-                            break;
+                        // digits past l are 0: with LT == 0 am[k >= l] gathers a
+                        // sum nobody reads (no dynamic indexes, no scratch)
                         const uint32_t d = (mask >> (2 * k)) & 3u;
                         am[k] += d == 0 ? sa : 0.0;
                         ad[k][0] += d == 1 ? sb : 0.0;
@@ -1031,8 +1031,6 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
         const double aw = a.wtab ? (double)window_weight(a.wtab, wx, i) : 1.0;
 #pragma unroll
         for (int k = 0; k < LM; k++) {
-            if (LT == 0 && k >= l)
-                break;
             for (int off = WAVE / 2; off > 0; off >>= 1) {
                 am[k] += __shfl_xor(am[k], off, WAVE);
                 ad[k][0] += __shfl_xor(ad[k][0], off, WAVE);
@@ -1044,9 +1042,10 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
             const int k = lane >> 2, slot = lane & 3;
             double val = 0.0;
 #pragma unroll
-            for (int kk = 0; kk < LM; kk++)
+            for (int kk = 0; kk < LM; kk++)  // static indexes only (no scratch)
                 if (kk == k)
-                    val = slot == 0 ? am[kk] : ad[kk][slot - 1];
+                    val = slot == 0 ? am[kk] : slot == 1 ? ad[kk][0] : slot == 2 ? ad[kk][1]
+                                                                            : ad[kk][2];
             val *= aw;
             const uint32_t uk = (u >> (2 * k)) & 3u;
             const int pos = i - i0 + k;
