@@ -805,6 +805,16 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
 // ---------------------------------------------------------------------------
 
 constexpr int INDEX_THREADS = 1024;  // 16 waves share one LDS tile (one block per CU)
+constexpr int INDEX_PROBES = 1;      // probes per lane per group (64 INDEX_PROBES masks)
+constexpr int INDEX_WALK = 12;       // entries per probe per round of the lane walk
+constexpr int INDEX_WALK_MAX_C = 48; // buckets up to this size are walked by their lane
+constexpr int INDEX_PASSES = 4;      // 64-entry passes of a balanced walk issued together
+constexpr int RECORD_IDS = 12;       // ids held in a 32-byte bucket record
+
+// A bucket's [start, end) offsets in one 8-byte load (4-byte aligned).
+struct __attribute__((aligned(4))) bounds2 {
+    uint32_t x, y;
+};
 
 struct IndexArgs {
     const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
@@ -813,30 +823,43 @@ struct IndexArgs {
     const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
     const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
     const uint8_t *vals;     // [entries] window weight of each entry, or NULL
+    const uint32_t *records; // [keys][8] start, count, first RECORD_IDS ids (uint16)
     const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood (sorted)
     const uint8_t *levels;   // [n_masks] mismatch count of each mask
     const uint8_t *wtab;     // weight by distance, or NULL
-    uint32_t *partial;       // [nchunks][B][S][nm] per-chunk counts
+    void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK16: uint16)
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
     int64_t col_min, row_base;
 };
 
 // Block = (SV tile, chunk of query windows, query row).  Its LDS holds the
-// tile's counts [tile][nm]; each wave takes a query window and its lanes 64
-// neighbour masks at a time: v = w ^ mask is an l-mer within the mask's
-// mismatch count of w, and every window of a tile SV equal to v adds (its
-// weight times the query window's) to N[level] of that pair -- the pairs
-// kernels/lmer_index_cpu.py:23-37 (_add_window) visits.  The tile's counts go
-// out with plain stores to the chunk's slice of `partial`.
+// tile's counts [tile][nm]; each wave takes a query window and its lanes
+// 64 * INDEX_PROBES neighbour masks at a time: v = w ^ mask is an l-mer within
+// the mask's mismatch count of w, and every window of a tile SV equal to v
+// adds (its weight times the query window's) to N[level] of that pair -- the
+// pairs kernels/lmer_index_cpu.py:23-37 (_add_window) visits.  The tile's
+// counts go out with plain stores to the chunk's slice of `partial`.
 // PACK16: two 16-bit counts per 32-bit LDS word (unweighted; the host caps the
 // window chunk so no count of a block can reach 2^16), which doubles the SVs a
 // tile holds and halves the probes.
+// Memory: the kernel is bound by random cache lines -- per neighbour l-mer the
+// bucket bounds and the bucket's ids.  REC: the bucket records (start, count,
+// the first RECORD_IDS ids) put both in one 32-byte load, and the n_tiles
+// records of an l-mer share a line, which the tile blocks of a chunk fetch
+// together (same XCD, see below).  One block fills a CU's LDS, so a CU runs 16
+// waves; each wave loads the next group's records (or bounds) before walking
+// the current group's buckets.  When no bucket of the group holds more than
+// INDEX_WALK_MAX_C entries each lane walks its own bucket, the ids past the
+// record INDEX_WALK per round with the round's loads issued together.  Larger
+// buckets (repeats) are walked wave-balanced: the wave's buckets laid end to
+// end, each pass's entries located by a binary search over the prefix sums.
 // Block order: the XCDs take blocks round-robin by linear id (gridDim.x is a
 // multiple of 8), so block p runs on XCD p % 8.  The blocks of one window
 // chunk -- one per SV tile, probing the same offsets lines -- are given ids
 // congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
-// chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk exit at once.
-template <bool PACK16>
+// chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk, and chunks past
+// the row's last window, exit at once.
+template <bool PACK16, bool REC>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -847,6 +870,10 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     if (chunk >= a.nchunks)
         return;
     const int b = blockIdx.y;
+    const int wx = a.xlen[b] - a.l + 1;
+    const int i0 = chunk * a.chunk_windows;
+    if (i0 >= wx)
+        return;  // no windows of this row: the reduce skips this chunk
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int wps = PACK16 ? (a.nm + 1) / 2 : a.nm;  // LDS words per SV
@@ -855,74 +882,171 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         cnt[i] = 0u;
     __syncthreads();
     const uint32_t nl = 1u << (2 * a.l);
-    const int wx = a.xlen[b] - a.l + 1;
-    const int i0 = chunk * a.chunk_windows;
     const int i1 = min(wx, i0 + a.chunk_windows);
     const int s_base = tile_id * a.tile;
     const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
-    for (int i = i0 + wave; i < i1; i += INDEX_THREADS / WAVE) {
-        const uint32_t w = a.xwin[(int64_t)b * a.Wx + i];
-        if (w >= nl)
-            continue;  // PAD
+    auto count = [&](int sid, int lv, uint32_t add) {
+        if (sid < lo_col)
+            return;
+        if (PACK16)
+            atomicAdd(&cnt[sid * wps + (lv >> 1)], 1u << (16 * (lv & 1)));
+        else
+            atomicAdd(&cnt[sid * a.nm + lv], add);
+    };
+    // The wave's (window, mask group) pairs in order: probe p of lane t in
+    // group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
+    // v = w ^ mask[m], whose bucket is [e0, e1).
+    const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
+    int ni = i0 + wave, ng = 0;  // the pair being fetched
+    uint4 nra[INDEX_PROBES], nrb[INDEX_PROBES];  // record, or (start, end) in nra.x/y
+    int nlv[INDEX_PROBES];
+    auto fetch = [&]() {
+        const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
+#pragma unroll
+        for (int p = 0; p < INDEX_PROBES; ++p) {
+            const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
+            nra[p] = nrb[p] = make_uint4(0u, 0u, 0u, 0u);
+            nlv[p] = 0;
+            if (w < nl && t < a.n_masks) {
+                const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
+                nlv[p] = a.levels[t];
+                if (REC) {
+                    const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
+                    nra[p] = r[0];
+                    nrb[p] = r[1];
+                } else {
+                    const bounds2 r = *(const bounds2 *)(a.offsets + key);
+                    nra[p].x = r.x;
+                    nra[p].y = r.y - r.x;
+                }
+            }
+        }
+    };
+    fetch();
+    while (ni < i1) {
+        const int i = ni;
+        uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][RECORD_IDS / 2];
+        int c[INDEX_PROBES], lv[INDEX_PROBES];
+        bool small = true;
+#pragma unroll
+        for (int p = 0; p < INDEX_PROBES; ++p) {
+            e0[p] = nra[p].x;
+            c[p] = (int)nra[p].y;
+            lv[p] = nlv[p];
+            small = small && c[p] <= INDEX_WALK_MAX_C;
+            rid[p][0] = nra[p].z;
+            rid[p][1] = nra[p].w;
+            rid[p][2] = nrb[p].x;
+            rid[p][3] = nrb[p].y;
+            rid[p][4] = nrb[p].z;
+            rid[p][5] = nrb[p].w;
+        }
+        if (++ng == n_groups) {
+            ng = 0;
+            ni += INDEX_THREADS / WAVE;
+        }
+        fetch();
         const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
         const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
-        for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
-            // lane t probes neighbour v = w ^ mask[t]: its bucket [e0, e0 + c)
-            const int t = t0 + lane;
-            uint32_t e0 = 0;
-            int c = 0, lv = 0;
-            if (t < a.n_masks) {
-                const uint32_t v = w ^ a.masks[t];
-                lv = a.levels[t];
-                const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
-                e0 = a.offsets[key];
-                c = (int)(a.offsets[key + 1] - e0);
+        if (__ballot(!small) == 0) {
+            // lane walk: every probe's next INDEX_WALK entries per round
+            int cmax = c[0];
+#pragma unroll
+            for (int p = 1; p < INDEX_PROBES; ++p)
+                cmax = max(cmax, c[p]);
+            if (REC) {  // the ids in the records
+#pragma unroll
+                for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                    for (int k = 0; k < RECORD_IDS; ++k)
+                        if (k < c[p])
+                            count((int)((rid[p][k >> 1] >> (16 * (k & 1))) & 0xFFFFu), lv[p], 1u);
             }
-            // the wave's buckets laid end to end (exclusive prefix sums), then
-            // walked 64 entries at a time: lane j of a pass takes entry base + j,
-            // found in probe p = the last lane with excl[p] <= base + j
-            int incl = c;
+            for (int kb = REC ? RECORD_IDS : 0; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
+                int sid[INDEX_PROBES][INDEX_WALK];
+                uint32_t wv[INDEX_PROBES][INDEX_WALK];
+#pragma unroll
+                for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                    for (int k = 0; k < INDEX_WALK; ++k) {
+                        const bool in = kb + k < c[p];
+                        const uint32_t e = e0[p] + (uint32_t)(kb + k);
+                        sid[p][k] = in ? (int)a.ids[e] : 0;
+                        if (!PACK16)
+                            wv[p][k] = in && a.vals ? a.vals[e] : 1u;
+                    }
+#pragma unroll
+                for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                    for (int k = 0; k < INDEX_WALK; ++k)
+                        if (kb + k < c[p])
+                            count(sid[p][k], lv[p], PACK16 ? 1u : aw * wv[p][k]);
+            }
+            continue;
+        }
+#pragma unroll
+        for (int p = 0; p < INDEX_PROBES; ++p) {
+            // balanced walk of this probe's buckets: laid end to end (exclusive
+            // prefix sums) and walked 64 entries per pass: lane j of a pass takes
+            // entry base + j, in the bucket of the last lane r with excl[r] <= base + j
+            int incl = c[p];
             for (int off = 1; off < WAVE; off <<= 1) {
                 const int u = __shfl_up(incl, off, WAVE);
                 incl += lane >= off ? u : 0;
             }
-            const int excl = incl - c;
+            const int excl = incl - c[p];
             const int total = __shfl(incl, WAVE - 1, WAVE);
-            for (int base = 0; base < total; base += WAVE) {
-                const int j = base + lane;
-                int pr = 0;
+            // INDEX_PASSES passes at a time: their searches, gathers and id
+            // loads are independent, so their latencies overlap
+            for (int base = 0; base < total; base += INDEX_PASSES * WAVE) {
+                int pr[INDEX_PASSES];
+#pragma unroll
+                for (int u = 0; u < INDEX_PASSES; ++u)
+                    pr[u] = 0;
+#pragma unroll
                 for (int step = WAVE / 2; step > 0; step >>= 1) {
-                    const int ex = __shfl(excl, pr + step, WAVE);
-                    pr += ex <= j ? step : 0;
+#pragma unroll
+                    for (int u = 0; u < INDEX_PASSES; ++u) {
+                        const int ex = __shfl(excl, pr[u] + step, WAVE);
+                        pr[u] += ex <= base + u * WAVE + lane ? step : 0;
+                    }
                 }
-                const uint32_t pe0 = (uint32_t)__shfl((int)e0, pr, WAVE);
-                const int plv = __shfl(lv, pr, WAVE);
-                const int pex = __shfl(excl, pr, WAVE);
-                if (j >= total)
-                    continue;
-                const uint32_t e = pe0 + (uint32_t)(j - pex);
-                const int sid = a.ids[e];
-                if (sid < lo_col)
-                    continue;
-                if (PACK16) {
-                    atomicAdd(&cnt[sid * wps + (plv >> 1)], 1u << (16 * (plv & 1)));
-                } else {
-                    const uint32_t add = a.vals ? aw * a.vals[e] : 1u;
-                    atomicAdd(&cnt[sid * a.nm + plv], add);
+                uint32_t e[INDEX_PASSES];
+                int plv[INDEX_PASSES], sid[INDEX_PASSES];
+#pragma unroll
+                for (int u = 0; u < INDEX_PASSES; ++u) {
+                    const int j = base + u * WAVE + lane;
+                    const uint32_t pe0 = (uint32_t)__shfl((int)e0[p], pr[u], WAVE);
+                    plv[u] = __shfl(lv[p], pr[u], WAVE);
+                    const int pex = __shfl(excl, pr[u], WAVE);
+                    e[u] = pe0 + (uint32_t)(j - pex);
+                    sid[u] = j < total ? (int)a.ids[e[u]] : -1;
                 }
+#pragma unroll
+                for (int u = 0; u < INDEX_PASSES; ++u)
+                    if (sid[u] >= 0)
+                        count(sid[u], plv[u], PACK16 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
             }
         }
     }
     __syncthreads();
     const int ns = min(a.tile, a.S - s_base);
-    uint32_t *dst = a.partial + (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
-    for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
-        if (PACK16) {
-            const int sid = i / a.nm, lv = i - sid * a.nm;
-            dst[i] = (cnt[sid * wps + (lv >> 1)] >> (16 * (lv & 1))) & 0xFFFFu;
-        } else {
+    const int64_t off = (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
+    if (PACK16 && (a.nm & 1) == 0) {
+        // the LDS words are the tile's [ns][nm] uint16 counts in order
+        uint32_t *dst = (uint32_t *)((uint16_t *)a.partial + off);
+        for (int i = threadIdx.x; i < ns * wps; i += INDEX_THREADS)
             dst[i] = cnt[i];
+    } else if (PACK16) {
+        uint16_t *dst = (uint16_t *)a.partial + off;
+        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+            const int sid = i / a.nm, lv = i - sid * a.nm;
+            dst[i] = (uint16_t)(cnt[sid * wps + (lv >> 1)] >> (16 * (lv & 1)));
         }
+    } else {
+        uint32_t *dst = (uint32_t *)a.partial + off;
+        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS)
+            dst[i] = cnt[i];
     }
 }
 
@@ -1075,23 +1199,28 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
     }
 }
 
-// hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m].
-__global__ void k_index_reduce(const uint32_t *__restrict__ partial, int64_t n, int nchunks,
-                               unsigned long long *__restrict__ hist)
+// hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m], over the
+// chunks that hold windows of row b (k_index_tile writes no others).
+template <typename T>
+__global__ void k_index_reduce(const T *__restrict__ partial, int64_t n, int64_t row_n,
+                               const int32_t *__restrict__ xlen, int l, int chunk_windows,
+                               int nchunks, unsigned long long *__restrict__ hist)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    const int wx = xlen[i / row_n] - l + 1;
+    const int used = wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
     unsigned long long v = 0;
-    for (int c = 0; c < nchunks; c++)
+    for (int c = 0; c < used; c++)
         v += partial[(int64_t)c * n + i];
     hist[i] += v;
 }
 
 GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                            const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                           const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
-                           int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                           const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
+                           int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
                            int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
                            void *partial, void *hist, void *stream)
 {
@@ -1121,7 +1250,7 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.masks = masks;
     a.levels = levels;
     a.wtab = wtab;
-    a.partial = (uint32_t *)partial;
+    a.partial = partial;
     a.B = B;
     a.Wx = Wx;
     a.n_masks = n_masks;
@@ -1134,19 +1263,31 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     a.col_min = col_min;
     a.row_base = row_base;
-    const void *kfn = pack16 ? (const void *)k_index_tile<true> : (const void *)k_index_tile<false>;
+    const bool rec = pack16 && records != nullptr;
+    a.records = records;
+    const void *kfn = rec ? (const void *)k_index_tile<true, true>
+                    : pack16 ? (const void *)k_index_tile<true, false>
+                             : (const void *)k_index_tile<false, false>;
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
     dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
-    if (pack16)
-        hipLaunchKernelGGL(k_index_tile<true>, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    if (rec)
+        hipLaunchKernelGGL((k_index_tile<true, true>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    else if (pack16)
+        hipLaunchKernelGGL((k_index_tile<true, false>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     else
-        hipLaunchKernelGGL(k_index_tile<false>, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+        hipLaunchKernelGGL((k_index_tile<false, false>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_hist");
     int64_t n = (int64_t)B * S * nm;
-    hipLaunchKernelGGL(k_index_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S_(stream),
-                       (const uint32_t *)partial, n, a.nchunks, (unsigned long long *)hist);
+    if (pack16)
+        hipLaunchKernelGGL(k_index_reduce<uint16_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           S_(stream), (const uint16_t *)partial, n, (int64_t)S * nm, xlen, l,
+                           chunk_windows, a.nchunks, (unsigned long long *)hist);
+    else
+        hipLaunchKernelGGL(k_index_reduce<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           S_(stream), (const uint32_t *)partial, n, (int64_t)S * nm, xlen, l,
+                           chunk_windows, a.nchunks, (unsigned long long *)hist);
     GKM_CHECK_LAUNCH("gkm_index_hist reduce");
     return 0;
 }

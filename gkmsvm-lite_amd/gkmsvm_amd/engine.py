@@ -319,6 +319,22 @@ class DeviceIndex:
         if self.n_entries >= 2**31:
             raise MemoryError("l-mer index over 2^31 entries")
         self.offsets = offsets.to(torch.int32)  # uint32 bits
+        # bucket records of unweighted indexes: (start, count, first
+        # RECORD_IDS ids as uint16) in 32 bytes per key, so the n_tiles
+        # records of an l-mer share a cache line and a probe is one load
+        self.records = None
+        nk = nl * self.n_tiles
+        if self.pack16 and nk * 32 <= RECORD_MAX_BYTES:
+            rec = torch.zeros((nk, 16), dtype=torch.int16, device=dev)
+            r32 = rec.view(torch.int32)
+            r32[:, 0] = self.offsets[:-1]
+            r32[:, 1] = (offsets[1:] - offsets[:-1]).to(torch.int32)
+            ks = key[order]
+            rank = torch.arange(self.n_entries, device=dev) - offsets[ks]
+            sel = rank < RECORD_IDS
+            rec.view(-1)[ks[sel] * 16 + 4 + rank[sel]] = self.ids[sel]
+            self.records = r32
+            del rec, ks, rank, sel
         masks, levels = neighbor_masks(l, kp.h, kp.dmax)
         self.masks = torch.from_numpy(masks.view(np.int32)).to(dev)
         self.levels = torch.from_numpy(levels).to(dev)
@@ -332,6 +348,8 @@ class DeviceIndex:
         t = [self.ids, self.offsets, self.masks, self.levels]
         if self.vals is not None:
             t.append(self.vals)
+        if self.records is not None:
+            t.append(self.records)
         return sum(x.numel() * x.element_size() for x in t)
 
 
@@ -345,6 +363,8 @@ def index_tile(nm: int, pack16: bool = False) -> int:
 
 
 MAX_PARTIAL_BYTES = 2 << 30
+RECORD_IDS = 12              # ids held in a bucket record
+RECORD_MAX_BYTES = 4 << 30   # larger record tables (l >= 13 with many tiles) are not built
 
 
 def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: int, S: int, *,
@@ -360,7 +380,7 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     # rows per launch: the per-chunk partials of the smallest chunk (16 windows,
     # or the 16-bit cap) must fit the budget
     min_cw = 16 if not idx.pack16 else max(1, min(16, 65535 // max(1, idx.max_sv_windows)))
-    per_row = -(-Wx // min_cw) * S * kp.nm * 4
+    per_row = -(-Wx // min_cw) * S * kp.nm * (2 if idx.pack16 else 4)
     step = max(1, min(B, MAX_PARTIAL_BYTES // max(1, per_row), 65535))
     for r0 in range(0, B, step):
         r1 = min(B, r0 + step)
@@ -382,7 +402,8 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     elif idx.pack16:  # nor the 16-bit ones: a window meets an SV window at most once
         cw = max(1, min(cw, 65535 // max(1, idx.max_sv_windows)))
     nchunks = -(-Wx // cw)
-    partial = torch.empty((nchunks, B, S, kp.nm), dtype=torch.int32, device=kp.device)
+    partial = torch.empty((nchunks, B, S, kp.nm), device=kp.device,
+                          dtype=torch.int16 if idx.pack16 else torch.int32)
     wt, _ = kp.wtab(xs.lmax)
     # each row's windows in l-mer order (PAD last): a block's windows then share
     # high digits, and so do their neighbours' offsets -- cache lines reused
@@ -393,7 +414,8 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
            lambda: _lib.call(
                "gkm_index_hist", _p(swin), _p(spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
-               _p(idx.ids), _p(idx.vals), _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l,
+               _p(idx.ids), _p(idx.vals), _p(idx.records), _p(idx.masks), _p(idx.levels),
+               idx.n_masks, kp.l,
                kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, int(idx.pack16), col_min, x0,
                _p(partial), _p(out), stream_handle(stream)))
     return out

@@ -161,20 +161,24 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * (needed for window weights; NULL: in order); the
  * index is a CSR over keys lmer * n_tiles + tile (offsets [4^l n_tiles + 1]
  * uint32; ids uint16 SV within its tile of `tile` SVs; vals uint8 window
- * weights or NULL) of every SV window's l-mer, forward and RC; masks / levels
+ * weights or NULL) of every SV window's l-mer, forward and RC; records: NULL
+ * or, for an unweighted index, [4^l n_tiles][8] uint32 bucket records (start,
+ * count, then the bucket's first 12 ids as uint16), one 32-byte load per
+ * probe; masks / levels
  * [n_masks] the XOR neighbourhood of mismatch counts < nm with non-zero
  * weight.  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
- * uint32.  pack_counts (unweighted only): 16-bit LDS counts, two per word (the
- * caller keeps chunk_windows * windows per SV below 2^16).  l <= 13,
+ * counts, uint16 with pack_counts, else uint32.  pack_counts (unweighted
+ * only): 16-bit counts, two per LDS word (the caller keeps chunk_windows *
+ * windows per SV below 2^16).  l <= 13,
  * B <= 65535, tile * nm * 4 (packed: tile * ceil(nm/2) * 4) bytes <= 160 KiB.
  * LmerIndex.pairwise (kernels/lmer_index.py:239-268, 400-441;
  * kernels/lmer_index_cpu.py:23-67). */
 int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                    const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                   const uint32_t *masks, const uint8_t *levels, int n_masks, int l, int nm,
-                   const uint8_t *wtab, int S, int tile, int n_tiles, int chunk_windows,
-                   int pack_counts, int64_t col_min, int64_t row_base, void *partial, void *hist,
-                   void *stream);
+                   const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
+                   int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                   int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
+                   void *partial, void *hist, void *stream);
 
 /* ---- Exact l-mer tables of non-RBF models (gkm_tables.hip) ----------------
  * V, table  [4^l] float64 indexed by the packed l-mer (base k in bits 2k).

@@ -363,6 +363,34 @@ def test_index_histograms_equal_brute_force(gpu, seed):
     np.testing.assert_array_equal(s_idx, s_bru)
 
 
+def test_index_mixed_bucket_sizes(gpu):
+    """Buckets of a few entries (walked by their lanes) and of thousands (poly-A
+    and dinucleotide repeats, walked wave-balanced) in the same launch give the
+    brute-force integers, weighted and unweighted."""
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(17)
+    L = 400
+    sv = rng.integers(0, 4, (120, L)).astype(np.uint8)
+    sv[:12] = 0                      # poly-A
+    sv[12:20] = np.tile([0, 2], L // 2)  # AGAG...
+    q = rng.integers(0, 4, (6, L)).astype(np.uint8)
+    q[0, 100:250] = 0
+    q[1, :] = np.tile([0, 2], L // 2)
+    q[2, 50:120] = 0
+    for kt, p in ((2, {"L": 11, "k": 7, "d": 3}), (4, {"L": 10, "k": 6, "d": 3, "M": 50, "H": 50})):
+        m = _model(kt, p, sv, np.full(len(sv), L), rng.normal(size=len(sv)), 0.0)
+        s = m._svs()
+        xs = E.DeviceSeqs(q, np.full(len(q), L), gpu, rc=s.kp.rc)
+        idx = s.index()
+        assert idx.n_entries / (4 ** s.kp.l) < 1 and int(
+            (idx.offsets[1:] - idx.offsets[:-1]).max()) > 1000  # both kinds of bucket
+        a = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S)
+        b = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 def test_index_plain_config(gpu):
     """Plain -t 2 scoring through the l-mer index equals the oracle's index path."""
     rng = np.random.default_rng(1)
