@@ -809,7 +809,21 @@ constexpr int INDEX_PROBES = 1;      // probes per lane per group (64 INDEX_PROB
 constexpr int INDEX_WALK = 12;       // entries per probe per round of the lane walk
 constexpr int INDEX_WALK_MAX_C = 48; // buckets up to this size are walked by their lane
 constexpr int INDEX_PASSES = 4;      // 64-entry passes of a balanced walk issued together
-constexpr int RECORD_IDS = 12;       // ids held in a 32-byte bucket record
+constexpr int RECORD_IDS = 12;       // ids held in a 32-byte bucket record (unweighted)
+constexpr int RECORD_IDS_W = 8;      // (id, weight) pairs held in one (weighted index)
+
+// Bucket records, 8 words: start, count, then (RK 1) RECORD_IDS uint16 ids or
+// (RK 2) RECORD_IDS_W uint16 ids and their uint8 weights; wd = words 2..7.
+template <int RK>
+constexpr int record_ids() { return RK == 2 ? RECORD_IDS_W : RK == 1 ? RECORD_IDS : 0; }
+__device__ __forceinline__ uint32_t rec_id(const uint32_t *wd, int k)
+{
+    return (wd[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+}
+__device__ __forceinline__ uint32_t rec_val(const uint32_t *wd, int k)
+{
+    return (wd[4 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
+}
 
 // A bucket's [start, end) offsets in one 8-byte load (4-byte aligned).
 struct __attribute__((aligned(4))) bounds2 {
@@ -823,7 +837,7 @@ struct IndexArgs {
     const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
     const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
     const uint8_t *vals;     // [entries] window weight of each entry, or NULL
-    const uint32_t *records; // [keys][8] start, count, first RECORD_IDS ids (uint16)
+    const uint32_t *records; // [keys][8] bucket records (RK 1 or 2), or NULL
     const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood (sorted)
     const uint8_t *levels;   // [n_masks] mismatch count of each mask
     const uint8_t *wtab;     // weight by distance, or NULL
@@ -843,8 +857,8 @@ struct IndexArgs {
 // window chunk so no count of a block can reach 2^16), which doubles the SVs a
 // tile holds and halves the probes.
 // Memory: the kernel is bound by random cache lines -- per neighbour l-mer the
-// bucket bounds and the bucket's ids.  REC: the bucket records (start, count,
-// the first RECORD_IDS ids) put both in one 32-byte load, and the n_tiles
+// bucket bounds and the bucket's ids.  RK: the bucket records (start, count,
+// the first ids and weights) put both in one 32-byte load, and the n_tiles
 // records of an l-mer share a line, which the tile blocks of a chunk fetch
 // together (same XCD, see below).  One block fills a CU's LDS, so a CU runs 16
 // waves; each wave loads the next group's records (or bounds) before walking
@@ -859,7 +873,7 @@ struct IndexArgs {
 // congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
 // chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk, and chunks past
 // the row's last window, exit at once.
-template <bool PACK16, bool REC>
+template <bool PACK16, int RK>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             if (w < nl && t < a.n_masks) {
                 const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
                 nlv[p] = a.levels[t];
-                if (REC) {
+                if (RK) {
                     const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
                     nra[p] = r[0];
                     nrb[p] = r[1];
@@ -925,7 +939,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     fetch();
     while (ni < i1) {
         const int i = ni;
-        uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][RECORD_IDS / 2];
+        uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
         int c[INDEX_PROBES], lv[INDEX_PROBES];
         bool small = true;
 #pragma unroll
@@ -954,15 +968,14 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
             for (int p = 1; p < INDEX_PROBES; ++p)
                 cmax = max(cmax, c[p]);
-            if (REC) {  // the ids in the records
 #pragma unroll
-                for (int p = 0; p < INDEX_PROBES; ++p)
+            for (int p = 0; p < INDEX_PROBES; ++p)  // the entries in the records
 #pragma unroll
-                    for (int k = 0; k < RECORD_IDS; ++k)
-                        if (k < c[p])
-                            count((int)((rid[p][k >> 1] >> (16 * (k & 1))) & 0xFFFFu), lv[p], 1u);
-            }
-            for (int kb = REC ? RECORD_IDS : 0; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
+                for (int k = 0; k < record_ids<RK>(); ++k)
+                    if (k < c[p])
+                        count((int)rec_id(rid[p], k), lv[p],
+                              PACK16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
+            for (int kb = record_ids<RK>(); __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
                 int sid[INDEX_PROBES][INDEX_WALK];
                 uint32_t wv[INDEX_PROBES][INDEX_WALK];
 #pragma unroll
@@ -1055,6 +1068,11 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 // ---------------------------------------------------------------------------
 
 constexpr int IXEXP_THREADS = 256;
+// SV tiles whose buckets a lane gathers together, entries per bucket and round
+// (records: a first round of RECORD_IDS from the record, then IXEXP_ROUND)
+constexpr int IXEXP_TILES_REC = 2;
+constexpr int IXEXP_TILES = 4;
+constexpr int IXEXP_ROUND = 4;
 
 struct IndexExplainArgs {
     const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
@@ -1062,6 +1080,7 @@ struct IndexExplainArgs {
     const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
     const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
     const uint8_t *vals;     // [entries] window weight of each entry, or NULL
+    const uint32_t *records; // [keys][8] bucket records (unweighted), or NULL
     const uint32_t *masks;   // [n_masks] XOR neighbourhood (levels 0..cmax)
     const uint8_t *levels;   // [n_masks]
     const uint8_t *wtab;     // query window weight by distance, or NULL
@@ -1081,7 +1100,11 @@ struct IndexExplainArgs {
 // registers (am[k]: matches; ad[k][d - 1]: mismatches with digit delta d);
 // after the window's last mask the wave adds them up and 4 l lanes put them
 // into the block's [4][lmax] LDS sums, which go out with FP64 atomics.
-template <int LT>
+// Probe walk: a lane takes the buckets of XT tiles at a time -- their
+// records (RK: start, count, the first ids and weights) or bounds -- and gathers
+// the first round of coefficients of all of them with independent loads; a
+// bucket's entries past the first round follow IXEXP_ROUND at a time.
+template <int LT, int RK>
 __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArgs a)
 {
     constexpr int LM = LT > 0 ? LT : 15;
@@ -1123,15 +1146,71 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
             if (t < a.n_masks) {
                 const uint32_t mask = a.masks[t];
                 const uint32_t v = u ^ mask;
-                const uint32_t *off = a.offsets + (int64_t)v * nt;
                 double sv = 0.0;
-                uint32_t e = off[0];
-                for (int tl = 0; tl < nt; tl++) {
-                    const uint32_t e1 = off[tl + 1];
-                    const int base = tl * a.tile;
-                    for (; e < e1; e++) {
-                        const double c = ecb[base + a.ids[e]];
-                        sv += a.vals ? c * (double)a.vals[e] : c;
+                constexpr int R0 = RK ? record_ids<RK>() : IXEXP_ROUND;  // first round
+                constexpr int XT = RK ? IXEXP_TILES_REC : IXEXP_TILES;
+                for (int tl0 = 0; tl0 < nt; tl0 += XT) {
+                    uint32_t e0[XT], rid[XT][R0], rwt[XT][R0];
+                    int c[XT];
+#pragma unroll
+                    for (int x = 0; x < XT; ++x) {
+                        const uint32_t key = v * (uint32_t)nt + (uint32_t)(tl0 + x);
+                        e0[x] = 0u;
+                        c[x] = 0;
+                        if (tl0 + x < nt) {
+                            if (RK) {
+                                const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
+                                const uint4 ra = r[0], rb = r[1];
+                                const uint32_t wd[6] = {ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+                                e0[x] = ra.x;
+                                c[x] = (int)ra.y;
+#pragma unroll
+                                for (int k = 0; k < R0; ++k) {
+                                    rid[x][k] = rec_id(wd, k);
+                                    rwt[x][k] = RK == 2 ? rec_val(wd, k) : 1u;
+                                }
+                            } else {
+                                const bounds2 r = *(const bounds2 *)(a.offsets + key);
+                                e0[x] = r.x;
+                                c[x] = (int)(r.y - r.x);
+                            }
+                        }
+                    }
+                    double cv[XT][R0];
+#pragma unroll
+                    for (int x = 0; x < XT; ++x) {
+                        const double *ect = ecb + (size_t)(tl0 + x) * a.tile;
+#pragma unroll
+                        for (int k = 0; k < R0; ++k) {
+                            const bool in = k < c[x];
+                            if (!RK) {
+                                rid[x][k] = in ? a.ids[e0[x] + k] : 0u;
+                                rwt[x][k] = a.vals && in ? a.vals[e0[x] + k] : 1u;
+                            }
+                            cv[x][k] = in ? ect[rid[x][k]] * (double)rwt[x][k] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int x = 0; x < XT; ++x)
+#pragma unroll
+                        for (int k = 0; k < R0; ++k)
+                            sv += cv[x][k];
+#pragma unroll
+                    for (int x = 0; x < XT; ++x) {
+                        const double *ect = ecb + (size_t)(tl0 + x) * a.tile;
+                        for (int kb = R0; kb < c[x]; kb += IXEXP_ROUND) {
+                            double cw[IXEXP_ROUND];
+#pragma unroll
+                            for (int k = 0; k < IXEXP_ROUND; ++k) {
+                                const bool in = kb + k < c[x];
+                                const uint32_t e = e0[x] + (uint32_t)(kb + k);
+                                const double wv = a.vals && in ? (double)a.vals[e] : 1.0;
+                                cw[k] = in ? ect[a.ids[e]] * wv : 0.0;
+                            }
+#pragma unroll
+                            for (int k = 0; k < IXEXP_ROUND; ++k)
+                                sv += cw[k];
+                        }
                     }
                 }
                 if (sv != 0.0) {
@@ -1263,21 +1342,18 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     a.col_min = col_min;
     a.row_base = row_base;
-    const bool rec = pack16 && records != nullptr;
+    const int rk = records == nullptr ? 0 : vals ? 2 : 1;
     a.records = records;
-    const void *kfn = rec ? (const void *)k_index_tile<true, true>
-                    : pack16 ? (const void *)k_index_tile<true, false>
-                             : (const void *)k_index_tile<false, false>;
+    typedef void (*tile_fn)(IndexArgs);
+    const tile_fn kfn = pack16 ? (rk ? k_index_tile<true, 1> : k_index_tile<true, 0>)
+                               : rk == 2 ? k_index_tile<false, 2>
+                               : rk == 1 ? k_index_tile<false, 1> : k_index_tile<false, 0>;
     if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
     dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
-    if (rec)
-        hipLaunchKernelGGL((k_index_tile<true, true>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
-    else if (pack16)
-        hipLaunchKernelGGL((k_index_tile<true, false>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
-    else
-        hipLaunchKernelGGL((k_index_tile<false, false>), grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_hist");
     int64_t n = (int64_t)B * S * nm;
     if (pack16)
@@ -1687,7 +1763,8 @@ GKM_API int gkm_explain(int perturbation, const void *xplanes, const int32_t *xl
 
 GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B,
                               int Wx, int lmax, const uint32_t *offsets, const uint16_t *ids,
-                              const uint8_t *vals, const uint32_t *masks, const uint8_t *levels,
+                              const uint8_t *vals, const uint32_t *records,
+                              const uint32_t *masks, const uint8_t *levels,
                               int n_masks, int l, const uint8_t *wtab, int S, int tile,
                               int n_tiles, int chunk_windows, const double *ta, const double *tb,
                               const double *ec, double *out, void *stream)
@@ -1707,6 +1784,7 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
     a.offsets = offsets;
     a.ids = ids;
     a.vals = vals;
+    a.records = records;
     a.masks = masks;
     a.levels = levels;
     a.wtab = wtab;
@@ -1729,12 +1807,14 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
     if (lds > 64 * 1024)
         return fail(-2, "gkm_index_explain: chunk_windows %d too large", chunk_windows);
     dim3 grid((unsigned)((Wx + chunk_windows - 1) / chunk_windows), (unsigned)B);
-    switch (l) {
-    case 10: hipLaunchKernelGGL(k_index_explain<10>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
-    case 11: hipLaunchKernelGGL(k_index_explain<11>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
-    case 12: hipLaunchKernelGGL(k_index_explain<12>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
-    default: hipLaunchKernelGGL(k_index_explain<0>, grid, dim3(IXEXP_THREADS), lds, S_(stream), a); break;
-    }
+    typedef void (*ixexp_fn)(IndexExplainArgs);
+    const int rk = records == nullptr ? 0 : vals ? 2 : 1;
+#define GKM_IXEXP(LT) (rk == 2 ? k_index_explain<LT, 2> : rk == 1 ? k_index_explain<LT, 1> \
+                                                                : k_index_explain<LT, 0>)
+    const ixexp_fn fn = l == 10 ? GKM_IXEXP(10) : l == 11 ? GKM_IXEXP(11)
+                      : l == 12 ? GKM_IXEXP(12) : GKM_IXEXP(0);
+#undef GKM_IXEXP
+    hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_explain");
     return 0;
 }

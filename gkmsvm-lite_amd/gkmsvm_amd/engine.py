@@ -266,8 +266,9 @@ def neighbor_masks(l: int, table, dmax: int):
 # per second of gkm_cross_hist; (window, mask, SV tile) probes and index
 # entries per second of gkm_index_hist.
 BRUTE_COMPARISONS_PER_S = 1.8e13
-INDEX_PROBES_PER_S = 5.0e10
-INDEX_ENTRIES_PER_S = 1.0e12
+INDEX_PROBES_PER_S = 1.0e11          # unweighted: bucket records
+INDEX_PROBES_PER_S_WEIGHTED = 5.0e10  # weighted: bounds, then ids and weights
+INDEX_ENTRIES_PER_S = 2.0e12
 INDEX_MAX_L = 12
 
 
@@ -319,20 +320,24 @@ class DeviceIndex:
         if self.n_entries >= 2**31:
             raise MemoryError("l-mer index over 2^31 entries")
         self.offsets = offsets.to(torch.int32)  # uint32 bits
-        # bucket records of unweighted indexes: (start, count, first
-        # RECORD_IDS ids as uint16) in 32 bytes per key, so the n_tiles
-        # records of an l-mer share a cache line and a probe is one load
+        # bucket records, 32 bytes per key: (start, count, the first ids as
+        # uint16 -- RECORD_IDS of them, or RECORD_IDS_W followed by their uint8
+        # weights), so the n_tiles records of an l-mer share a cache line and a
+        # probe is one load
         self.records = None
         nk = nl * self.n_tiles
-        if self.pack16 and nk * 32 <= RECORD_MAX_BYTES:
+        if (self.pack16 or self.vals is not None) and nk * 32 <= RECORD_MAX_BYTES:
             rec = torch.zeros((nk, 16), dtype=torch.int16, device=dev)
             r32 = rec.view(torch.int32)
             r32[:, 0] = self.offsets[:-1]
             r32[:, 1] = (offsets[1:] - offsets[:-1]).to(torch.int32)
             ks = key[order]
             rank = torch.arange(self.n_entries, device=dev) - offsets[ks]
-            sel = rank < RECORD_IDS
+            sel = rank < (RECORD_IDS if self.vals is None else RECORD_IDS_W)
             rec.view(-1)[ks[sel] * 16 + 4 + rank[sel]] = self.ids[sel]
+            if self.vals is not None:
+                rb = rec.view(torch.uint8)
+                rb.view(-1)[ks[sel] * 32 + 24 + rank[sel]] = self.vals[sel].to(torch.uint8)
             self.records = r32
             del rec, ks, rank, sel
         masks, levels = neighbor_masks(l, kp.h, kp.dmax)
@@ -364,6 +369,7 @@ def index_tile(nm: int, pack16: bool = False) -> int:
 
 MAX_PARTIAL_BYTES = 2 << 30
 RECORD_IDS = 12              # ids held in a bucket record
+RECORD_IDS_W = 8             # (id, weight) pairs held in a record of a weighted index
 RECORD_MAX_BYTES = 4 << 30   # larger record tables (l >= 13 with many tiles) are not built
 
 
@@ -476,7 +482,8 @@ class SVModel:
         per_lmer = norient * ws / 4**self.kp.l
         probes = n_windows * n_masks
         n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted))
-        index_s = probes * n_tiles / INDEX_PROBES_PER_S + probes * per_lmer / INDEX_ENTRIES_PER_S
+        rate = INDEX_PROBES_PER_S_WEIGHTED if self.kp.weighted else INDEX_PROBES_PER_S
+        index_s = probes * n_tiles / rate + probes * per_lmer / INDEX_ENTRIES_PER_S
         return index_s, brute_s
 
     def query_seconds(self, n_windows: float) -> float:

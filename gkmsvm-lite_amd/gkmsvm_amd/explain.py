@@ -66,8 +66,8 @@ SWEEP_COMPARISONS_PER_S = 5e12
 # gkm_index_explain rates (MI355X, l = 11): (window, mask) probes and index
 # entries per second; with the sweep's rate, the cost model that picks the
 # per-SV explain kernel
-INDEX_EXPLAIN_PROBES_PER_S = 2.2e10
-INDEX_EXPLAIN_ENTRIES_PER_S = 7e10
+INDEX_EXPLAIN_PROBES_PER_S = 3.5e10   # (window, mask, SV tile) bucket records read
+INDEX_EXPLAIN_ENTRIES_PER_S = 2.0e11
 
 # Per-SV explain kernel: "auto" (the l-mer index when the cost model favours
 # it and the model allows it, else the sweep), "index" (gkm_index_explain:
@@ -166,7 +166,8 @@ def _explain_rows(model, codes, lens, resolved, out):
                  lambda: float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks) * 8.0,
                  None, lambda: _lib.call(
                      "gkm_index_explain", int(pert), E._p(fwd), E._p(xs.len), B, Wx, xs.lmax,
-                     E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(masks),
+                     E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(idx.records),
+                     E._p(masks),
                      E._p(levels), len(masks), kp.l, E._p(wq), sv.S, idx.tile, idx.n_tiles,
                      chunk, E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
     else:
@@ -204,7 +205,9 @@ def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
     wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
     per_lmer = norient * float(np.clip(sv.seqs.lengths - kp.l + 1, 0, None).sum()) / 4**kp.l
     probes = wx * n_masks
-    index_s = probes / INDEX_EXPLAIN_PROBES_PER_S + probes * per_lmer / INDEX_EXPLAIN_ENTRIES_PER_S
+    n_tiles = -(-sv.S // E.index_tile(kp.nm, not kp.weighted))
+    index_s = (probes * n_tiles / INDEX_EXPLAIN_PROBES_PER_S
+               + probes * per_lmer / INDEX_EXPLAIN_ENTRIES_PER_S)
     return index_s, pair_s
 
 

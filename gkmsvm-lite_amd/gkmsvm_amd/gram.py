@@ -26,7 +26,8 @@ def _gram_index(kp, x_lengths, ys, force: bool = False):
     n_masks = sum(comb(kp.l, m) * 3**m for m in range(kp.dmax + 1) if kp.h[m] != 0)
     n_tiles = -(-ys.n // E.index_tile(kp.nm, not kp.weighted))
     probes = wx * n_masks
-    index_s = (probes * n_tiles / E.INDEX_PROBES_PER_S
+    rate = E.INDEX_PROBES_PER_S_WEIGHTED if kp.weighted else E.INDEX_PROBES_PER_S
+    index_s = (probes * n_tiles / rate
                + probes * norient * wy / 4**kp.l / E.INDEX_ENTRIES_PER_S)
     # the pairwise kernel skips the lower triangle; the index probes every row
     brute_s = 0.5 * wx * norient * wy / E.BRUTE_COMPARISONS_PER_S

@@ -145,12 +145,15 @@ int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int
  * gkm_index_hist CSR) give s = sum_e ec[b][sv_e] * vals_e, which adds
  * s * ta[m] at matching positions (to base u_k; perturbation: to the three
  * other bases) and s * tb[m] at mismatching ones (to base v_k), times the
- * query window's weight (wtab, or 1).  chunk_windows: windows per block.
+ * query window's weight (wtab, or 1).  records: the bucket records of an
+ * unweighted index (see gkm_index_hist), or NULL.  chunk_windows: windows per
+ * block.
  * l <= 13, B <= 65535.  _explain_index_cpu, _distribute
  * (explain/cpu.py:68-86, 146-207); explain_lmer (explain/cuda.py:174-351). */
 int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B, int Wx,
                       int lmax, const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                      const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                      const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
+                      int n_masks, int l,
                       const uint8_t *wtab, int S, int tile, int n_tiles, int chunk_windows,
                       const double *ta, const double *tb, const double *ec, double *out,
                       void *stream);
