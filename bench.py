@@ -28,8 +28,9 @@ with the most GPU time in the value run (the others under "others"; the
 table run's under config.served_lmer_tables.roofline).  Pairwise kernels
 (k_cross_hist, k_self_hist, k_sweep): window-pair comparisons x 6 integer
 operations (XOR, shift, OR, AND, popcount, accumulate) / launch time,
-against the int32 VALU peak.  HBM-bound kernels (k_index_tile: 8 B offsets
-per probe + 2-3 B per entry; k_table_score: 4 B window + 8 B table entry per
+against the int32 VALU peak.  HBM-bound kernels (k_index_tile and
+k_index_explain: 16 B per (window, mask) probe + 12 B per hit, a 4 B id and
+an 8 B FP64 read-modify-write; k_table_score: 4 B window + 8 B table entry per
 window; k_table_explain: 4 B window + 32 l B table row per window + 32 B
 output per position) / launch time, against 8 TB/s.
 cpu_baseline: the reference's CPU algorithm on the host cores (the oracle, C
@@ -191,10 +192,22 @@ HBM_PEAK = 8.0e12  # bytes/s
 
 
 HBM_KINDS = {
-    "k_index_tile": "8 B offsets per (window, mask, SV tile) probe + 2 B id (+1 B weight) per entry",
+    "k_index_tile": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
+    "k_index_explain": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
     "k_table_score": "4 B packed window + 8 B table entry per query window",
     "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",
 }
+
+
+# HBM traffic per launch (bytes) of the serving run's per-SV kernels: mean per
+# dispatch of FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from device copies of
+# known size) in rocprofv3 --pmc runs of this benchmark's serve command, as
+# recorded in profiles/serving_72kSV_mix_trace.txt.
+PROFILED_TRAFFIC = {
+    "k_index_tile": (5_299_166 * 2 + 231_620) * 1024.0,
+    "k_index_explain": (3_998_263 * 2 + 364) * 1024.0,
+}
+TRAFFIC_SOURCE = "profiles/serving_72kSV_mix_trace.txt (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
 
 
 def roofline_of(summary: dict) -> dict:
@@ -216,7 +229,9 @@ def roofline_of(summary: dict) -> dict:
                  "comparisons_per_launch": d["units_per_launch"],
                  "work": "6 int32 ops per window-pair comparison (XOR, SHR, OR, AND, POPC, add)"}
         r.update({"kernel": kind, "launches": d["launches"], "mean_launch_ms": d["mean_ms"],
-                  "total_ms": d["total_ms"], "traffic": None})
+                  "total_ms": d["total_ms"], "traffic": PROFILED_TRAFFIC.get(kind)})
+        if kind in PROFILED_TRAFFIC:
+            r["traffic_source"] = TRAFFIC_SOURCE
         rows[kind] = r
     top = max(rows.values(), key=lambda r: r["total_ms"])
     out = dict(top)

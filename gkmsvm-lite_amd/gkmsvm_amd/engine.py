@@ -427,13 +427,17 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     return out
 
 
+INDEX_PROBE_BYTES = 16.0  # algorithmic bytes per (window, mask) probe: bucket offsets
+INDEX_HIT_BYTES = 12.0    # per hit: a 4-byte id and an 8-byte FP64 read-modify-write
+
+
 def index_bytes(kp: KernelParams, idx: DeviceIndex, x_lengths) -> float:
-    """Algorithmic bytes of an index query: per (window, mask, tile) probe the
-    8-byte offsets pair, per entry its 2-byte SV id (+1 byte weight)."""
+    """Algorithmic bytes of an index query (the reference algorithm's traffic,
+    kernels/lmer_index_gpu.py:9-39): 16 per (window, mask) probe, 12 per hit
+    (SV window equal to the probed l-mer)."""
     wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
     probes = wx * idx.n_masks
-    per_entry = 3.0 if idx.vals is not None else 2.0
-    return probes * idx.n_tiles * 8.0 + probes * idx.per_lmer * per_entry
+    return probes * INDEX_PROBE_BYTES + probes * idx.per_lmer * INDEX_HIT_BYTES
 
 
 class SVModel:

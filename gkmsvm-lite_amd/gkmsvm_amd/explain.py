@@ -162,8 +162,9 @@ def _explain_rows(model, codes, lens, resolved, out):
         Wx = fwd.shape[1]
         wq, _ = kp.wtab(xs.lmax)
         chunk = max(4, min(64, -(-Wx * B // 1024)))
+        probes = float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks)
         E._timed("k_index_explain",
-                 lambda: float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks) * 8.0,
+                 lambda: probes * (E.INDEX_PROBE_BYTES + idx.per_lmer * E.INDEX_HIT_BYTES),
                  None, lambda: _lib.call(
                      "gkm_index_explain", int(pert), E._p(fwd), E._p(xs.len), B, Wx, xs.lmax,
                      E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(idx.records),
