@@ -58,7 +58,11 @@ import numpy as np  # noqa: E402
 S_SV = 72_000
 SV_LEN = 500
 PARAMS = {"L": 11, "k": 7, "d": 3}
-INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+# int32 lane-ops/s: measured by --mode peaks (chains of single v_xor_b32,
+# v_add_u32, v_bcnt_u32_b32 instructions, best launch shape), above the
+# 256 CUs x 64 lanes x 2.4 GHz = 3.93e13 derived from the clock.
+INT_VALU_PEAK = 5.08e13
+INT_VALU_DERIVED = 256 * 64 * 2.4e9
 OPS_PER_COMPARISON = 6
 
 
@@ -254,6 +258,8 @@ def kernel_only(args):
         torch.cuda.synchronize()
         print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
         return
+    if args.mode == "peaks":
+        return peaks(args)
     model, _, _ = make_model()
     sv = model._svs()
     kp = model._params()
@@ -289,6 +295,57 @@ def kernel_only(args):
             raise ValueError(args.mode)
     torch.cuda.synchronize()
     print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
+
+
+def peaks(args):
+    """The two peaks the rooflines use, measured: HBM read bandwidth (a
+    streaming read of 4 GiB, and torch's device copy of 2 GiB for comparison)
+    and the int32 VALU rate (chains of single XOR / add / popcount
+    instructions), each the best of several launch shapes, HIP events on the
+    launch stream."""
+    import torch
+
+    from gkmsvm_amd import _lib
+    from gkmsvm_amd import engine as E
+
+    st = torch.cuda.current_stream()
+    h = E.stream_handle(st)
+    sink = torch.zeros(4, dtype=torch.int32, device="cuda")
+
+    def timed(f, reps):
+        f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            f()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    nbytes = 4 << 30
+    src = torch.ones(nbytes // 8, dtype=torch.int64, device="cuda")
+    read = {}
+    for blocks in (2048, 8192, 32768):
+        t = timed(lambda: _lib.call("gkm_peak_read", src.data_ptr(), nbytes, sink.data_ptr(),
+                                    blocks, h), 10)
+        read[str(blocks)] = nbytes / t / 1e9
+    half = src[: src.numel() // 2]
+    dst = torch.empty_like(half)
+    t = timed(lambda: dst.copy_(half), 10)
+    copy_gbs = 2 * half.numel() * 8 / t / 1e9  # bytes read + written
+    del src, half, dst
+    valu = {}
+    rounds = 8192
+    for blocks in (2048, 8192, 32768):
+        t = timed(lambda: _lib.call("gkm_peak_valu", rounds, sink.data_ptr(), blocks, h), 3)
+        valu[str(blocks)] = blocks * 256.0 * rounds * 24 / t / 1e12
+    print(json.dumps({
+        "mode": "peaks",
+        "hbm_read_GBps": max(read.values()), "hbm_read_by_blocks": read,
+        "hbm_copy_read_plus_write_GBps": copy_gbs, "hbm_peak_GBps": HBM_PEAK / 1e9,
+        "valu_int32_TOPs": max(valu.values()), "valu_by_blocks": valu,
+        "valu_peak_used_TOPs": INT_VALU_PEAK / 1e12,
+        "valu_derived_TOPs": INT_VALU_DERIVED / 1e12}), flush=True)
 
 
 def _gc_seqs(rng, n, L, gc=0.42):
@@ -409,7 +466,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--mode", default="serve",
-                    choices=["serve", "cross", "index", "table", "copy", "configs"],
+                    choices=["serve", "cross", "index", "table", "copy", "peaks", "configs"],
                     help="serve: the benchmark; cross/index/table/copy: one kernel family on "
                          "the serving shape, for profiling; "
                          "configs: bounded timings of the other configurations")

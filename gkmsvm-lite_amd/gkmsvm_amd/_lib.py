@@ -50,6 +50,8 @@ SIGNATURES = {
     "gkm_table_ism": [P, P, P, I, I, I, I, P, P, P, P, P, I, P, P],
     "gkm_deltasvm_weights": [P, I, D, P, I, I, P, P],
     "gkm_deltasvm_score": [P, P, I, I, I, I, P, I, P, I, D, P, P],
+    "gkm_peak_read": [P, I64, P, I, P],
+    "gkm_peak_valu": [I, P, I, P],
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p}
 

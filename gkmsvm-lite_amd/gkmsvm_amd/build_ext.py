@@ -30,7 +30,7 @@ HIP_FLAGS = [
     f"-I{CSRC}", f"-I{INCLUDE}",
 ]
 
-HIP_SOURCES = ["gkm_kernels.hip", "gkm_tables.hip"]
+HIP_SOURCES = ["gkm_kernels.hip", "gkm_tables.hip", "gkm_peaks.hip"]
 
 
 def _run(cmd):

@@ -256,6 +256,16 @@ int gkm_deltasvm_score(const uint8_t *codes, const int32_t *xlen, int B, int lma
                        const int32_t *combos, int n_combos, const double *weights, int rc,
                        double bias, double *out, void *stream);
 
+/* ---- Peak microbenchmarks (gkm_peaks.hip) ---------------------------------
+ * Measurement only, no counterpart in the reference: the HBM read bandwidth
+ * and int32 VALU rate the hot kernels' rooflines are priced against.
+ * gkm_peak_read streams `bytes` (a multiple of 16) from src with `blocks`
+ * blocks of 256 threads; gkm_peak_valu runs `rounds` rounds of 24 int32 lane
+ * operations (8 chains of XOR, add, popcount) per thread on `blocks` blocks of
+ * 256 threads.  sink: 4 device bytes, written only if a sentinel comes up. */
+int gkm_peak_read(const void *src, int64_t bytes, void *sink, int blocks, void *stream);
+int gkm_peak_valu(int rounds, void *sink, int blocks, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
