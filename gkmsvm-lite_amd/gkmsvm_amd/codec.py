@@ -41,6 +41,29 @@ def encode_codes(sequence: str, *, allow_n: bool = False, allow_other: bool = Fa
     return np.where(raw > 3, 0, raw).astype(np.uint8)
 
 
+def encode_batch(sequences, min_width: int = 1):
+    """Codes of many DNA strings in one pass: ([n, max(Lmax, min_width)] uint8
+    padded with A, lengths [n] int64, bad [n] bool).  N is stored as A, as by
+    encode_codes(allow_n=True); a string with any other letter, or empty, is
+    flagged in bad (its row is left as decoded) for the caller to reject."""
+    n = len(sequences)
+    lens = np.fromiter((len(q) for q in sequences), dtype=np.int64, count=n)
+    raw = _LUT[np.frombuffer("".join(sequences).encode("ascii", "replace"), dtype=np.uint8)]
+    starts = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=starts[1:])
+    other = raw == 5
+    bad = lens == 0
+    if other.any():
+        cs = np.zeros(raw.size + 1, dtype=np.int64)
+        np.cumsum(other, out=cs[1:])
+        bad |= cs[starts[1:]] > cs[starts[:-1]]
+    raw[raw > 3] = 0
+    out = np.zeros((n, max(int(lens.max()) if n else 0, min_width)), dtype=np.uint8)
+    for i in range(n):
+        out[i, : lens[i]] = raw[starts[i]:starts[i + 1]]
+    return out, lens, bad
+
+
 def one_hot_encode(sequence: str, *, dtype=np.float32, allow_n: bool = False,
                    allow_other: bool = False) -> np.ndarray:
     """[4, L] one-hot (channels A, C, G, T); N is an all-zero column."""

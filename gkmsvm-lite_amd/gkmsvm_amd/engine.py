@@ -12,6 +12,8 @@ lanes) holds SVs of nearly one length, and stay resident until dropped.
 
 from __future__ import annotations
 
+import contextlib
+
 import math
 
 import numpy as np
@@ -58,8 +60,13 @@ class DeviceSeqs:
         self.nw = self.lmax // 32 + 4
         self.lengths = lengths
         self.device = device
-        self.codes = torch.from_numpy(codes).to(device, non_blocking=False)
-        self.len = torch.from_numpy(lengths.astype(np.int32)).to(device)
+        # staged through pinned host memory: asynchronous copies on the stream
+        # the packing runs on (the pinned buffers stay referenced with this object)
+        self._host = (torch.from_numpy(codes).pin_memory(),
+                      torch.from_numpy(lengths.astype(np.int32)).pin_memory())
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            self.codes = self._host[0].to(device, non_blocking=True)
+            self.len = self._host[1].to(device, non_blocking=True)
         self.planes = torch.empty((self.n, self.nw, 2), dtype=torch.int32, device=device)
         self.planes_rc = torch.empty_like(self.planes) if rc else None
         _lib.call("gkm_pack_planes", _p(self.codes), _p(self.len), self.n, self.lmax, self.nw,

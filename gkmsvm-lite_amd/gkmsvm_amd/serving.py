@@ -34,7 +34,7 @@ import torch
 
 from . import engine as E
 from . import tables as T
-from .codec import encode_codes
+from .codec import encode_batch
 from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
 
 
@@ -215,8 +215,9 @@ class GkmServer:
                 still.append(st)
                 continue
             now = time.perf_counter()
+            scores = st.score_host.numpy() if st.score_host is not None else None
             for req, r0, n in st.spans:
-                vals = st.score_host[r0:r0 + n].numpy()
+                vals = scores[r0:r0 + n]
                 if req.kind == "variant":
                     half = n // 2
                     res = vals[half:] - vals[:half]  # alt - ref
@@ -235,31 +236,34 @@ class GkmServer:
     def _launch(self, batch, stream) -> _Step:
         kp, sv, model = self.kp, self.sv, self.model
         t = time.perf_counter()
-        spans, codes_rows, keep = [], [], []
+        spans, strings, keep = [], [], []
         for req in batch:
             req.t_admit = t
             if req.kind == "score":
-                rows = [encode_codes(s, allow_n=True) for s in req.seqs]
+                rows = req.seqs
             elif req.kind == "variant":
-                rows = ([encode_codes(r, allow_n=True) for r, _ in req.seqs]
-                        + [encode_codes(a, allow_n=True) for _, a in req.seqs])
+                rows = [r for r, _ in req.seqs] + [a for _, a in req.seqs]
             else:
                 continue
-            spans.append((req, len(codes_rows), len(rows)))
-            codes_rows.extend(rows)
+            spans.append((req, len(strings), len(rows)))
+            strings.extend(rows)
         score_host = None
         explains = [r for r in batch if r.kind == "explain"]
         with torch.cuda.stream(stream):
-            if codes_rows:
-                lens = np.array([len(r) for r in codes_rows], dtype=np.int64)
-                codes = np.zeros((len(codes_rows), max(int(lens.max()), kp.l)), dtype=np.uint8)
-                for i, r in enumerate(codes_rows):
-                    codes[i, : len(r)] = r
+            if strings:
+                # every scoring row of the step encoded in one pass; a request
+                # with an empty string or a letter other than ACGTN fails alone
+                codes, lens, bad = encode_batch(strings, kp.l)
+                if bad.any():
+                    for req, r0, n in spans:
+                        if bad[r0:r0 + n].any():
+                            req.future.set_exception(ValueError(
+                                "Invalid sequence (empty, or a letter other than ACGTN)"))
+                    spans = [sp for sp in spans if not sp[0].future.done()]
                 xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
-                out = torch.empty(len(codes_rows), dtype=torch.float64, device=kp.device)
+                out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
                 E.score_rows(sv, xs, model.bias, out, table=self.table)
-                short = torch.from_numpy(lens < kp.l).to(kp.device)
-                out = torch.where(short, torch.full_like(out, float("nan")), out)
+                out = torch.where(xs.len < kp.l, torch.full_like(out, float("nan")), out)
                 score_host = torch.empty(out.shape, dtype=torch.float64, pin_memory=True)
                 score_host.copy_(out, non_blocking=True)
                 keep += [xs, out]

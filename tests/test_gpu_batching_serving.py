@@ -124,3 +124,28 @@ def test_server_admission_rejects_oversized_and_small_budget_still_exact(gpu):
         futs = [srv.submit("score", [q]) for q in qs]  # one row each: admitted 3 per step
         got = np.array([f.result(timeout=120)[0] for f in futs])
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("tables", [False, True])
+def test_server_bad_request_fails_alone(gpu, tables):
+    """A request with an empty string or a letter other than ACGTN fails with
+    ValueError (as model(x) would) while the requests of the same step score
+    as alone; short rows give NaN."""
+    from gkmsvm_amd.serving import GkmServer
+
+    m = _random_model(S=300, seed=6)
+    m.use_lmer_table = tables  # the path the server takes
+    qs = _queries(6, 20, 300, 11) + ["ACGTACG"]  # the last one shorter than l
+    want = m(qs).cpu().numpy().ravel()
+    with GkmServer(m, hbm_budget=1 << 30, tables=tables) as srv:
+        srv.stop()  # queue everything first, so the bad requests share a step
+        futs = [srv.submit("score", [q]) for q in qs[:3]]
+        bad = [srv.submit("score", ["ACGTXACGTACGTACGT"]), srv.submit("variant", [("", "")])]
+        futs += [srv.submit("score", [q]) for q in qs[3:]]
+        srv.start()
+        for f in bad:
+            with pytest.raises(ValueError):
+                f.result(timeout=60)
+        got = np.array([f.result(timeout=60)[0] for f in futs])
+    np.testing.assert_array_equal(got, want)
+    assert np.isnan(got[-1])

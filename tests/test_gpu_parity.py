@@ -371,9 +371,9 @@ def test_index_mixed_bucket_sizes(gpu):
 
     rng = np.random.default_rng(17)
     L = 400
-    sv = rng.integers(0, 4, (120, L)).astype(np.uint8)
-    sv[:12] = 0                      # poly-A
-    sv[12:20] = np.tile([0, 2], L // 2)  # AGAG...
+    sv = rng.integers(0, 4, (300, L)).astype(np.uint8)
+    sv[:180] = 0                     # poly-A: a bucket of 70,200 entries (past a record's count)
+    sv[180:190] = np.tile([0, 2], L // 2)  # AGAG...
     q = rng.integers(0, 4, (6, L)).astype(np.uint8)
     q[0, 100:250] = 0
     q[1, :] = np.tile([0, 2], L // 2)
@@ -384,7 +384,7 @@ def test_index_mixed_bucket_sizes(gpu):
         xs = E.DeviceSeqs(q, np.full(len(q), L), gpu, rc=s.kp.rc)
         idx = s.index()
         assert idx.n_entries / (4 ** s.kp.l) < 1 and int(
-            (idx.offsets[1:] - idx.offsets[:-1]).max()) > 1000  # both kinds of bucket
+            (idx.offsets[1:] - idx.offsets[:-1]).max()) > 65535  # both kinds of bucket
         a = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S)
         b = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups)
         torch.cuda.synchronize()
