@@ -60,13 +60,18 @@ class DeviceSeqs:
         self.nw = self.lmax // 32 + 4
         self.lengths = lengths
         self.device = device
-        # staged through pinned host memory: asynchronous copies on the stream
-        # the packing runs on (the pinned buffers stay referenced with this object)
-        self._host = (torch.from_numpy(codes).pin_memory(),
-                      torch.from_numpy(lengths.astype(np.int32)).pin_memory())
+        # codes and int32 lengths staged in one pinned host buffer and sent with
+        # one asynchronous copy on the stream the packing runs on (the pinned
+        # buffer stays referenced with this object)
+        nc = -(-codes.size // 4) * 4
+        self._host = torch.empty(nc + 4 * self.n, dtype=torch.uint8, pin_memory=True)
+        hb = self._host.numpy()
+        hb[: codes.size] = codes.reshape(-1)
+        hb[nc:].view(np.int32)[:] = lengths
         with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
-            self.codes = self._host[0].to(device, non_blocking=True)
-            self.len = self._host[1].to(device, non_blocking=True)
+            buf = self._host.to(device, non_blocking=True)
+        self.codes = buf[: codes.size].view(self.n, self.lmax)
+        self.len = buf[nc:].view(torch.int32)
         self.planes = torch.empty((self.n, self.nw, 2), dtype=torch.int32, device=device)
         self.planes_rc = torch.empty_like(self.planes) if rc else None
         _lib.call("gkm_pack_planes", _p(self.codes), _p(self.len), self.n, self.lmax, self.nw,
