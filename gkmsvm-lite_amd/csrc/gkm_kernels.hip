@@ -899,13 +899,15 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     const int i1 = min(wx, i0 + a.chunk_windows);
     const int s_base = tile_id * a.tile;
     const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
+    const int ns = min(a.tile, a.S - s_base);
+    // counts laid out [word][SV], so that a wave's atomics spread over the banks
     auto count = [&](int sid, int lv, uint32_t add) {
         if (sid < lo_col)
             return;
         if (PACK16)
-            atomicAdd(&cnt[sid * wps + (lv >> 1)], 1u << (16 * (lv & 1)));
+            atomicAdd(&cnt[(lv >> 1) * ns + sid], 1u << (16 * (lv & 1)));
         else
-            atomicAdd(&cnt[sid * a.nm + lv], add);
+            atomicAdd(&cnt[lv * ns + sid], add);
     };
     // The wave's (window, mask group) pairs in order: probe p of lane t in
     // group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
@@ -1043,23 +1045,26 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         }
     }
     __syncthreads();
-    const int ns = min(a.tile, a.S - s_base);
     const int64_t off = (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
     if (PACK16 && (a.nm & 1) == 0) {
-        // the LDS words are the tile's [ns][nm] uint16 counts in order
+        // word j of SV s holds levels 2j, 2j + 1: the [ns][nm] uint16 counts
         uint32_t *dst = (uint32_t *)((uint16_t *)a.partial + off);
-        for (int i = threadIdx.x; i < ns * wps; i += INDEX_THREADS)
-            dst[i] = cnt[i];
+        for (int i = threadIdx.x; i < ns * wps; i += INDEX_THREADS) {
+            const int sid = i / wps, j = i - sid * wps;
+            dst[i] = cnt[j * ns + sid];
+        }
     } else if (PACK16) {
         uint16_t *dst = (uint16_t *)a.partial + off;
         for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
             const int sid = i / a.nm, lv = i - sid * a.nm;
-            dst[i] = (uint16_t)(cnt[sid * wps + (lv >> 1)] >> (16 * (lv & 1)));
+            dst[i] = (uint16_t)(cnt[(lv >> 1) * ns + sid] >> (16 * (lv & 1)));
         }
     } else {
         uint32_t *dst = (uint32_t *)a.partial + off;
-        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS)
-            dst[i] = cnt[i];
+        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+            const int sid = i / a.nm, lv = i - sid * a.nm;
+            dst[i] = cnt[lv * ns + sid];
+        }
     }
 }
 
