@@ -814,8 +814,19 @@ constexpr int RECORD_IDS_W = 8;      // (id, weight) pairs held in one (weighted
 
 // Bucket records, 8 words: start, count, then (RK 1) RECORD_IDS uint16 ids or
 // (RK 2) RECORD_IDS_W uint16 ids and their uint8 weights; wd = words 2..7.
+// RK 3: one 64-byte record per l-mer for unweighted indexes of at most 4 SV
+// tiles, whose buckets are consecutive in ids: word 0 the start of the l-mer's
+// run, word 1 the tiles' counts as bytes (255: saturated, read the offsets),
+// then the run's first LREC_IDS ids (uint16; wd = words 2..15).
+constexpr int LREC_IDS = 28;
+constexpr int LREC_MAX_TILES = 4;
+
 template <int RK>
 constexpr int record_ids() { return RK == 2 ? RECORD_IDS_W : RK == 1 ? RECORD_IDS : 0; }
+__device__ __forceinline__ uint32_t lrec_id(const uint32_t *wd, int k)
+{
+    return (wd[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+}
 __device__ __forceinline__ uint32_t rec_id(const uint32_t *wd, int k)
 {
     return (wd[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
@@ -823,6 +834,14 @@ __device__ __forceinline__ uint32_t rec_id(const uint32_t *wd, int k)
 __device__ __forceinline__ uint32_t rec_val(const uint32_t *wd, int k)
 {
     return (wd[4 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
+}
+
+// The layout of the bucket records gkm_index_explain is passed: none (0), 32
+// bytes per key (1: unweighted, 2: weighted) or, unweighted with at most 4
+// tiles, 64 bytes per l-mer (3).
+static inline int record_kind(const uint32_t *records, const uint8_t *vals, int n_tiles)
+{
+    return records == nullptr ? 0 : vals ? 2 : n_tiles <= LREC_MAX_TILES ? 3 : 1;
 }
 
 // A bucket's [start, end) offsets in one 8-byte load (4-byte aligned).
@@ -914,26 +933,26 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     // v = w ^ mask[m], whose bucket is [e0, e1).
     const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
     int ni = i0 + wave, ng = 0;  // the pair being fetched
-    uint4 nra[INDEX_PROBES], nrb[INDEX_PROBES];  // record, or (start, end) in nra.x/y
+    uint4 nr[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count) in nr[.][0]
     int nlv[INDEX_PROBES];
     auto fetch = [&]() {
         const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
 #pragma unroll
         for (int p = 0; p < INDEX_PROBES; ++p) {
             const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
-            nra[p] = nrb[p] = make_uint4(0u, 0u, 0u, 0u);
+            nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
             nlv[p] = 0;
             if (w < nl && t < a.n_masks) {
                 const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
                 nlv[p] = a.levels[t];
                 if (RK) {
                     const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
-                    nra[p] = r[0];
-                    nrb[p] = r[1];
+                    nr[p][0] = r[0];
+                    nr[p][1] = r[1];
                 } else {
                     const bounds2 r = *(const bounds2 *)(a.offsets + key);
-                    nra[p].x = r.x;
-                    nra[p].y = r.y - r.x;
+                    nr[p][0].x = r.x;
+                    nr[p][0].y = r.y - r.x;
                 }
             }
         }
@@ -942,20 +961,21 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     while (ni < i1) {
         const int i = ni;
         uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
-        int c[INDEX_PROBES], lv[INDEX_PROBES];
+        int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
         bool small = true;
 #pragma unroll
         for (int p = 0; p < INDEX_PROBES; ++p) {
-            e0[p] = nra[p].x;
-            c[p] = (int)nra[p].y;
+            e0[p] = nr[p][0].x;
+            c[p] = (int)nr[p][0].y;
             lv[p] = nlv[p];
+            n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
             small = small && c[p] <= INDEX_WALK_MAX_C;
-            rid[p][0] = nra[p].z;
-            rid[p][1] = nra[p].w;
-            rid[p][2] = nrb[p].x;
-            rid[p][3] = nrb[p].y;
-            rid[p][4] = nrb[p].z;
-            rid[p][5] = nrb[p].w;
+            rid[p][0] = nr[p][0].z;
+            rid[p][1] = nr[p][0].w;
+            rid[p][2] = nr[p][1].x;
+            rid[p][3] = nr[p][1].y;
+            rid[p][4] = nr[p][1].z;
+            rid[p][5] = nr[p][1].w;
         }
         if (++ng == n_groups) {
             ng = 0;
@@ -977,14 +997,18 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                     if (k < c[p])
                         count((int)rec_id(rid[p], k), lv[p],
                               PACK16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
-            for (int kb = record_ids<RK>(); __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
+            int kmin = n_in[0];
+#pragma unroll
+            for (int p = 1; p < INDEX_PROBES; ++p)
+                kmin = min(kmin, n_in[p]);
+            for (int kb = kmin; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
                 int sid[INDEX_PROBES][INDEX_WALK];
                 uint32_t wv[INDEX_PROBES][INDEX_WALK];
 #pragma unroll
                 for (int p = 0; p < INDEX_PROBES; ++p)
 #pragma unroll
                     for (int k = 0; k < INDEX_WALK; ++k) {
-                        const bool in = kb + k < c[p];
+                        const bool in = kb + k >= n_in[p] && kb + k < c[p];
                         const uint32_t e = e0[p] + (uint32_t)(kb + k);
                         sid[p][k] = in ? (int)a.ids[e] : 0;
                         if (!PACK16)
@@ -994,7 +1018,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 for (int p = 0; p < INDEX_PROBES; ++p)
 #pragma unroll
                     for (int k = 0; k < INDEX_WALK; ++k)
-                        if (kb + k < c[p])
+                        if (kb + k >= n_in[p] && kb + k < c[p])
                             count(sid[p][k], lv[p], PACK16 ? 1u : aw * wv[p][k]);
             }
             continue;
@@ -1152,9 +1176,56 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                 const uint32_t mask = a.masks[t];
                 const uint32_t v = u ^ mask;
                 double sv = 0.0;
-                constexpr int R0 = RK ? record_ids<RK>() : IXEXP_ROUND;  // first round
-                constexpr int XT = RK ? IXEXP_TILES_REC : IXEXP_TILES;
-                for (int tl0 = 0; tl0 < nt; tl0 += XT) {
+                bool walked = false;
+                if (RK == 3) {
+                    // the l-mer's record: its run over all tiles, the first
+                    // LREC_IDS ids inline; run entry k is in the tile whose
+                    // prefix end pe first exceeds k
+                    const uint4 *r = (const uint4 *)(a.records + (size_t)v * 16);
+                    const uint4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+                    const uint32_t wd[14] = {r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x,
+                                             r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
+                    int pe[LREC_MAX_TILES], tot = 0;
+                    bool sat = false;
+#pragma unroll
+                    for (int u = 0; u < LREC_MAX_TILES; ++u) {
+                        const int cu = u < nt ? (int)((r0.y >> (8 * u)) & 0xFFu) : 0;
+                        sat = sat || cu == 0xFF;
+                        tot += cu;
+                        pe[u] = tot;
+                    }
+                    if (!sat) {
+                        const int n0 = min(tot, LREC_IDS);
+                        double cv[LREC_IDS];
+#pragma unroll
+                        for (int k = 0; k < LREC_IDS; ++k) {
+                            const int tk = (k >= pe[0]) + (k >= pe[1]) + (k >= pe[2]);
+                            cv[k] = k < n0 ? ecb[(size_t)tk * a.tile + lrec_id(wd, k)] : 0.0;
+                        }
+#pragma unroll
+                        for (int k = 0; k < LREC_IDS; ++k)
+                            sv += cv[k];
+                        for (int kb = LREC_IDS; kb < tot; kb += IXEXP_ROUND) {
+                            double cw[IXEXP_ROUND];
+#pragma unroll
+                            for (int k = 0; k < IXEXP_ROUND; ++k) {
+                                const int e = kb + k;
+                                const int tk = (e >= pe[0]) + (e >= pe[1]) + (e >= pe[2]);
+                                cw[k] = e < tot ? ecb[(size_t)tk * a.tile + a.ids[r0.x + (uint32_t)e]]
+                                                : 0.0;
+                            }
+#pragma unroll
+                            for (int k = 0; k < IXEXP_ROUND; ++k)
+                                sv += cw[k];
+                        }
+                        walked = true;
+                    }
+                }
+                // per tile: 32-byte records (RK 1, 2) or bounds (RK 0; RK 3 saturated)
+                constexpr int RT = RK == 3 ? 0 : RK;
+                constexpr int R0 = RT ? record_ids<RT>() : IXEXP_ROUND;  // first round
+                constexpr int XT = RT ? IXEXP_TILES_REC : IXEXP_TILES;
+                for (int tl0 = 0; tl0 < (walked ? 0 : nt); tl0 += XT) {
                     uint32_t e0[XT], rid[XT][R0], rwt[XT][R0];
                     int c[XT];
 #pragma unroll
@@ -1163,7 +1234,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                         e0[x] = 0u;
                         c[x] = 0;
                         if (tl0 + x < nt) {
-                            if (RK) {
+                            if (RT) {
                                 const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
                                 const uint4 ra = r[0], rb = r[1];
                                 const uint32_t wd[6] = {ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
@@ -1172,7 +1243,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
 #pragma unroll
                                 for (int k = 0; k < R0; ++k) {
                                     rid[x][k] = rec_id(wd, k);
-                                    rwt[x][k] = RK == 2 ? rec_val(wd, k) : 1u;
+                                    rwt[x][k] = RT == 2 ? rec_val(wd, k) : 1u;
                                 }
                             } else {
                                 const bounds2 r = *(const bounds2 *)(a.offsets + key);
@@ -1188,7 +1259,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
 #pragma unroll
                         for (int k = 0; k < R0; ++k) {
                             const bool in = k < c[x];
-                            if (!RK) {
+                            if (!RT) {
                                 rid[x][k] = in ? a.ids[e0[x] + k] : 0u;
                                 rwt[x][k] = a.vals && in ? a.vals[e0[x] + k] : 1u;
                             }
@@ -1347,7 +1418,7 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     a.col_min = col_min;
     a.row_base = row_base;
-    const int rk = records == nullptr ? 0 : vals ? 2 : 1;
+    const int rk = records == nullptr ? 0 : vals ? 2 : 1;  // 32-byte records per key
     a.records = records;
     typedef void (*tile_fn)(IndexArgs);
     const tile_fn kfn = pack16 ? (rk ? k_index_tile<true, 1> : k_index_tile<true, 0>)
@@ -1813,9 +1884,9 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
         return fail(-2, "gkm_index_explain: chunk_windows %d too large", chunk_windows);
     dim3 grid((unsigned)((Wx + chunk_windows - 1) / chunk_windows), (unsigned)B);
     typedef void (*ixexp_fn)(IndexExplainArgs);
-    const int rk = records == nullptr ? 0 : vals ? 2 : 1;
-#define GKM_IXEXP(LT) (rk == 2 ? k_index_explain<LT, 2> : rk == 1 ? k_index_explain<LT, 1> \
-                                                                : k_index_explain<LT, 0>)
+    const int rk = record_kind(records, vals, n_tiles);
+#define GKM_IXEXP(LT) (rk == 3 ? k_index_explain<LT, 3> : rk == 2 ? k_index_explain<LT, 2> \
+                       : rk == 1 ? k_index_explain<LT, 1> : k_index_explain<LT, 0>)
     const ixexp_fn fn = l == 10 ? GKM_IXEXP(10) : l == 11 ? GKM_IXEXP(11)
                       : l == 12 ? GKM_IXEXP(12) : GKM_IXEXP(0);
 #undef GKM_IXEXP

@@ -332,18 +332,18 @@ class DeviceIndex:
         if self.n_entries >= 2**31:
             raise MemoryError("l-mer index over 2^31 entries")
         self.offsets = offsets.to(torch.int32)  # uint32 bits
-        # bucket records, 32 bytes per key: (start, count, the first ids as
-        # uint16 -- RECORD_IDS of them, or RECORD_IDS_W followed by their uint8
-        # weights), so the n_tiles records of an l-mer share a cache line and a
-        # probe is one load
+        # bucket records, 32 bytes per key (the l-mer index query): start,
+        # count, the bucket's first RECORD_IDS ids (or RECORD_IDS_W ids and their
+        # uint8 weights) -- bounds and ids in one load, the n_tiles records of
+        # an l-mer in one cache line
         self.records = None
         nk = nl * self.n_tiles
+        ks = key[order]
         if (self.pack16 or self.vals is not None) and nk * 32 <= RECORD_MAX_BYTES:
             rec = torch.zeros((nk, 16), dtype=torch.int16, device=dev)
             r32 = rec.view(torch.int32)
             r32[:, 0] = self.offsets[:-1]
             r32[:, 1] = (offsets[1:] - offsets[:-1]).to(torch.int32)
-            ks = key[order]
             rank = torch.arange(self.n_entries, device=dev) - offsets[ks]
             sel = rank < (RECORD_IDS if self.vals is None else RECORD_IDS_W)
             rec.view(-1)[ks[sel] * 16 + 4 + rank[sel]] = self.ids[sel]
@@ -351,7 +351,25 @@ class DeviceIndex:
                 rb = rec.view(torch.uint8)
                 rb.view(-1)[ks[sel] * 32 + 24 + rank[sel]] = self.vals[sel].to(torch.uint8)
             self.records = r32
-            del rec, ks, rank, sel
+            del rec, rank, sel
+        # l-mer records, 64 bytes per l-mer (GkmExplain through the index, which
+        # sums over every tile): unweighted with at most LREC_MAX_TILES tiles,
+        # whose buckets of one l-mer are consecutive -- the run's start, the
+        # tiles' counts as bytes (255: saturated), the run's first LREC_IDS ids
+        self.lmer_records = None
+        if self.pack16 and self.n_tiles <= LREC_MAX_TILES and nl * 64 <= RECORD_MAX_BYTES:
+            rec = torch.zeros((nl, 32), dtype=torch.int16, device=dev)
+            r32 = rec.view(torch.int32)
+            r32[:, 0] = self.offsets[:-1:self.n_tiles][:nl]
+            cnt = (offsets[1:] - offsets[:-1]).clamp(max=255).to(torch.uint8).view(nl, self.n_tiles)
+            rec.view(torch.uint8)[:, 4:4 + self.n_tiles] = cnt
+            lmer = ks // self.n_tiles
+            rank = torch.arange(self.n_entries, device=dev) - offsets[lmer * self.n_tiles]
+            sel = rank < LREC_IDS
+            rec.view(-1)[lmer[sel] * 32 + 4 + rank[sel]] = self.ids[sel]
+            self.lmer_records = r32
+            del rec, lmer, rank, sel, cnt
+        del ks
         masks, levels = neighbor_masks(l, kp.h, kp.dmax)
         self.masks = torch.from_numpy(masks.view(np.int32)).to(dev)
         self.levels = torch.from_numpy(levels).to(dev)
@@ -365,8 +383,7 @@ class DeviceIndex:
         t = [self.ids, self.offsets, self.masks, self.levels]
         if self.vals is not None:
             t.append(self.vals)
-        if self.records is not None:
-            t.append(self.records)
+        t += [r for r in (self.records, self.lmer_records) if r is not None]
         return sum(x.numel() * x.element_size() for x in t)
 
 
@@ -380,6 +397,8 @@ def index_tile(nm: int, pack16: bool = False) -> int:
 
 
 MAX_PARTIAL_BYTES = 2 << 30
+LREC_IDS = 28                # ids held in a 64-byte l-mer record
+LREC_MAX_TILES = 4           # tiles an l-mer record covers
 RECORD_IDS = 12              # ids held in a bucket record
 RECORD_IDS_W = 8             # (id, weight) pairs held in a record of a weighted index
 RECORD_MAX_BYTES = 4 << 30   # larger record tables (l >= 13 with many tiles) are not built

@@ -66,7 +66,7 @@ SWEEP_COMPARISONS_PER_S = 5e12
 # gkm_index_explain rates (MI355X, l = 11): (window, mask) probes and index
 # entries per second; with the sweep's rate, the cost model that picks the
 # per-SV explain kernel
-INDEX_EXPLAIN_PROBES_PER_S = 3.5e10   # (window, mask, SV tile) bucket records read
+INDEX_EXPLAIN_PROBES_PER_S = 3.5e10   # (window, mask, SV tile) records read (l-mer records: per probe)
 INDEX_EXPLAIN_ENTRIES_PER_S = 2.0e11
 
 # Per-SV explain kernel: "auto" (the l-mer index when the cost model favours
@@ -167,7 +167,7 @@ def _explain_rows(model, codes, lens, resolved, out):
                  lambda: probes * (E.INDEX_PROBE_BYTES + idx.per_lmer * E.INDEX_HIT_BYTES),
                  None, lambda: _lib.call(
                      "gkm_index_explain", int(pert), E._p(fwd), E._p(xs.len), B, Wx, xs.lmax,
-                     E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(idx.records),
+                     E._p(idx.offsets), E._p(idx.ids), E._p(idx.vals), E._p(_explain_records(idx)),
                      E._p(masks),
                      E._p(levels), len(masks), kp.l, E._p(wq), sv.S, idx.tile, idx.n_tiles,
                      chunk, E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
@@ -192,6 +192,17 @@ def _explain_masks(sv, ta, tb, cmax, pert):
     return sv.tables[key]
 
 
+def _explain_records(idx):
+    """The records gkm_index_explain reads for this index: the l-mer records
+    (unweighted, at most LREC_MAX_TILES tiles), else the per-key records of a
+    weighted index or of more tiles, else none (the bounds)."""
+    if idx.lmer_records is not None:
+        return idx.lmer_records
+    if idx.vals is not None or idx.n_tiles > E.LREC_MAX_TILES:
+        return idx.records
+    return None
+
+
 def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
     """(index, sweep) estimated GPU seconds of explaining rows of these
     lengths through the per-SV kernels (index None when it does not apply)."""
@@ -207,6 +218,8 @@ def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
     per_lmer = norient * float(np.clip(sv.seqs.lengths - kp.l + 1, 0, None).sum()) / 4**kp.l
     probes = wx * n_masks
     n_tiles = -(-sv.S // E.index_tile(kp.nm, not kp.weighted))
+    if not kp.weighted and n_tiles <= E.LREC_MAX_TILES:
+        n_tiles = 1  # one l-mer record per probe covers every tile
     index_s = (probes * n_tiles / INDEX_EXPLAIN_PROBES_PER_S
                + probes * per_lmer / INDEX_EXPLAIN_ENTRIES_PER_S)
     return index_s, pair_s

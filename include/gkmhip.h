@@ -145,8 +145,11 @@ int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int
  * gkm_index_hist CSR) give s = sum_e ec[b][sv_e] * vals_e, which adds
  * s * ta[m] at matching positions (to base u_k; perturbation: to the three
  * other bases) and s * tb[m] at mismatching ones (to base v_k), times the
- * query window's weight (wtab, or 1).  records: the bucket records of an
- * unweighted index (see gkm_index_hist), or NULL.  chunk_windows: windows per
+ * query window's weight (wtab, or 1).  records: NULL, the per-key records of
+ * gkm_index_hist or -- unweighted with n_tiles <= 4, where an l-mer's buckets
+ * are consecutive across tiles -- [4^l][16] uint32 l-mer records: the run's
+ * start, the tiles' counts as bytes (255 = saturated: read the offsets), the
+ * run's first 28 ids (uint16).  chunk_windows: windows per
  * block.
  * l <= 13, B <= 65535.  _explain_index_cpu, _distribute
  * (explain/cpu.py:68-86, 146-207); explain_lmer (explain/cuda.py:174-351). */
@@ -165,8 +168,9 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * index is a CSR over keys lmer * n_tiles + tile (offsets [4^l n_tiles + 1]
  * uint32; ids uint16 SV within its tile of `tile` SVs; vals uint8 window
  * weights or NULL) of every SV window's l-mer, forward and RC; records: NULL
- * or, for an unweighted index, [4^l n_tiles][8] uint32 bucket records (start,
- * count, then the bucket's first 12 ids as uint16), one 32-byte load per
+ * or [4^l n_tiles][8] uint32 bucket records, one per key: start, count, the
+ * bucket's first 12 ids (uint16), or with vals its first 8 ids and their uint8
+ * weights -- one 32-byte load per
  * probe; masks / levels
  * [n_masks] the XOR neighbourhood of mismatch counts < nm with non-zero
  * weight.  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
