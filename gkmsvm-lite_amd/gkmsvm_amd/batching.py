@@ -99,7 +99,7 @@ class BatchScorer:
             return [_Plan(False, table=table)]
         cands = []
         if self.kp.l <= E.INDEX_MAX_L and self.kp.dmax >= 0:
-            cands += [_Plan(True, target_blocks=t) for t in (1024, 2048, 4096)]
+            cands += [_Plan(True, target_blocks=t) for t in (1024, 2048, 16384)]
         norient = 2 if self.kp.rc else 1
         cands += [_Plan(False, shape=E.launch_shape(self.rows, self.sv.S, norient))]
         cands += [_Plan(False, shape=s) for s in ((1, 1), (2, 1), (1, 2), (4, 1))]
