@@ -208,8 +208,8 @@ HBM_KINDS = {
 # known size) in rocprofv3 --pmc runs of this benchmark's serve command, as
 # recorded in profiles/serving_72kSV_mix_trace.txt.
 PROFILED_TRAFFIC = {
-    "k_index_tile": (5_299_166 * 2 + 231_620) * 1024.0,
-    "k_index_explain": (3_998_263 * 2 + 364) * 1024.0,
+    "k_index_tile": (4_753_098 * 2 + 523_212) * 1024.0,
+    "k_index_explain": (2_467_422 * 2 + 397) * 1024.0,
 }
 TRAFFIC_SOURCE = "profiles/serving_72kSV_mix_trace.txt (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
 

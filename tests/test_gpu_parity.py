@@ -229,6 +229,43 @@ def test_explain_kernels_agree_with_oracle(gpu, seed):
             H.assert_close(got, want, 1e-10, f"{kernel} {mode} l={l} t={kt}")
 
 
+@pytest.mark.parametrize("kt", [2, 3, 5])
+def test_index_explain_with_repeats(gpu, kt):
+    """GkmExplain through the index on SVs with poly-A and dinucleotide runs
+    (l-mer buckets of thousands of entries: saturated tile counts in the
+    l-mer records, long runs past the record; weighted: the per-key records)
+    equals the oracle's per-SV loop, hypothetical and perturbation."""
+    from gkmsvm_amd import explain as X
+    from gkmsvm_amd import gkmexplain
+
+    rng = np.random.default_rng(41 + kt)
+    L = 300
+    yc = rng.integers(0, 4, (150, L)).astype(np.uint8)
+    yc[:40] = 0
+    yc[40:48] = np.tile([0, 2], L // 2)
+    p = {"L": 11, "k": 7, "d": 3}
+    if kt == 5:
+        p.update(M=50, H=50.0, gamma=1.5)
+    if kt == 3:
+        p["gamma"] = 1.5
+    m = _model(kt, p, yc, np.full(len(yc), L), rng.normal(size=len(yc)), 0.1)
+    idx = m._svs().index()
+    assert int((idx.offsets[1:] - idx.offsets[:-1]).max()) > 255
+    xc = rng.integers(0, 4, (3, 400)).astype(np.uint8)
+    xc[0, 50:200] = 0
+    xc[1, :] = np.tile([0, 2], 200)
+    xl = np.array([400, 400, 350])
+    om = _oracle_model(m)
+    for mode in ("hypothetical", "perturbation"):
+        want = O.explain(om, xc, xl, mode)
+        X.EXPLAIN_KERNEL = "index"
+        try:
+            got = gkmexplain(m, xc, mode=mode, lengths=xl).cpu().numpy()
+        finally:
+            X.EXPLAIN_KERNEL = "auto"
+        H.assert_close(got, want, 1e-10, f"index {mode} t={kt}")
+
+
 @pytest.mark.parametrize("name", sorted(k for k in H.EXPLAIN_SETS if not k.endswith("_51")))
 def test_mutation_impact_fixtures(gpu, name):
     """score_variants(method='gkmexplain') equals the oracle and gkmexplain -m 5."""
