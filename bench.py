@@ -45,7 +45,6 @@ import argparse
 import json
 import os
 import sys
-import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -120,27 +119,21 @@ def run_plain(model, reqs):
 
 
 def run_served(server, reqs, rate, rng):
-    """Submit reqs at Poisson arrival times (rate per second) from a client
-    thread; returns the wall time from the first arrival to the last result."""
+    """Submit reqs at Poisson arrival times (rate per second) from this (the
+    client's) thread while the server's worker runs; returns the wall time
+    from the first arrival to the last result."""
     gaps = rng.exponential(1.0 / rate, len(reqs))
     futs = []
-
-    def client():
-        t0 = time.perf_counter()
-        due = t0
-        for (kind, payload), g in zip(reqs, gaps):
-            due += g
-            while True:
-                now = time.perf_counter()
-                if now >= due:
-                    break
-                time.sleep(min(due - now, 5e-4))
-            futs.append(server.submit(kind, payload))
-
     t0 = time.perf_counter()
-    th = threading.Thread(target=client)
-    th.start()
-    th.join()
+    due = t0
+    for (kind, payload), g in zip(reqs, gaps):
+        due += g
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            time.sleep(min(due - now, 5e-4))
+        futs.append(server.submit(kind, payload))
     for f in futs:
         f.result()
     return time.perf_counter() - t0

@@ -351,8 +351,6 @@ __global__ void k_combine_gram(const unsigned long long *__restrict__ hist, int 
     out[i] = k;
 }
 
-constexpr int RED_THREADS = 256;
-
 // Block-wide sum in a fixed order (deterministic).
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
@@ -372,65 +370,97 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 
 // Decision values: out[b] = sum_s coef_s F(K[b,s] / max(sqrt(dx ds), 1e-10)) + bias
 // (svm.py:43-61, _normalized_dot), one block per row.
-__global__ __launch_bounds__(RED_THREADS) void k_combine_score(
+// One block of SCORE_THREADS per row; each thread keeps SCORE_UNROLL
+// independent sums (their loads in flight together) added in a fixed order,
+// then the fixed-order block sum: a row's score depends only on that row.
+constexpr int SCORE_THREADS = 1024;
+constexpr int SCORE_UNROLL = 4;
+
+__global__ __launch_bounds__(SCORE_THREADS) void k_combine_score(
     const unsigned long long *__restrict__ hist, int S, int nm, const double *__restrict__ tab,
     const double *__restrict__ dx, const double *__restrict__ ds,
     const double *__restrict__ coef, int normalize, int rbf, double gamma, double bias,
     double *__restrict__ out)
 {
-    __shared__ double red[RED_THREADS / WAVE];
+    __shared__ double red[SCORE_THREADS / WAVE];
     __shared__ double stab[16];
     if (threadIdx.x < 16)
         stab[threadIdx.x] = (int)threadIdx.x < nm ? tab[threadIdx.x] : 0.0;
     __syncthreads();
     const int b = blockIdx.x;
-    double acc = 0.0;
+    double acc[SCORE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SCORE_UNROLL; ++u)
+        acc[u] = 0.0;
     const double dxb = dx ? dx[b] : 1.0;
-    for (int s = threadIdx.x; s < S; s += RED_THREADS) {
-        double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
-        if (normalize)
-            k = k / fmax(sqrt(dxb * ds[s]), 1e-10);
-        if (rbf)
-            k = exp(gamma * (k - 1.0));
-        acc += coef[s] * k;
+    for (int s0 = threadIdx.x; s0 < S; s0 += SCORE_THREADS * SCORE_UNROLL) {
+#pragma unroll
+        for (int u = 0; u < SCORE_UNROLL; ++u) {
+            const int s = s0 + u * SCORE_THREADS;
+            if (s >= S)
+                continue;
+            double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
+            if (normalize)
+                k = k / fmax(sqrt(dxb * ds[s]), 1e-10);
+            if (rbf)
+                k = exp(gamma * (k - 1.0));
+            acc[u] += coef[s] * k;
+        }
     }
-    double t = block_sum(acc, red);
+    double v = acc[0];
+#pragma unroll
+    for (int u = 1; u < SCORE_UNROLL; ++u)
+        v += acc[u];
+    double t = block_sum(v, red);
     if (threadIdx.x == 0)
         out[b] = t + bias;
 }
 
 // Mutation impact (explain/impact.py:22-48): rows [ref; alt] of hist (2n rows),
 // impact[b] = sum_s coef_s dk (RBF: dk (exp(g(k-1)) - exp(-g)) / k for k > 0).
-__global__ __launch_bounds__(RED_THREADS) void k_combine_impact(
+__global__ __launch_bounds__(SCORE_THREADS) void k_combine_impact(
     const unsigned long long *__restrict__ hist, int n, int S, int nm,
     const double *__restrict__ tab, const double *__restrict__ dref,
     const double *__restrict__ ds, const double *__restrict__ coef, int normalize, int rbf,
     double gamma, double *__restrict__ out)
 {
-    __shared__ double red[RED_THREADS / WAVE];
+    __shared__ double red[SCORE_THREADS / WAVE];
     __shared__ double stab[16];
     if (threadIdx.x < 16)
         stab[threadIdx.x] = (int)threadIdx.x < nm ? tab[threadIdx.x] : 0.0;
     __syncthreads();
     const int b = blockIdx.x;
     const double floor_ = rbf ? exp(-gamma) : 0.0;
-    double acc = 0.0;
-    for (int s = threadIdx.x; s < S; s += RED_THREADS) {
-        double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
-        double dk = kraw_of(hist + ((int64_t)(n + b) * S + s) * nm, stab, nm) - k;
-        if (normalize) {
-            double z = fmax(sqrt(dref[b] * ds[s]), 1e-10);
-            k /= z;
-            dk /= z;
-        }
-        if (rbf) {
-            if (k <= 0.0)
+    double acc[SCORE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SCORE_UNROLL; ++u)
+        acc[u] = 0.0;
+    for (int s0 = threadIdx.x; s0 < S; s0 += SCORE_THREADS * SCORE_UNROLL) {
+#pragma unroll
+        for (int u = 0; u < SCORE_UNROLL; ++u) {
+            const int s = s0 + u * SCORE_THREADS;
+            if (s >= S)
                 continue;
-            dk *= (exp(gamma * (k - 1.0)) - floor_) / k;
+            double k = kraw_of(hist + ((int64_t)b * S + s) * nm, stab, nm);
+            double dk = kraw_of(hist + ((int64_t)(n + b) * S + s) * nm, stab, nm) - k;
+            if (normalize) {
+                double z = fmax(sqrt(dref[b] * ds[s]), 1e-10);
+                k /= z;
+                dk /= z;
+            }
+            if (rbf) {
+                if (k <= 0.0)
+                    continue;
+                dk *= (exp(gamma * (k - 1.0)) - floor_) / k;
+            }
+            acc[u] += coef[s] * dk;
         }
-        acc += coef[s] * dk;
     }
-    double t = block_sum(acc, red);
+    double v = acc[0];
+#pragma unroll
+    for (int u = 1; u < SCORE_UNROLL; ++u)
+        v += acc[u];
+    double t = block_sum(v, red);
     if (threadIdx.x == 0)
         out[b] = t;
 }
@@ -1709,7 +1739,7 @@ GKM_API int gkm_combine_score(const void *hist, int B, int S, int nm, const doub
         return fail(-1, "gkm_combine_score: nm > 16");
     if (B <= 0)
         return 0;
-    hipLaunchKernelGGL(k_combine_score, dim3((unsigned)B), dim3(RED_THREADS), 0, S_(stream),
+    hipLaunchKernelGGL(k_combine_score, dim3((unsigned)B), dim3(SCORE_THREADS), 0, S_(stream),
                        (const unsigned long long *)hist, S, nm, tab, dx, ds, coef, normalize, rbf,
                        gamma, bias, out);
     GKM_CHECK_LAUNCH("gkm_combine_score");
@@ -1724,7 +1754,7 @@ GKM_API int gkm_combine_impact(const void *hist, int n, int S, int nm, const dou
         return fail(-1, "gkm_combine_impact: nm > 16");
     if (n <= 0)
         return 0;
-    hipLaunchKernelGGL(k_combine_impact, dim3((unsigned)n), dim3(RED_THREADS), 0, S_(stream),
+    hipLaunchKernelGGL(k_combine_impact, dim3((unsigned)n), dim3(SCORE_THREADS), 0, S_(stream),
                        (const unsigned long long *)hist, n, S, nm, tab, dref, ds, coef, normalize,
                        rbf, gamma, out);
     GKM_CHECK_LAUNCH("gkm_combine_impact");

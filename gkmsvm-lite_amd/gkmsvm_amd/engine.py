@@ -456,9 +456,11 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     wt, _ = kp.wtab(xs.lmax)
     # each row's windows in l-mer order (PAD last): a block's windows then share
     # high digits, and so do their neighbours' offsets -- cache lines reused
-    keys = fwd[x0:x1].to(torch.int64) & 0xFFFFFFFF
+    # (32-bit keys: PAD, 0xFFFFFFFF, sorts last as INT32_MAX, still >= 4^l)
+    keys = fwd[x0:x1]
+    keys = torch.where(keys < 0, torch.full_like(keys, 2**31 - 1), keys)
     keys, pos = torch.sort(keys, dim=1)
-    swin = keys.to(torch.int32).contiguous()
+    swin = keys.contiguous()
     spos = pos.to(torch.int32).contiguous() if wt is not None else None
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
            lambda: _lib.call(

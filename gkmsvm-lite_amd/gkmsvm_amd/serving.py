@@ -27,7 +27,7 @@ import threading
 import time
 from collections import deque
 from concurrent.futures import Future
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -38,23 +38,33 @@ from .codec import encode_batch
 from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
 
 
-@dataclass
 class Request:
-    kind: str                 # "score" | "variant" | "explain"
-    seqs: list                # score/explain: [str]; variant: [(ref, alt)]
-    mode: str = "hypothetical"
-    future: Future = field(default_factory=Future)
-    t_submit: float = 0.0
-    t_admit: float = 0.0
-    t_done: float = 0.0
+    """One client request: kind "score" | "variant" | "explain"; seqs [str] or,
+    for variants, [(ref, alt)]; its Future, timestamps, and the rows, longest
+    sequence and HBM bytes it takes (computed once)."""
+
+    __slots__ = ("kind", "seqs", "mode", "future", "t_submit", "t_admit", "t_done",
+                 "n_rows", "lmax", "nbytes")
+
+    def __init__(self, kind: str, seqs: list, mode: str = "hypothetical"):
+        self.kind = kind
+        self.seqs = seqs
+        self.mode = mode
+        self.future = Future()
+        self.t_submit = self.t_admit = self.t_done = 0.0
+        if kind == "variant":
+            self.n_rows = 2 * len(seqs)
+            self.lmax = max((max(len(r), len(a)) for r, a in seqs), default=0)
+        else:
+            self.n_rows = len(seqs)
+            self.lmax = max((len(q) for q in seqs), default=0)
+        self.nbytes = 0
 
     def rows(self) -> int:
-        return 2 * len(self.seqs) if self.kind == "variant" else len(self.seqs)
+        return self.n_rows
 
     def max_len(self) -> int:
-        if self.kind == "variant":
-            return max(max(len(r), len(a)) for r, a in self.seqs)
-        return max(len(s) for s in self.seqs)
+        return self.lmax
 
 
 @dataclass
@@ -120,7 +130,7 @@ class GkmServer:
             raise ValueError(f"unknown request kind {kind!r}")
         req = Request(kind, list(seqs), mode)
         req.t_submit = time.perf_counter()
-        need = self._bytes(req)
+        need = req.nbytes = self._bytes(req)
         if need > self.hbm_budget:
             req.future.set_exception(MemoryError(
                 f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))
@@ -168,7 +178,7 @@ class GkmServer:
         batch, used, rows, n_explain = [], 0, 0, 0
         while self._pending:
             req = self._pending[0]
-            need = self._bytes(req)
+            need = req.nbytes
             is_ex = req.kind == "explain"
             if batch and (used + need > self.hbm_budget
                           or (not is_ex and rows + req.rows() > self.max_rows)
@@ -215,18 +225,22 @@ class GkmServer:
                 still.append(st)
                 continue
             now = time.perf_counter()
-            scores = st.score_host.numpy() if st.score_host is not None else None
+            # one copy of the step's scores (the pinned buffer is reused);
+            # each request gets its own slice of it
+            scores = st.score_host.numpy().copy() if st.score_host is not None else None
             for req, r0, n in st.spans:
                 vals = scores[r0:r0 + n]
                 if req.kind == "variant":
                     half = n // 2
                     res = vals[half:] - vals[:half]  # alt - ref
                 else:
-                    res = vals.copy()
+                    res = vals
                 req.t_done = now
                 req.future.set_result(res)
                 self.completed.append(req)
             for req, host, r0, n, L in st.explain_out:
+                if req.future.done():  # rejected at launch
+                    continue
                 req.t_done = now
                 req.future.set_result(host[r0:r0 + n, :, :L].numpy().copy())
                 self.completed.append(req)
@@ -274,10 +288,29 @@ class GkmServer:
             for mode, reqs in by_mode.items():
                 # every explain request of this mode in one call (rows of any
                 # length; each row's attributions depend only on that row)
-                from .explain import gkmexplain
+                from .explain import _MODE_ALIASES, _explain_rows_table, gkmexplain
 
                 seqs = [q for req in reqs for q in req.seqs]
-                res = gkmexplain(model, seqs, mode=mode, batch_size=len(seqs))
+                resolved = _MODE_ALIASES.get(mode)
+                if self.explain_tables and resolved in ("hypothetical", "perturbation"):
+                    # straight to the explain table: the step's rows encoded in
+                    # one pass, as gkmexplain does through the same table
+                    codes, lens, bad = encode_batch(seqs, kp.l)
+                    r0 = 0
+                    for req in reqs:
+                        if bad[r0:r0 + len(req.seqs)].any():
+                            req.future.set_exception(ValueError(
+                                "Invalid sequence (empty, or a letter other than ACGTN)"))
+                        r0 += len(req.seqs)
+                    pert = resolved == "perturbation"
+                    res = torch.empty((len(seqs), 4, codes.shape[1]), dtype=torch.float64,
+                                      device=kp.device)
+                    _explain_rows_table(model, codes, lens, sv.tables[("explain", pert)], pert,
+                                        res)
+                    short = torch.from_numpy(lens < kp.l).to(kp.device, non_blocking=True)
+                    res = torch.where(short[:, None, None], torch.full_like(res, float("nan")), res)
+                else:
+                    res = gkmexplain(model, seqs, mode=mode, batch_size=len(seqs))
                 host = torch.empty(res.shape, dtype=torch.float64, pin_memory=True)
                 host.copy_(res, non_blocking=True)
                 keep.append(res)
