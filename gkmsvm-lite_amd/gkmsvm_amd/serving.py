@@ -2,15 +2,16 @@
 
 The model's SV tables, coefficients and self-kernels stay in HBM.  Clients
 submit scoring, variant (score_variants, kernel method) and GkmExplain
-requests from any thread and get a Future.  One worker thread admits pending
+requests from any thread and get a Future.  A launcher thread admits pending
 requests into GPU steps while earlier steps run: each step packs every
-admitted scoring/variant row into one batch (rows of any length: each row's
-diagonal walk uses its own length) and launches the cross kernel once for all
-of them on its own HIP stream; the step's explain requests follow as one
-batch per mode.  Up
-to ``streams`` steps are in flight, so a request that arrives while others
-are on the GPU is admitted as soon as a stream frees up and its work overlaps
-theirs.  Admission is by memory: a step takes requests while the bytes it
+admitted scoring/variant row into one batch (rows of any length, encoded in
+one pass and sent in one pinned copy) and launches the index or pairwise
+kernel (or the l-mer table lookup) once for all of them on its own HIP
+stream; the step's explain requests follow as one batch per mode.  A
+harvester thread waits on each step's completion event and hands out its
+results.  Up to ``streams`` steps are in flight, so a request that arrives
+while others are on the GPU is admitted as soon as a stream frees up and its
+work overlaps theirs.  Admission is by memory: a step takes requests while the bytes it
 would hold (histograms, planes, outputs) stay within ``hbm_budget``; a
 request larger than the whole budget is rejected with MemoryError.
 
@@ -110,7 +111,8 @@ class GkmServer:
         self._inflight: list = []
         self._lock = threading.Condition()
         self._stop = False
-        self._thread = None
+        self._thread = self._harvester = None
+        self._launcher_done = False
         self.completed: list = []
 
     def _build_explain(self, pert: bool):
@@ -137,23 +139,29 @@ class GkmServer:
             return req.future
         with self._lock:
             self._pending.append(req)
-            self._lock.notify()
+            self._lock.notify_all()
         return req.future
 
     def start(self):
         if self._thread is None:
             self._stop = False
+            self._launcher_done = False
             self._thread = threading.Thread(target=self._loop, name="gkm-server", daemon=True)
+            self._harvester = threading.Thread(target=self._harvest_loop, name="gkm-results",
+                                               daemon=True)
             self._thread.start()
+            self._harvester.start()
         return self
 
     def stop(self):
+        """Serve what is pending, then stop both threads."""
         with self._lock:
             self._stop = True
-            self._lock.notify()
+            self._lock.notify_all()
         if self._thread is not None:
             self._thread.join()
-            self._thread = None
+            self._harvester.join()
+            self._thread = self._harvester = None
 
     def __enter__(self):
         return self.start()
@@ -196,56 +204,73 @@ class GkmServer:
     # -- the worker ---------------------------------------------------------------------
 
     def _loop(self):
+        """The launcher: whenever requests are pending and a stream is free, it
+        admits the next step and launches it (results are the harvester's)."""
         torch.cuda.set_device(self.kp.device)
         while True:
-            self._harvest()
             with self._lock:
-                if self._stop and not self._pending and not self._inflight:
-                    return
-                batch = self._admit() if (self._pending and self._free) else []
-                if not batch:
-                    if self._inflight:
-                        self._lock.wait(timeout=5e-5)
-                    else:
-                        self._lock.wait(timeout=1e-3)
-                    continue
-            stream = self._free.pop()
+                while not (self._pending and self._free):
+                    if self._stop and not self._pending:
+                        self._launcher_done = True
+                        self._lock.notify_all()
+                        return
+                    self._lock.wait()
+                batch = self._admit()
+                stream = self._free.pop()
             try:
-                self._inflight.append(self._launch(batch, stream))
+                st = self._launch(batch, stream)
             except Exception as exc:  # surface launch errors on the requests
-                self._free.append(stream)
+                with self._lock:
+                    self._free.append(stream)
+                    self._lock.notify_all()
                 for r in batch:
                     if not r.future.done():
                         r.future.set_exception(exc)
-
-    def _harvest(self):
-        still = []
-        for st in self._inflight:
-            if not st.done.query():
-                still.append(st)
                 continue
-            now = time.perf_counter()
-            # one copy of the step's scores (the pinned buffer is reused);
-            # each request gets its own slice of it
-            scores = st.score_host.numpy().copy() if st.score_host is not None else None
-            for req, r0, n in st.spans:
-                vals = scores[r0:r0 + n]
-                if req.kind == "variant":
-                    half = n // 2
-                    res = vals[half:] - vals[:half]  # alt - ref
-                else:
-                    res = vals
-                req.t_done = now
-                req.future.set_result(res)
-                self.completed.append(req)
-            for req, host, r0, n, L in st.explain_out:
-                if req.future.done():  # rejected at launch
-                    continue
-                req.t_done = now
-                req.future.set_result(host[r0:r0 + n, :, :L].numpy().copy())
-                self.completed.append(req)
-            self._free.append(st.stream)
-        self._inflight = still
+            with self._lock:
+                self._inflight.append(st)
+                self._lock.notify_all()
+
+    def _harvest_loop(self):
+        """The harvester: waits on each step's completion event (oldest first,
+        blocking in the runtime rather than polling), hands out its results and
+        frees its stream."""
+        torch.cuda.set_device(self.kp.device)
+        while True:
+            with self._lock:
+                while not self._inflight:
+                    if self._launcher_done:
+                        return
+                    self._lock.wait()
+                st = self._inflight[0]
+            st.done.synchronize()
+            self._deliver(st)
+            with self._lock:
+                self._inflight.pop(0)
+                self._free.append(st.stream)
+                self._lock.notify_all()
+
+    def _deliver(self, st):
+        now = time.perf_counter()
+        # one copy of the step's scores (the pinned buffer is reused); each
+        # request gets its own slice of it
+        scores = st.score_host.numpy().copy() if st.score_host is not None else None
+        for req, r0, n in st.spans:
+            vals = scores[r0:r0 + n]
+            if req.kind == "variant":
+                half = n // 2
+                res = vals[half:] - vals[:half]  # alt - ref
+            else:
+                res = vals
+            req.t_done = now
+            req.future.set_result(res)
+            self.completed.append(req)
+        for req, host, r0, n, L in st.explain_out:
+            if req.future.done():  # rejected at launch
+                continue
+            req.t_done = now
+            req.future.set_result(host[r0:r0 + n, :, :L].numpy().copy())
+            self.completed.append(req)
 
     def _launch(self, batch, stream) -> _Step:
         kp, sv, model = self.kp, self.sv, self.model

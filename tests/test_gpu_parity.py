@@ -441,3 +441,20 @@ def test_index_plain_config(gpu):
     got = m(q).cpu().numpy().ravel()
     om = O.Model(O.Kern.make(2, p), svc, np.full(500, 300), coef, 0.0)
     H.assert_close(got, O.LmerIndex(om).score(q, np.full(200, 300)), 1e-10, "index C1")
+
+
+def test_peak_microbenchmarks_run(gpu):
+    """The measurement-only peak kernels run and leave their sink untouched
+    (their sentinels never come up on these inputs)."""
+    from gkmsvm_amd import _lib
+    from gkmsvm_amd import engine as E
+
+    src = torch.ones(1 << 20, dtype=torch.int64, device=gpu)
+    sink = torch.zeros(4, dtype=torch.int32, device=gpu)
+    _lib.call("gkm_peak_read", src.data_ptr(), src.numel() * 8, sink.data_ptr(), 64,
+              E.stream_handle())
+    _lib.call("gkm_peak_valu", 16, sink.data_ptr(), 64, E.stream_handle())
+    torch.cuda.synchronize()
+    assert int(sink.abs().sum()) == 0
+    with pytest.raises(_lib.GkmError):
+        _lib.call("gkm_peak_read", src.data_ptr(), 15, sink.data_ptr(), 64, E.stream_handle())
