@@ -197,12 +197,12 @@ HBM_KINDS = {
 
 
 # HBM traffic per launch (bytes) of the serving run's per-SV kernels: mean per
-# dispatch of FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from device copies of
-# known size) in rocprofv3 --pmc runs of this benchmark's serve command, as
-# recorded in profiles/serving_72kSV_mix_trace.txt.
+# timed dispatch of FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from device
+# copies of known size) in rocprofv3 --pmc runs of this benchmark's headline
+# serve run, as recorded in profiles/serving_72kSV_mix_trace.txt.
 PROFILED_TRAFFIC = {
-    "k_index_tile": (4_753_098 * 2 + 523_212) * 1024.0,
-    "k_index_explain": (2_467_422 * 2 + 397) * 1024.0,
+    "k_index_tile": (5_392_703 * 2 + 684_738) * 1024.0,
+    "k_index_explain": (4_460_388 * 2 + 701) * 1024.0,
 }
 TRAFFIC_SOURCE = "profiles/serving_72kSV_mix_trace.txt (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
 
@@ -458,6 +458,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tables", action="store_true",
+                    help="skip the l-mer table runs (profiles of the per-SV kernels)")
     ap.add_argument("--mode", default="serve",
                     choices=["serve", "cross", "index", "table", "copy", "peaks", "configs"],
                     help="serve: the benchmark; cross/index/table/copy: one kernel family on "
@@ -529,28 +531,30 @@ def main():
     lat = latency_stats(server_k.completed)
     server_k.stop()
 
-    # 2. The same stream through the exact l-mer tables, built when the server
-    # starts (non-RBF model): no query x SV pair is evaluated; evaluations
-    # are counted as the per-SV path's for comparison.
-    t = time.perf_counter()
-    server = GkmServer(model, hbm_budget=16 << 30)
-    torch.cuda.synchronize()
-    build_s = time.perf_counter() - t
-    server.start()
-    ttimed = steps[args.warmup:args.warmup + args.table_steps]
-    t_evals = sum(evals(s_) for s_ in ttimed)
-    served_t_s, summary_t, lat_t_sat = served_run(server, ttimed)
-    cap_t = args.requests * len(ttimed) / served_t_s
-    server.completed.clear()
-    for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
-        run_served(server, s_, args.load * cap_t, rng)
-    lat_t = latency_stats(server.completed)
-    server.stop()
-    served_t = {"value": t_evals / served_t_s, "ms_per_step": served_t_s / len(ttimed) * 1e3,
-                "steps": len(ttimed), "table_build_s": build_s,
-                "latency_ms_at_saturation": lat_t_sat,
-                "latency_load_rate_per_s": args.load * cap_t, "latency_ms": lat_t,
-                "roofline": roofline_of(summary_t)}
+    served_t = None
+    if not args.no_tables:
+        # 2. The same stream through the exact l-mer tables, built when the server
+        # starts (non-RBF model): no query x SV pair is evaluated; evaluations
+        # are counted as the per-SV path's for comparison.
+        t = time.perf_counter()
+        server = GkmServer(model, hbm_budget=16 << 30)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t
+        server.start()
+        ttimed = steps[args.warmup:args.warmup + args.table_steps]
+        t_evals = sum(evals(s_) for s_ in ttimed)
+        served_t_s, summary_t, lat_t_sat = served_run(server, ttimed)
+        cap_t = args.requests * len(ttimed) / served_t_s
+        server.completed.clear()
+        for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
+            run_served(server, s_, args.load * cap_t, rng)
+        lat_t = latency_stats(server.completed)
+        server.stop()
+        served_t = {"value": t_evals / served_t_s, "ms_per_step": served_t_s / len(ttimed) * 1e3,
+                    "steps": len(ttimed), "table_build_s": build_s,
+                    "latency_ms_at_saturation": lat_t_sat,
+                    "latency_load_rate_per_s": args.load * cap_t, "latency_ms": lat_t,
+                    "roofline": roofline_of(summary_t)}
 
     plain = plain_t = None
     if not args.no_plain:
@@ -563,16 +567,17 @@ def main():
         plain_s = time.perf_counter() - t
         plain = {"value": k_evals / plain_s, "ms_per_step": plain_s / len(ktimed) * 1e3,
                  "steps": len(ktimed)}
-        # 4. Plain calls with the tables already built.
-        run_plain(model, steps[0][:8])
-        t = time.perf_counter()
-        for s_ in ttimed:
-            run_plain(model, s_)
-        plain_t_s = time.perf_counter() - t
-        plain_t = {"value": t_evals / plain_t_s, "ms_per_step": plain_t_s / len(ttimed) * 1e3,
-                   "steps": len(ttimed)}
-        served_t["served_over_plain_tables"] = served_t["value"] / plain_t["value"]
-        served_t["served_over_plain"] = served_t["value"] / plain["value"]
+        if served_t is not None:
+            # 4. Plain calls with the tables already built.
+            run_plain(model, steps[0][:8])
+            t = time.perf_counter()
+            for s_ in ttimed:
+                run_plain(model, s_)
+            plain_t_s = time.perf_counter() - t
+            plain_t = {"value": t_evals / plain_t_s, "ms_per_step": plain_t_s / len(ttimed) * 1e3,
+                       "steps": len(ttimed)}
+            served_t["served_over_plain_tables"] = served_t["value"] / plain_t["value"]
+            served_t["served_over_plain"] = served_t["value"] / plain["value"]
 
     roofline = roofline_of(summary)
     line = {

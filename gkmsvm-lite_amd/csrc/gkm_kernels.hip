@@ -890,8 +890,9 @@ struct IndexArgs {
     const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood (sorted)
     const uint8_t *levels;   // [n_masks] mismatch count of each mask
     const uint8_t *wtab;     // weight by distance, or NULL
-    void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK16: uint16)
+    void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16)
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
+    int sub_tile;            // PACK 4: SVs per pass of the 16-bit walk
     int64_t col_min, row_base;
 };
 
@@ -902,9 +903,15 @@ struct IndexArgs {
 // adds (its weight times the query window's) to N[level] of that pair -- the
 // pairs kernels/lmer_index_cpu.py:23-37 (_add_window) visits.  The tile's
 // counts go out with plain stores to the chunk's slice of `partial`.
-// PACK16: two 16-bit counts per 32-bit LDS word (unweighted; the host caps the
-// window chunk so no count of a block can reach 2^16), which doubles the SVs a
-// tile holds and halves the probes.
+// PACK: counts per 32-bit LDS word.  1: uint32 counts (weighted).  2: two
+// 16-bit counts (unweighted; the host caps the window chunk so no count of a
+// block can reach 2^16), which doubles the SVs a tile holds and halves the
+// probes.  4: four 8-bit counts -- twice the SVs again -- made exact by a
+// check: an add that carries out of a byte lowers the word's byte sum below
+// the number of adds (a carry turns 255 + 1 into 0 and adds at most 1 to the
+// next byte), so the block compares the byte sum of its words with its count
+// of adds, and where they differ (repeats) it walks its chunk again with 16-bit
+// counts, a part of the tile per pass.
 // Memory: the kernel is bound by random cache lines -- per neighbour l-mer the
 // bucket bounds and the bucket's ids.  RK: the bucket records (start, count,
 // the first ids and weights) put both in one 32-byte load, and the n_tiles
@@ -922,7 +929,7 @@ struct IndexArgs {
 // congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
 // chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk, and chunks past
 // the row's last window, exit at once.
-template <bool PACK16, int RK>
+template <int PACK, int RK>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -939,185 +946,259 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         return;  // no windows of this row: the reduce skips this chunk
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int wps = PACK16 ? (a.nm + 1) / 2 : a.nm;  // LDS words per SV
-    const int ncnt = min(a.tile, a.S - tile_id * a.tile) * wps;  // this tile's SVs
+    const int wps = (a.nm + PACK - 1) / PACK;  // LDS words per SV
+    const int s_base = tile_id * a.tile;
+    const int ns = min(a.tile, a.S - s_base);  // this tile's SVs
+    const int ncnt = ns * wps;
     for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS)
         cnt[i] = 0u;
     __syncthreads();
     const uint32_t nl = 1u << (2 * a.l);
     const int i1 = min(wx, i0 + a.chunk_windows);
-    const int s_base = tile_id * a.tile;
     const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
-    const int ns = min(a.tile, a.S - s_base);
-    // counts laid out [word][SV], so that a wave's atomics spread over the banks
-    auto count = [&](int sid, int lv, uint32_t add) {
-        if (sid < lo_col)
-            return;
-        if (PACK16)
-            atomicAdd(&cnt[(lv >> 1) * ns + sid], 1u << (16 * (lv & 1)));
-        else
-            atomicAdd(&cnt[lv * ns + sid], add);
-    };
-    // The wave's (window, mask group) pairs in order: probe p of lane t in
-    // group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
-    // v = w ^ mask[m], whose bucket is [e0, e1).
-    const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
-    int ni = i0 + wave, ng = 0;  // the pair being fetched
-    uint4 nr[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count) in nr[.][0]
-    int nlv[INDEX_PROBES];
-    auto fetch = [&]() {
-        const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
+    const int64_t off = (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
+    // One walk of the block's (window, mask group) pairs, every bucket entry
+    // handed to count(sid, level, weight).  Probe p of lane t in group g takes
+    // mask m = (INDEX_PROBES g + p) 64 + t, neighbour v = w ^ mask[m], whose
+    // bucket is [e0, e1).
+    auto walk = [&](auto count) {
+        const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
+        int ni = i0 + wave, ng = 0;  // the pair being fetched
+        uint4 nr[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count) in nr[.][0]
+        int nlv[INDEX_PROBES];
+        auto fetch = [&]() {
+            const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
 #pragma unroll
-        for (int p = 0; p < INDEX_PROBES; ++p) {
-            const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
-            nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
-            nlv[p] = 0;
-            if (w < nl && t < a.n_masks) {
-                const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
-                nlv[p] = a.levels[t];
-                if (RK) {
-                    const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
-                    nr[p][0] = r[0];
-                    nr[p][1] = r[1];
-                } else {
-                    const bounds2 r = *(const bounds2 *)(a.offsets + key);
-                    nr[p][0].x = r.x;
-                    nr[p][0].y = r.y - r.x;
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
+                nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
+                nlv[p] = 0;
+                if (w < nl && t < a.n_masks) {
+                    const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
+                    nlv[p] = a.levels[t];
+                    if (RK) {
+                        const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
+                        nr[p][0] = r[0];
+                        nr[p][1] = r[1];
+                    } else {
+                        const bounds2 r = *(const bounds2 *)(a.offsets + key);
+                        nr[p][0].x = r.x;
+                        nr[p][0].y = r.y - r.x;
+                    }
                 }
             }
-        }
-    };
-    fetch();
-    while (ni < i1) {
-        const int i = ni;
-        uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
-        int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
-        bool small = true;
-#pragma unroll
-        for (int p = 0; p < INDEX_PROBES; ++p) {
-            e0[p] = nr[p][0].x;
-            c[p] = (int)nr[p][0].y;
-            lv[p] = nlv[p];
-            n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
-            small = small && c[p] <= INDEX_WALK_MAX_C;
-            rid[p][0] = nr[p][0].z;
-            rid[p][1] = nr[p][0].w;
-            rid[p][2] = nr[p][1].x;
-            rid[p][3] = nr[p][1].y;
-            rid[p][4] = nr[p][1].z;
-            rid[p][5] = nr[p][1].w;
-        }
-        if (++ng == n_groups) {
-            ng = 0;
-            ni += INDEX_THREADS / WAVE;
-        }
+        };
         fetch();
-        const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
-        const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
-        if (__ballot(!small) == 0) {
-            // lane walk: every probe's next INDEX_WALK entries per round
-            int cmax = c[0];
+        while (ni < i1) {
+            const int i = ni;
+            uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
+            int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
+            bool small = true;
 #pragma unroll
-            for (int p = 1; p < INDEX_PROBES; ++p)
-                cmax = max(cmax, c[p]);
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                e0[p] = nr[p][0].x;
+                c[p] = (int)nr[p][0].y;
+                lv[p] = nlv[p];
+                n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
+                small = small && c[p] <= INDEX_WALK_MAX_C;
+                rid[p][0] = nr[p][0].z;
+                rid[p][1] = nr[p][0].w;
+                rid[p][2] = nr[p][1].x;
+                rid[p][3] = nr[p][1].y;
+                rid[p][4] = nr[p][1].z;
+                rid[p][5] = nr[p][1].w;
+            }
+            if (++ng == n_groups) {
+                ng = 0;
+                ni += INDEX_THREADS / WAVE;
+            }
+            fetch();
+            const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
+            const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
+            if (__ballot(!small) == 0) {
+                // lane walk: every probe's next INDEX_WALK entries per round
+                int cmax = c[0];
 #pragma unroll
-            for (int p = 0; p < INDEX_PROBES; ++p)  // the entries in the records
+                for (int p = 1; p < INDEX_PROBES; ++p)
+                    cmax = max(cmax, c[p]);
 #pragma unroll
-                for (int k = 0; k < record_ids<RK>(); ++k)
-                    if (k < c[p])
-                        count((int)rec_id(rid[p], k), lv[p],
-                              PACK16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
-            int kmin = n_in[0];
+                for (int p = 0; p < INDEX_PROBES; ++p)  // the entries in the records
 #pragma unroll
-            for (int p = 1; p < INDEX_PROBES; ++p)
-                kmin = min(kmin, n_in[p]);
-            for (int kb = kmin; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
-                int sid[INDEX_PROBES][INDEX_WALK];
-                uint32_t wv[INDEX_PROBES][INDEX_WALK];
+                    for (int k = 0; k < record_ids<RK>(); ++k)
+                        if (k < c[p])
+                            count((int)rec_id(rid[p], k), lv[p],
+                                  PACK > 1 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
+                int kmin = n_in[0];
 #pragma unroll
-                for (int p = 0; p < INDEX_PROBES; ++p)
+                for (int p = 1; p < INDEX_PROBES; ++p)
+                    kmin = min(kmin, n_in[p]);
+                for (int kb = kmin; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
+                    int sid[INDEX_PROBES][INDEX_WALK];
+                    uint32_t wv[INDEX_PROBES][INDEX_WALK];
 #pragma unroll
-                    for (int k = 0; k < INDEX_WALK; ++k) {
-                        const bool in = kb + k >= n_in[p] && kb + k < c[p];
-                        const uint32_t e = e0[p] + (uint32_t)(kb + k);
-                        sid[p][k] = in ? (int)a.ids[e] : 0;
-                        if (!PACK16)
-                            wv[p][k] = in && a.vals ? a.vals[e] : 1u;
+                    for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                        for (int k = 0; k < INDEX_WALK; ++k) {
+                            const bool in = kb + k >= n_in[p] && kb + k < c[p];
+                            const uint32_t e = e0[p] + (uint32_t)(kb + k);
+                            sid[p][k] = in ? (int)a.ids[e] : 0;
+                            if (PACK == 1)
+                                wv[p][k] = in && a.vals ? a.vals[e] : 1u;
+                        }
+#pragma unroll
+                    for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                        for (int k = 0; k < INDEX_WALK; ++k)
+                            if (kb + k >= n_in[p] && kb + k < c[p])
+                                count(sid[p][k], lv[p], PACK > 1 ? 1u : aw * wv[p][k]);
+                }
+                continue;
+            }
+#pragma unroll
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                // balanced walk of this probe's buckets: laid end to end (exclusive
+                // prefix sums) and walked 64 entries per pass: lane j of a pass takes
+                // entry base + j, in the bucket of the last lane r with excl[r] <= base + j
+                int incl = c[p];
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    const int u = __shfl_up(incl, o, WAVE);
+                    incl += lane >= o ? u : 0;
+                }
+                const int excl = incl - c[p];
+                const int total = __shfl(incl, WAVE - 1, WAVE);
+                // INDEX_PASSES passes at a time: their searches, gathers and id
+                // loads are independent, so their latencies overlap
+                for (int base = 0; base < total; base += INDEX_PASSES * WAVE) {
+                    int pr[INDEX_PASSES];
+#pragma unroll
+                    for (int u = 0; u < INDEX_PASSES; ++u)
+                        pr[u] = 0;
+#pragma unroll
+                    for (int step = WAVE / 2; step > 0; step >>= 1) {
+#pragma unroll
+                        for (int u = 0; u < INDEX_PASSES; ++u) {
+                            const int ex = __shfl(excl, pr[u] + step, WAVE);
+                            pr[u] += ex <= base + u * WAVE + lane ? step : 0;
+                        }
                     }
-#pragma unroll
-                for (int p = 0; p < INDEX_PROBES; ++p)
-#pragma unroll
-                    for (int k = 0; k < INDEX_WALK; ++k)
-                        if (kb + k >= n_in[p] && kb + k < c[p])
-                            count(sid[p][k], lv[p], PACK16 ? 1u : aw * wv[p][k]);
-            }
-            continue;
-        }
-#pragma unroll
-        for (int p = 0; p < INDEX_PROBES; ++p) {
-            // balanced walk of this probe's buckets: laid end to end (exclusive
-            // prefix sums) and walked 64 entries per pass: lane j of a pass takes
-            // entry base + j, in the bucket of the last lane r with excl[r] <= base + j
-            int incl = c[p];
-            for (int off = 1; off < WAVE; off <<= 1) {
-                const int u = __shfl_up(incl, off, WAVE);
-                incl += lane >= off ? u : 0;
-            }
-            const int excl = incl - c[p];
-            const int total = __shfl(incl, WAVE - 1, WAVE);
-            // INDEX_PASSES passes at a time: their searches, gathers and id
-            // loads are independent, so their latencies overlap
-            for (int base = 0; base < total; base += INDEX_PASSES * WAVE) {
-                int pr[INDEX_PASSES];
-#pragma unroll
-                for (int u = 0; u < INDEX_PASSES; ++u)
-                    pr[u] = 0;
-#pragma unroll
-                for (int step = WAVE / 2; step > 0; step >>= 1) {
+                    uint32_t e[INDEX_PASSES];
+                    int plv[INDEX_PASSES], sid[INDEX_PASSES];
 #pragma unroll
                     for (int u = 0; u < INDEX_PASSES; ++u) {
-                        const int ex = __shfl(excl, pr[u] + step, WAVE);
-                        pr[u] += ex <= base + u * WAVE + lane ? step : 0;
+                        const int j = base + u * WAVE + lane;
+                        const uint32_t pe0 = (uint32_t)__shfl((int)e0[p], pr[u], WAVE);
+                        plv[u] = __shfl(lv[p], pr[u], WAVE);
+                        const int pex = __shfl(excl, pr[u], WAVE);
+                        e[u] = pe0 + (uint32_t)(j - pex);
+                        sid[u] = j < total ? (int)a.ids[e[u]] : -1;
                     }
-                }
-                uint32_t e[INDEX_PASSES];
-                int plv[INDEX_PASSES], sid[INDEX_PASSES];
 #pragma unroll
-                for (int u = 0; u < INDEX_PASSES; ++u) {
-                    const int j = base + u * WAVE + lane;
-                    const uint32_t pe0 = (uint32_t)__shfl((int)e0[p], pr[u], WAVE);
-                    plv[u] = __shfl(lv[p], pr[u], WAVE);
-                    const int pex = __shfl(excl, pr[u], WAVE);
-                    e[u] = pe0 + (uint32_t)(j - pex);
-                    sid[u] = j < total ? (int)a.ids[e[u]] : -1;
+                    for (int u = 0; u < INDEX_PASSES; ++u)
+                        if (sid[u] >= 0)
+                            count(sid[u], plv[u], PACK > 1 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
                 }
-#pragma unroll
-                for (int u = 0; u < INDEX_PASSES; ++u)
-                    if (sid[u] >= 0)
-                        count(sid[u], plv[u], PACK16 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
             }
         }
-    }
-    __syncthreads();
-    const int64_t off = (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
-    if (PACK16 && (a.nm & 1) == 0) {
-        // word j of SV s holds levels 2j, 2j + 1: the [ns][nm] uint16 counts
-        uint32_t *dst = (uint32_t *)((uint16_t *)a.partial + off);
-        for (int i = threadIdx.x; i < ns * wps; i += INDEX_THREADS) {
-            const int sid = i / wps, j = i - sid * wps;
-            dst[i] = cnt[j * ns + sid];
+    };
+    // 16-bit counts [(nm + 1) / 2][n] of n SVs from SV s0 of the tile out as
+    // the [n][nm] uint16 partials
+    auto write16 = [&](int s0, int n) {
+        const int w16 = (a.nm + 1) / 2;
+        if ((a.nm & 1) == 0) {
+            // word j of SV s holds levels 2j, 2j + 1
+            uint32_t *dst = (uint32_t *)((uint16_t *)a.partial + off + (int64_t)s0 * a.nm);
+            for (int i = threadIdx.x; i < n * w16; i += INDEX_THREADS) {
+                const int sid = i / w16, j = i - sid * w16;
+                dst[i] = cnt[j * n + sid];
+            }
+        } else {
+            uint16_t *dst = (uint16_t *)a.partial + off + (int64_t)s0 * a.nm;
+            for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS) {
+                const int sid = i / a.nm, lv = i - sid * a.nm;
+                dst[i] = (uint16_t)(cnt[(lv >> 1) * n + sid] >> (16 * (lv & 1)));
+            }
         }
-    } else if (PACK16) {
-        uint16_t *dst = (uint16_t *)a.partial + off;
-        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
-            const int sid = i / a.nm, lv = i - sid * a.nm;
-            dst[i] = (uint16_t)(cnt[(lv >> 1) * ns + sid] >> (16 * (lv & 1)));
+    };
+    if constexpr (PACK == 4) {
+        uint32_t adds = 0;
+        walk([&](int sid, int lv, uint32_t) {
+            if (sid < lo_col)
+                return;
+            ++adds;
+            atomicAdd(&cnt[(lv >> 2) * ns + sid], 1u << (8 * (lv & 3)));
+        });
+        __syncthreads();
+        // adds - byte sums over the block: 0 exactly when no add carried (the
+        // block's adds stay below 2^32: at most 65,535 per SV, see the host cap)
+        uint32_t d = adds;
+        for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS) {
+            const uint32_t w = cnt[i];
+            d -= (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
         }
+        __shared__ uint32_t red[INDEX_THREADS / WAVE];
+        for (int o = WAVE / 2; o > 0; o >>= 1)
+            d += (uint32_t)__shfl_xor((int)d, o, WAVE);
+        if (lane == 0)
+            red[wave] = d;
+        __syncthreads();
+        uint32_t carried = 0;
+#pragma unroll
+        for (int k = 0; k < INDEX_THREADS / WAVE; ++k)
+            carried += red[k];
+        if (carried == 0) {
+            uint16_t *dst = (uint16_t *)a.partial + off;
+            if (a.nm == 4) {
+                uint2 *d2 = (uint2 *)dst;
+                for (int s = threadIdx.x; s < ns; s += INDEX_THREADS) {
+                    const uint32_t w = cnt[s];
+                    d2[s] = make_uint2((w & 0xFFu) | ((w & 0xFF00u) << 8),
+                                       ((w >> 16) & 0xFFu) | ((w >> 8) & 0xFF0000u));
+                }
+            } else {
+                for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+                    const int sid = i / a.nm, lv = i - sid * a.nm;
+                    dst[i] = (uint16_t)((cnt[(lv >> 2) * ns + sid] >> (8 * (lv & 3))) & 0xFFu);
+                }
+            }
+            return;
+        }
+        // a count reached 256: the chunk again with 16-bit counts, sub_tile
+        // SVs per pass (what the LDS holds at (nm + 1) / 2 words per SV)
+        for (int s0 = 0; s0 < ns; s0 += a.sub_tile) {
+            const int n = min(a.sub_tile, ns - s0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < n * ((a.nm + 1) / 2); i += INDEX_THREADS)
+                cnt[i] = 0u;
+            __syncthreads();
+            walk([&](int sid, int lv, uint32_t) {
+                const int r = sid - s0;
+                if (sid < lo_col || (unsigned)r >= (unsigned)n)
+                    return;
+                atomicAdd(&cnt[(lv >> 1) * n + r], 1u << (16 * (lv & 1)));
+            });
+            __syncthreads();
+            write16(s0, n);
+        }
+        return;
     } else {
-        uint32_t *dst = (uint32_t *)a.partial + off;
-        for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
-            const int sid = i / a.nm, lv = i - sid * a.nm;
-            dst[i] = cnt[lv * ns + sid];
+        // counts laid out [word][SV], so that a wave's atomics spread over the banks
+        walk([&](int sid, int lv, uint32_t add) {
+            if (sid < lo_col)
+                return;
+            if (PACK == 2)
+                atomicAdd(&cnt[(lv >> 1) * ns + sid], 1u << (16 * (lv & 1)));
+            else
+                atomicAdd(&cnt[lv * ns + sid], add);
+        });
+        __syncthreads();
+        if (PACK == 2) {
+            write16(0, ns);
+        } else {
+            uint32_t *dst = (uint32_t *)a.partial + off;
+            for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+                const int sid = i / a.nm, lv = i - sid * a.nm;
+                dst[i] = cnt[lv * ns + sid];
+            }
         }
     }
 }
@@ -1421,8 +1502,13 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
         return fail(-1, "gkm_index_hist: at most 65535 rows per call, got %d", B);
     if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
         return 0;
-    const bool pack16 = vals == nullptr && pack_counts;
-    size_t lds = (size_t)tile * (pack16 ? (nm + 1) / 2 : nm) * sizeof(uint32_t);
+    if (pack_counts < 0 || pack_counts > 2)
+        return fail(-1, "gkm_index_hist: pack_counts must be 0, 1 or 2, got %d", pack_counts);
+    // counts per LDS word: 1 (uint32, always with weights), 2 (16-bit), 4 (8-bit)
+    const int pack = vals != nullptr ? 1 : pack_counts == 2 ? 4 : pack_counts == 1 ? 2 : 1;
+    const bool pack16 = pack > 1;
+    const size_t words = (size_t)tile * ((nm + pack - 1) / pack);
+    size_t lds = words * sizeof(uint32_t);
     if (lds > 160 * 1024)
         return fail(-2, "gkm_index_hist: a tile of %d SVs x %d levels exceeds LDS", tile, nm);
     IndexArgs a;
@@ -1448,12 +1534,14 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     a.col_min = col_min;
     a.row_base = row_base;
+    a.sub_tile = (int)(words / ((nm + 1) / 2));
     const int rk = records == nullptr ? 0 : vals ? 2 : 1;  // 32-byte records per key
     a.records = records;
     typedef void (*tile_fn)(IndexArgs);
-    const tile_fn kfn = pack16 ? (rk ? k_index_tile<true, 1> : k_index_tile<true, 0>)
-                               : rk == 2 ? k_index_tile<false, 2>
-                               : rk == 1 ? k_index_tile<false, 1> : k_index_tile<false, 0>;
+    const tile_fn kfn = pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
+                      : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
+                      : rk == 2   ? k_index_tile<1, 2>
+                      : rk == 1   ? k_index_tile<1, 1> : k_index_tile<1, 0>;
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
@@ -1461,16 +1549,33 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
     hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_hist");
-    int64_t n = (int64_t)B * S * nm;
-    if (pack16)
+    if (hist == nullptr)
+        return 0;  // the caller sums the partials with gkm_index_reduce
+    return gkm_index_reduce(partial, pack16 ? 2 : 4, xlen, B, Wx, l, nm, S, chunk_windows, hist,
+                            stream);
+}
+
+GKM_API int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B,
+                             int Wx, int l, int nm, int S, int chunk_windows, void *hist,
+                             void *stream)
+{
+    if (count_bytes != 2 && count_bytes != 4)
+        return fail(-1, "gkm_index_reduce: count_bytes must be 2 or 4, got %d", count_bytes);
+    if (chunk_windows < 1)
+        return fail(-1, "gkm_index_reduce: chunk_windows must be >= 1, got %d", chunk_windows);
+    if (B <= 0 || Wx <= 0 || S <= 0 || nm <= 0)
+        return 0;
+    const int nchunks = (Wx + chunk_windows - 1) / chunk_windows;
+    const int64_t n = (int64_t)B * S * nm;
+    if (count_bytes == 2)
         hipLaunchKernelGGL(k_index_reduce<uint16_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                            S_(stream), (const uint16_t *)partial, n, (int64_t)S * nm, xlen, l,
-                           chunk_windows, a.nchunks, (unsigned long long *)hist);
+                           chunk_windows, nchunks, (unsigned long long *)hist);
     else
         hipLaunchKernelGGL(k_index_reduce<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                            S_(stream), (const uint32_t *)partial, n, (int64_t)S * nm, xlen, l,
-                           chunk_windows, a.nchunks, (unsigned long long *)hist);
-    GKM_CHECK_LAUNCH("gkm_index_hist reduce");
+                           chunk_windows, nchunks, (unsigned long long *)hist);
+    GKM_CHECK_LAUNCH("gkm_index_reduce");
     return 0;
 }
 

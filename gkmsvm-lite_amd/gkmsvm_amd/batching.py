@@ -29,7 +29,7 @@ from .inputs import as_batch, full_lengths, nan_rows, short_rows
 class _Plan:
     __slots__ = ("use_index", "shape", "target_blocks", "ms", "table")
 
-    def __init__(self, use_index, shape=None, target_blocks=2048, ms=float("inf"), table=None):
+    def __init__(self, use_index, shape=None, target_blocks=None, ms=float("inf"), table=None):
         self.use_index = use_index
         self.shape = shape
         self.target_blocks = target_blocks
@@ -99,7 +99,8 @@ class BatchScorer:
             return [_Plan(False, table=table)]
         cands = []
         if self.kp.l <= E.INDEX_MAX_L and self.kp.dmax >= 0:
-            cands += [_Plan(True, target_blocks=t) for t in (1024, 2048, 16384)]
+            nt = self.sv.index().n_tiles
+            cands += [_Plan(True, target_blocks=t) for t in (512 * nt, 1024 * nt, 16384)]
         norient = 2 if self.kp.rc else 1
         cands += [_Plan(False, shape=E.launch_shape(self.rows, self.sv.S, norient))]
         cands += [_Plan(False, shape=s) for s in ((1, 1), (2, 1), (1, 2), (4, 1))]
@@ -111,7 +112,7 @@ class BatchScorer:
             return self.plans[key]
         if not self.autotune:
             plan = _Plan(self.sv.use_index_for(lengths, self.model.use_lmer_index),
-                         target_blocks=2048, table=self.model._lmer_table_for())
+                         target_blocks=None, table=self.model._lmer_table_for())
             self.plans[key] = plan
             return plan
         best = None

@@ -291,17 +291,21 @@ class DeviceIndex:
     neighbourhood masks and their levels.  kernels/lmer_index.py:161-200 builds
     the same index without the tiles (a tile's counts fit one block's LDS)."""
 
-    def __init__(self, kp: KernelParams, seqs: DeviceSeqs, stream=None, tile: int | None = None):
+    def __init__(self, kp: KernelParams, seqs: DeviceSeqs, stream=None, tile: int | None = None,
+                 count_bits: int | None = None):
         l = kp.l
         nl = 1 << (2 * l)
         dev = kp.device
         S = seqs.n
-        # 16-bit packed LDS counts (unweighted, and a chunk of at least one window
-        # keeps every count below 2^16)
+        # packed LDS counts (unweighted, and a chunk of at least one window
+        # keeps every count below 2^16): 16-bit, or 8-bit with the kernel's
+        # exact 16-bit fallback where nm >= 3 makes them take fewer words;
+        # count_bits (32, 16, 8) forces a width where it applies
         nb = 2 if kp.rc else 1
         sv_windows = int(np.clip(seqs.lengths - l + 1, 0, None).max()) * nb if S else 0
-        self.pack16 = not kp.weighted and 0 < sv_windows < 65536
-        self.tile = tile or min(index_tile(kp.nm, self.pack16), max(256, -(-S // 256) * 256))
+        self.pack16 = not kp.weighted and 0 < sv_windows < 65536 and count_bits != 32
+        self.pack = index_pack(kp.nm, self.pack16) if count_bits != 16 else 2 if self.pack16 else 1
+        self.tile = tile or min(index_tile(kp.nm, self.pack), max(256, -(-S // 256) * 256))
         self.n_tiles = (S + self.tile - 1) // self.tile
         fwd, rcw = seqs.windows(l, stream)
         blocks = [fwd] + ([rcw] if kp.rc else [])
@@ -387,12 +391,23 @@ class DeviceIndex:
         return sum(x.numel() * x.element_size() for x in t)
 
 
-def index_tile(nm: int, pack16: bool = False) -> int:
-    """SVs per index tile: the tile's counts ([tile, nm] uint32, or two 16-bit
-    counts per word) take up to 144 KiB of the CU's 160 KiB of LDS -- the index
-    is probe-bound, so the largest tile that fits wins (fewer tiles, fewer
-    probes); at most 65,536 (uint16 ids)."""
-    words = (nm + 1) // 2 if pack16 else nm
+def index_pack(nm: int, unweighted: bool) -> int:
+    """Counts per 32-bit LDS word of the index query: 1 (uint32: weighted), 2
+    (16-bit) or 4 (8-bit, where that takes fewer words than 16-bit: nm >= 3)."""
+    if not unweighted:
+        return 1
+    return 4 if (nm + 3) // 4 < (nm + 1) // 2 else 2
+
+
+def index_tile(nm: int, pack: int | bool = 1) -> int:
+    """SVs per index tile: the tile's counts ([tile, nm] uint32, or `pack`
+    counts per word; True: the unweighted packing index_pack picks) take up to
+    144 KiB of the CU's 160 KiB of LDS -- the index is probe-bound, so the
+    largest tile that fits wins (fewer tiles, fewer probes); at most 65,536
+    (uint16 ids)."""
+    if pack is True or pack is False:
+        pack = index_pack(nm, pack)
+    words = -(-nm // pack)
     return min(65536, max(256, (36864 // words) // 256 * 256))
 
 
@@ -427,19 +442,19 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     return out
 
 
-def index_blocks(x_lengths, l: int) -> int:
-    """Default block target of an index launch (measured on 12-105 row batches
-    against 72k SVs): rows of very different lengths -> the finest chunks (16
-    windows), which balance the CUs; rows of one length -> fewer, fuller
-    blocks."""
+def index_blocks(x_lengths, l: int, n_tiles: int = 1) -> int:
+    """Default block target of an index launch (measured on 4-105 row batches
+    against 20k and 72k SVs): rows of very different lengths -> the finest
+    chunks (16 windows), which balance the CUs; rows of one length -> fewer,
+    fuller blocks, 512 per SV tile."""
     wx = np.clip(np.asarray(x_lengths, dtype=np.int64) - l + 1, 0, None)
-    return 16384 if wx.max() > 1.5 * max(1.0, wx.mean()) else 2048
+    return 16384 if wx.max() > 1.5 * max(1.0, wx.mean()) else 512 * max(1, n_tiles)
 
 
 def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     B = x1 - x0
     if target_blocks is None:
-        target_blocks = index_blocks(xs.lengths[x0:x1], kp.l)
+        target_blocks = index_blocks(xs.lengths[x0:x1], kp.l, idx.n_tiles)
     fwd, _ = xs.windows(kp.l, stream)
     Wx = fwd.shape[1]
     wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
@@ -462,13 +477,16 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     keys, pos = torch.sort(keys, dim=1)
     swin = keys.contiguous()
     spos = pos.to(torch.int32).contiguous() if wt is not None else None
+    # the query (timed on its own) and the sum of its per-chunk partials
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
            lambda: _lib.call(
                "gkm_index_hist", _p(swin), _p(spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
                _p(idx.ids), _p(idx.vals), _p(idx.records), _p(idx.masks), _p(idx.levels),
                idx.n_masks, kp.l,
-               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, int(idx.pack16), col_min, x0,
-               _p(partial), _p(out), stream_handle(stream)))
+               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, {1: 0, 2: 1, 4: 2}[idx.pack], col_min, x0,
+               _p(partial), None, stream_handle(stream)))
+    _lib.call("gkm_index_reduce", _p(partial), partial.element_size(), _p(xs.len[x0:x1]), B, Wx,
+              kp.l, kp.nm, S, cw, _p(out), stream_handle(stream))
     return out
 
 

@@ -175,9 +175,12 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * [n_masks] the XOR neighbourhood of mismatch counts < nm with non-zero
  * weight.  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
  * counts, uint16 with pack_counts, else uint32.  pack_counts (unweighted
- * only): 16-bit counts, two per LDS word (the caller keeps chunk_windows *
- * windows per SV below 2^16).  l <= 13,
- * B <= 65535, tile * nm * 4 (packed: tile * ceil(nm/2) * 4) bytes <= 160 KiB.
+ * only; the caller keeps chunk_windows * windows per SV below 2^16): 1 =
+ * 16-bit counts, two per LDS word; 2 = 8-bit counts, four per LDS word, with
+ * an exact fallback -- a block whose byte sums show a carry walks its chunk
+ * again with 16-bit counts, tile * ceil(nm/4) / ceil(nm/2) SVs per pass.
+ * l <= 13, B <= 65535, tile * ceil(nm / p) * 4 bytes <= 160 KiB (p = 1, 2, 4
+ * counts per word).  hist NULL: only the partials (see gkm_index_reduce).
  * LmerIndex.pairwise (kernels/lmer_index.py:239-268, 400-441;
  * kernels/lmer_index_cpu.py:23-67). */
 int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
@@ -186,6 +189,15 @@ int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xle
                    int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
                    int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
                    void *partial, void *hist, void *stream);
+
+/* hist [B][S][nm] uint64 = the sum over window chunks of gkm_index_hist's
+ * partial counts (count_bytes 2: uint16, 4: uint32), chunks past a row's last
+ * window skipped.  gkm_index_hist runs it itself unless its hist is NULL;
+ * called on its own it lets the caller time the index query and its sum apart.
+ * The second half of LmerIndex.pairwise (kernels/lmer_index.py:239-268), which
+ * accumulates its counts in place. */
+int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B, int Wx,
+                     int l, int nm, int S, int chunk_windows, void *hist, void *stream);
 
 /* ---- Exact l-mer tables of non-RBF models (gkm_tables.hip) ----------------
  * V, table  [4^l] float64 indexed by the packed l-mer (base k in bits 2k).

@@ -428,6 +428,41 @@ def test_index_mixed_bucket_sizes(gpu):
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5])
+def test_index_count_widths(gpu, d):
+    """8-bit LDS counts (with the exact 16-bit fallback), 16-bit and 32-bit
+    counts all give the brute-force integers: random rows (no count reaches
+    256), poly-A and dinucleotide repeats (counts past 255: the fallback, in
+    two passes over a 512-SV tile), d + 1 = 2..6 levels, several tiles, and the
+    symmetric Gram's upper triangle (col_min)."""
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(40 + d)
+    L = 300
+    sv = rng.integers(0, 4, (700, L)).astype(np.uint8)
+    sv[:40] = 0
+    sv[600:610] = np.tile([0, 2], L // 2)
+    q = rng.integers(0, 4, (5, L)).astype(np.uint8)
+    q[0, 30:200] = 0
+    q[1] = np.tile([0, 2], L // 2)
+    m = _model(2, {"L": 10, "k": 5, "d": d}, sv, np.full(len(sv), L), rng.normal(size=len(sv)))
+    s = m._svs()
+    assert s.kp.nm == d + 1
+    xs = E.DeviceSeqs(q, np.full(len(q), L), gpu, rc=s.kp.rc)
+    want = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups).cpu().numpy()
+    assert want[:2, :40, 0].max() > 255  # repeats: some counts need the fallback
+    g_want = E.cross_hist(s.kp, s.seqs, 0, s.S, s.seqs, s.groups, col_min=0).cpu().numpy()
+    upper = np.triu(np.ones((s.S, s.S), bool))
+    for bits in (8, 16, 32):
+        idx = E.DeviceIndex(s.kp, s.seqs, tile=512, count_bits=bits)
+        assert idx.pack == {8: 4 if d >= 2 else 2, 16: 2, 32: 1}[bits]
+        print(f"[case] d={d} count_bits={bits} pack={idx.pack} tiles={idx.n_tiles}", flush=True)
+        got = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S).cpu().numpy()
+        np.testing.assert_array_equal(got, want)
+        g = E.index_hist(s.kp, idx, s.seqs, 0, s.S, s.S, col_min=0).cpu().numpy()
+        np.testing.assert_array_equal(g[upper], g_want[upper])
+
+
 def test_index_plain_config(gpu):
     """Plain -t 2 scoring through the l-mer index equals the oracle's index path."""
     rng = np.random.default_rng(1)
