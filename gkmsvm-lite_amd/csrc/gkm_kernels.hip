@@ -15,6 +15,7 @@
 //   hist       [B][S][nm] u64  per-pair mismatch histograms N[m], m < nm
 //
 // Kernels:
+//   k_decode_ascii   ASCII DNA -> base codes, rows with other letters flagged
 //   k_pack_planes    codes -> planes, planes_rc
 //   k_pack_windows   codes -> 2-bit packed l-mer windows (mutant self-kernels)
 //   k_cross_hist     N[m] of query x against SV y (+ RC) -- the hot loop.  One
@@ -62,6 +63,39 @@ GKM_API int gkm_version(void) { return 1; }
 // ---------------------------------------------------------------------------
 // Packing
 // ---------------------------------------------------------------------------
+
+// ASCII DNA -> base codes (codec.py:40-137 of gkmsvm-lite, one string at a
+// time there): row b's bytes raw[starts[b] .. starts[b] + len[b]) become
+// codes[b][0 .. lmax) -- A=0 C=1 G=2 T=3 in either case, N and every other
+// byte stored as A, A past the row -- and bad[b] = 1 when the row holds a byte
+// other than ACGTN / acgtn, else 0.  For those ten letters
+// ((c >> 1) ^ (c >> 2)) & 3 is the code (N: 0).  One block per row.
+__global__ __launch_bounds__(256) void k_decode_ascii(const uint8_t *__restrict__ raw,
+                                                      const int64_t *__restrict__ starts,
+                                                      const int32_t *__restrict__ lengths, int lmax,
+                                                      uint8_t *__restrict__ codes,
+                                                      int32_t *__restrict__ bad)
+{
+    const int b = blockIdx.x;
+    const int len = lengths[b];
+    const uint8_t *src = raw + starts[b];
+    uint8_t *dst = codes + (int64_t)b * lmax;
+    int other = 0;
+    for (int p = threadIdx.x; p < lmax; p += 256) {
+        uint32_t code = 0;
+        if (p < len) {
+            const uint32_t c = src[p];
+            const uint32_t u = c & ~32u;  // the upper case of a letter
+            const bool ok = u == 'A' || u == 'C' || u == 'G' || u == 'T' || u == 'N';
+            other |= ok ? 0 : 1;
+            code = ok ? ((c >> 1) ^ (c >> 2)) & 3u : 0u;
+        }
+        dst[p] = (uint8_t)code;
+    }
+    other = __syncthreads_or(other);
+    if (threadIdx.x == 0)
+        bad[b] = other ? 1 : 0;
+}
 
 // One thread per (sequence, word): 32 codes -> two plane words, forward and
 // reverse complement.  Words past the sequence are zero (read as A, and every
@@ -1685,6 +1719,19 @@ GKM_API int gkm_pack_planes(const uint8_t *codes, const int32_t *lengths, int n,
     hipLaunchKernelGGL(k_pack_planes, dim3((unsigned)blocks), dim3(threads), 0, S_(stream), codes,
                        lengths, n, lmax, nw, (P2 *)planes, (P2 *)planes_rc);
     GKM_CHECK_LAUNCH("gkm_pack_planes");
+    return 0;
+}
+
+GKM_API int gkm_decode_ascii(const uint8_t *raw, const int64_t *starts, const int32_t *lengths,
+                             int n, int lmax, uint8_t *codes, int32_t *bad, void *stream)
+{
+    if (n < 0 || lmax < 1)
+        return fail(-1, "gkm_decode_ascii: need n >= 0 and lmax >= 1 (n=%d lmax=%d)", n, lmax);
+    if (n == 0)
+        return 0;
+    hipLaunchKernelGGL(k_decode_ascii, dim3((unsigned)n), dim3(256), 0, S_(stream), raw, starts,
+                       lengths, lmax, codes, bad);
+    GKM_CHECK_LAUNCH("gkm_decode_ascii");
     return 0;
 }
 

@@ -26,6 +26,7 @@ SIGNATURES = {
     "gkm_zero_u64": [P, I64, P],
     "gkm_pack_planes": [P, P, I, I, I, P, P, P],
     "gkm_pack_windows": [P, P, I, I, I, P, P, P],
+    "gkm_decode_ascii": [P, P, P, I, I, P, P, P],
     "gkm_cross_hist": [P, P, I, I, P, P, P, P, I, I, I, I, P, I, I, I, I64, I64, P, P],
     "gkm_self_hist": [P, P, P, I, I, I, I, P, I, I, P, P],
     "gkm_combine_kraw": [P, I64, I, P, P, P],

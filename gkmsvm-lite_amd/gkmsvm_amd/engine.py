@@ -72,7 +72,45 @@ class DeviceSeqs:
             buf = self._host.to(device, non_blocking=True)
         self.codes = buf[: codes.size].view(self.n, self.lmax)
         self.len = buf[nc:].view(torch.int32)
-        self.planes = torch.empty((self.n, self.nw, 2), dtype=torch.int32, device=device)
+        self._pack(rc, stream)
+
+    @classmethod
+    def from_strings(cls, strings, device, *, rc: bool = True, stream=None, min_width: int = 1):
+        """(DeviceSeqs of DNA strings, bad [n] int32 on the device): the
+        strings' bytes, unpadded, go over in one pinned copy and
+        gkm_decode_ascii writes the padded codes (width max(Lmax, min_width); N
+        as A) and flags rows with a letter other than ACGTN (bad = 1; empty
+        rows are the caller's to flag, from .lengths)."""
+        self = cls.__new__(cls)
+        n = len(strings)
+        lens = np.fromiter(map(len, strings), dtype=np.int64, count=n)
+        # one byte per character: a non-ASCII character becomes "?", flagged
+        raw = "".join(strings).encode("ascii", "replace")
+        self.n, self.lmax = n, max(int(lens.max()) if n else 0, min_width, 1)
+        self.nw = self.lmax // 32 + 4
+        self.lengths = lens
+        self.device = device
+        nr = -(-len(raw) // 8) * 8
+        self._host = torch.empty(nr + 12 * n, dtype=torch.uint8, pin_memory=True)
+        hb = self._host.numpy()
+        hb[: len(raw)] = np.frombuffer(raw, dtype=np.uint8)
+        starts = hb[nr: nr + 8 * n].view(np.int64)
+        if n:
+            starts[0] = 0
+            np.cumsum(lens[:-1], out=starts[1:])
+        hb[nr + 8 * n:].view(np.int32)[:] = lens
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            buf = self._host.to(device, non_blocking=True)
+        self.len = buf[nr + 8 * n:].view(torch.int32)
+        self.codes = torch.empty((n, self.lmax), dtype=torch.uint8, device=device)
+        bad = torch.empty(n, dtype=torch.int32, device=device)
+        _lib.call("gkm_decode_ascii", _p(buf), _p(buf[nr:]), _p(self.len), n, self.lmax,
+                  _p(self.codes), _p(bad), stream_handle(stream))
+        self._pack(rc, stream)
+        return self, bad
+
+    def _pack(self, rc: bool, stream):
+        self.planes = torch.empty((self.n, self.nw, 2), dtype=torch.int32, device=self.device)
         self.planes_rc = torch.empty_like(self.planes) if rc else None
         _lib.call("gkm_pack_planes", _p(self.codes), _p(self.len), self.n, self.lmax, self.nw,
                   _p(self.planes), _p(self.planes_rc), stream_handle(stream))

@@ -4,8 +4,9 @@ The model's SV tables, coefficients and self-kernels stay in HBM.  Clients
 submit scoring, variant (score_variants, kernel method) and GkmExplain
 requests from any thread and get a Future.  A launcher thread admits pending
 requests into GPU steps while earlier steps run: each step packs every
-admitted scoring/variant row into one batch (rows of any length, encoded in
-one pass and sent in one pinned copy) and launches the index or pairwise
+admitted scoring/variant row into one batch (rows of any length; their bytes
+go over in one pinned copy and are decoded on the device, where rows with a
+letter other than ACGTN are flagged to fail alone) and launches the index or pairwise
 kernel (or the l-mer table lookup) once for all of them on its own HIP
 stream; the step's explain requests follow as one batch per mode.  A
 harvester thread waits on each step's completion event and hands out its
@@ -36,7 +37,10 @@ import torch
 from . import engine as E
 from . import tables as T
 from .codec import encode_batch
+from . import _lib
 from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
+
+_BAD_SEQ = "Invalid sequence (empty, or a letter other than ACGTN)"
 
 
 class Request:
@@ -74,8 +78,12 @@ class _Step:
     stream: torch.cuda.Stream
     done: torch.cuda.Event
     score_host: torch.Tensor | None
+    score_short: np.ndarray | None  # scoring rows shorter than l (scored NaN)
+    bad_host: torch.Tensor | None   # [rows] int32: a letter other than ACGTN (device flags)
+    empty: np.ndarray | None        # [rows] empty strings
     spans: list               # (request, start row, n rows) for scoring/variant rows
-    explain_out: list         # (request, host tensor [rows, 4, L], first row, rows, length)
+    explain_out: list         # (request, host [rows, 4, L], first row, rows, length,
+                              #  short rows, first row in the step's rows or None)
     keep: list                # device tensors kept alive until the step completes
 
 
@@ -107,6 +115,9 @@ class GkmServer:
         self.max_explain = int(max_explain)
         self._streams = [torch.cuda.Stream(device=self.kp.device) for _ in range(streams)]
         self._free = list(self._streams)
+        # pinned result buffers, one set per stream: a stream's step is
+        # delivered (its results copied out) before the stream takes a new step
+        self._pinned = {id(st): {} for st in self._streams}
         self._pending: deque = deque()
         self._inflight: list = []
         self._lock = threading.Condition()
@@ -132,6 +143,10 @@ class GkmServer:
             raise ValueError(f"unknown request kind {kind!r}")
         req = Request(kind, list(seqs), mode)
         req.t_submit = time.perf_counter()
+        if req.n_rows == 0:  # nothing to compute: the empty result at once
+            req.t_admit = req.t_done = req.t_submit
+            req.future.set_result(np.empty((0, 4, 0) if kind == "explain" else 0))
+            return req.future
         need = req.nbytes = self._bytes(req)
         if need > self.hbm_budget:
             req.future.set_exception(MemoryError(
@@ -250,102 +265,229 @@ class GkmServer:
                 self._free.append(st.stream)
                 self._lock.notify_all()
 
+    def _host_buffer(self, stream, key: str, shape, dtype=torch.float64) -> torch.Tensor:
+        """A view of `shape` on the stream's pinned buffer `key` (grown when
+        too small)."""
+        pool = self._pinned[id(stream)]
+        n = int(np.prod(shape))
+        buf = pool.get(key)
+        if buf is None or buf.numel() < n or buf.dtype != dtype:
+            old = buf.numel() if buf is not None and buf.dtype == dtype else 0
+            buf = torch.empty(max(n, 2 * old), dtype=dtype, pin_memory=True)
+            pool[key] = buf
+        return buf[:n].view(shape)
+
     def _deliver(self, st):
         now = time.perf_counter()
-        # one copy of the step's scores (the pinned buffer is reused); each
-        # request gets its own slice of it
-        scores = st.score_host.numpy().copy() if st.score_host is not None else None
+        # rows with a letter other than ACGTN (flagged on the device) or empty
+        bad = None
+        if st.bad_host is not None:
+            bad = st.bad_host.numpy() != 0
+            bad |= st.empty
+            if not bad.any():
+                bad = None
+        # one copy of the step's scores out of the stream's pinned buffer; each
+        # request gets its own slice of it; rows shorter than l score NaN
+        scores = None
+        if st.score_host is not None:
+            scores = st.score_host.numpy().copy()
+            if st.score_short is not None:
+                scores[st.score_short] = np.nan
+        results = {}
         for req, r0, n in st.spans:
+            if bad is not None and bad[r0:r0 + n].any():
+                results[id(req)] = ValueError(_BAD_SEQ)  # fails alone
+                continue
             vals = scores[r0:r0 + n]
             if req.kind == "variant":
                 half = n // 2
-                res = vals[half:] - vals[:half]  # alt - ref
+                results[id(req)] = vals[half:] - vals[:half]  # alt - ref
             else:
-                res = vals
-            req.t_done = now
-            req.future.set_result(res)
-            self.completed.append(req)
-        for req, host, r0, n, L in st.explain_out:
+                results[id(req)] = vals
+        for req, host, r0, n, L, short, g0 in st.explain_out:
             if req.future.done():  # rejected at launch
                 continue
+            if g0 is not None and bad is not None and bad[g0:g0 + n].any():
+                results[id(req)] = ValueError(_BAD_SEQ)
+                continue
+            res = host[r0:r0 + n, :, :L].numpy().copy()
+            if short is not None and short[r0:r0 + n].any():
+                res[short[r0:r0 + n]] = np.nan
+            results[id(req)] = res
+        # results set newest first: a client that waits on its requests in
+        # submission order wakes once, after the step's last result, rather
+        # than contending for the interpreter while the rest are handed out
+        for req in reversed(st.reqs):
+            res = results.get(id(req))
+            if res is None:
+                continue  # failed at launch
             req.t_done = now
-            req.future.set_result(host[r0:r0 + n, :, :L].numpy().copy())
-            self.completed.append(req)
+            if isinstance(res, Exception):
+                req.future.set_exception(res)
+            else:
+                req.future.set_result(res)
+                self.completed.append(req)
 
-    def _launch(self, batch, stream) -> _Step:
-        kp, sv, model = self.kp, self.sv, self.model
-        t = time.perf_counter()
-        spans, strings, keep = [], [], []
+    def _rows(self, batch, t):
+        """The step's scoring/variant rows: (spans, strings), and its explain
+        requests."""
+        spans, strings, explains = [], [], []
         for req in batch:
             req.t_admit = t
             if req.kind == "score":
-                rows = req.seqs
+                spans.append((req, len(strings), len(req.seqs)))
+                strings.extend(req.seqs)
             elif req.kind == "variant":
-                rows = [r for r, _ in req.seqs] + [a for _, a in req.seqs]
+                spans.append((req, len(strings), 2 * len(req.seqs)))
+                strings.extend([r for r, _ in req.seqs])
+                strings.extend([a for _, a in req.seqs])
             else:
+                explains.append(req)
+        return spans, strings, explains
+
+    def _explain_calls(self, reqs, stream, keep, ex_out):
+        """Explain requests through gkmexplain, one call per mode (rows of any
+        length; each row's attributions depend only on that row); a request
+        with an empty string or a letter other than ACGTN fails alone."""
+        from .explain import gkmexplain
+
+        by_mode: dict = {}
+        for req in reqs:
+            by_mode.setdefault(req.mode, []).append(req)
+        for i, (mode, group) in enumerate(by_mode.items()):
+            _, _, bad = encode_batch([q for req in group for q in req.seqs])
+            good, r0 = [], 0
+            for req in group:
+                if bad[r0:r0 + len(req.seqs)].any():
+                    req.future.set_exception(ValueError(_BAD_SEQ))
+                else:
+                    good.append(req)
+                r0 += len(req.seqs)
+            if not good:
                 continue
-            spans.append((req, len(strings), len(rows)))
-            strings.extend(rows)
-        score_host = None
-        explains = [r for r in batch if r.kind == "explain"]
+            seqs = [q for req in good for q in req.seqs]
+            res = gkmexplain(self.model, seqs, mode=mode, batch_size=len(seqs))
+            hv = self._host_buffer(stream, f"explain{i}", tuple(res.shape))
+            hv.copy_(res, non_blocking=True)
+            keep.append(res)
+            rr = 0
+            for req in good:
+                ex_out.append((req, hv, rr, len(req.seqs), req.max_len(), None, None))
+                rr += len(req.seqs)
+
+    def _launch_tables(self, batch, stream) -> _Step:
+        """A step through the exact l-mer tables: the scoring/variant rows and
+        the explain rows the explain tables serve go to the device in one
+        pinned copy of their bytes, are decoded and packed there (rows with
+        other letters flagged, failing alone when delivered) and get their
+        self-kernels in one launch; then one table-score launch over the
+        scoring rows and one table-explain launch per mode write one device
+        buffer, which comes back in one pinned copy.  Explain requests of
+        other modes go through gkmexplain."""
+        from .explain import _MODE_ALIASES
+
+        kp, sv, model = self.kp, self.sv, self.model
+        spans, strings, explains = self._rows(batch, time.perf_counter())
+        ns = len(strings)
+        groups = {"hypothetical": [], "perturbation": []} if self.explain_tables else {}
+        others = []
+        for req in explains:
+            g = groups.get(_MODE_ALIASES.get(req.mode))
+            (g if g is not None else others).append(req)
+        ex_groups = []  # (perturbation, requests, first row, rows, output width)
+        for mode, reqs in groups.items():
+            r0 = len(strings)
+            for req in reqs:
+                strings.extend(req.seqs)
+            if len(strings) > r0:
+                width = max(kp.l, max(req.max_len() for req in reqs))
+                ex_groups.append((mode == "perturbation", reqs, r0, len(strings) - r0, width))
+        score_host = score_short = bad_host = empty = None
+        ex_out, keep = [], []
         with torch.cuda.stream(stream):
             if strings:
-                # every scoring row of the step encoded in one pass; a request
-                # with an empty string or a letter other than ACGTN fails alone
-                codes, lens, bad = encode_batch(strings, kp.l)
-                if bad.any():
-                    for req, r0, n in spans:
-                        if bad[r0:r0 + n].any():
-                            req.future.set_exception(ValueError(
-                                "Invalid sequence (empty, or a letter other than ACGTN)"))
-                    spans = [sp for sp in spans if not sp[0].future.done()]
-                xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
-                out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
-                E.score_rows(sv, xs, model.bias, out, table=self.table)
-                out = torch.where(xs.len < kp.l, torch.full_like(out, float("nan")), out)
-                score_host = torch.empty(out.shape, dtype=torch.float64, pin_memory=True)
-                score_host.copy_(out, non_blocking=True)
-                keep += [xs, out]
-            ex_out = []
-            by_mode: dict = {}
-            for req in explains:
-                by_mode.setdefault(req.mode, []).append(req)
-            for mode, reqs in by_mode.items():
-                # every explain request of this mode in one call (rows of any
-                # length; each row's attributions depend only on that row)
-                from .explain import _MODE_ALIASES, _explain_rows_table, gkmexplain
-
-                seqs = [q for req in reqs for q in req.seqs]
-                resolved = _MODE_ALIASES.get(mode)
-                if self.explain_tables and resolved in ("hypothetical", "perturbation"):
-                    # straight to the explain table: the step's rows encoded in
-                    # one pass, as gkmexplain does through the same table
-                    codes, lens, bad = encode_batch(seqs, kp.l)
-                    r0 = 0
+                xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
+                                                     min_width=kp.l)
+                lens = xs.lengths
+                empty = lens == 0
+                short = lens < kp.l  # NaN rows, set when delivered
+                dx = E.self_kraw(kp, xs, stream) if kp.normalize else None
+                fwd, _ = xs.windows(kp.l, stream)
+                wt, _ = kp.wtab(xs.lmax)
+                W = fwd.shape[1]
+                total = ns + sum(n * 4 * w for _, _, _, n, w in ex_groups)
+                dev = torch.empty(total, dtype=torch.float64, device=kp.device)
+                sh = E.stream_handle(stream)
+                if ns:
+                    E._timed("k_table_score",
+                             lambda: float(np.clip(lens[:ns] - kp.l + 1, 0, None).sum()) * 12.0,
+                             stream, lambda: _lib.call(
+                                 "gkm_table_score", E._p(fwd), E._p(xs.len), ns, W, kp.l,
+                                 E._p(wt), E._p(self.table), E._p(dx), int(kp.normalize),
+                                 float(model.bias), E._p(dev), sh))
+                off = ns
+                for pert, reqs, r0, n, w in ex_groups:
+                    E._timed("k_table_explain",
+                             lambda: (float(np.clip(lens[r0:r0 + n] - kp.l + 1, 0, None).sum())
+                                      * (4 + 32 * kp.l) + n * 32.0 * w),
+                             stream, lambda: _lib.call(
+                                 "gkm_table_explain", E._p(fwd[r0:]), E._p(xs.len[r0:]), n, W, w,
+                                 kp.l, E._p(wt), E._p(sv.tables[("explain", pert)]),
+                                 E._p(dx[r0:]) if dx is not None else None, int(kp.normalize),
+                                 int(pert), E._p(dev[off:]), sh))
+                    off += n * 4 * w
+                host = self._host_buffer(stream, "out", (total,))
+                host.copy_(dev, non_blocking=True)
+                bad_host = self._host_buffer(stream, "bad", (len(strings),), torch.int32)
+                bad_host.copy_(dbad, non_blocking=True)
+                keep += [xs, dev, dx, dbad]
+                if ns:
+                    score_host = host[:ns]
+                    score_short = short[:ns] if short[:ns].any() else None
+                off = ns
+                for _, reqs, r0, n, w in ex_groups:
+                    hv = host[off:off + n * 4 * w].view(n, 4, w)
+                    sg = short[r0:r0 + n] if short[r0:r0 + n].any() else None
+                    rr = 0
                     for req in reqs:
-                        if bad[r0:r0 + len(req.seqs)].any():
-                            req.future.set_exception(ValueError(
-                                "Invalid sequence (empty, or a letter other than ACGTN)"))
-                        r0 += len(req.seqs)
-                    pert = resolved == "perturbation"
-                    res = torch.empty((len(seqs), 4, codes.shape[1]), dtype=torch.float64,
-                                      device=kp.device)
-                    _explain_rows_table(model, codes, lens, sv.tables[("explain", pert)], pert,
-                                        res)
-                    short = torch.from_numpy(lens < kp.l).to(kp.device, non_blocking=True)
-                    res = torch.where(short[:, None, None], torch.full_like(res, float("nan")), res)
-                else:
-                    res = gkmexplain(model, seqs, mode=mode, batch_size=len(seqs))
-                host = torch.empty(res.shape, dtype=torch.float64, pin_memory=True)
-                host.copy_(res, non_blocking=True)
-                keep.append(res)
-                r0 = 0
-                for req in reqs:
-                    ex_out.append((req, host, r0, len(req.seqs), req.max_len()))
-                    r0 += len(req.seqs)
+                        ex_out.append((req, hv, rr, len(req.seqs), req.max_len(), sg, r0 + rr))
+                        rr += len(req.seqs)
+                    off += n * 4 * w
+            self._explain_calls(others, stream, keep, ex_out)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return _Step(batch, stream, ev, score_host, spans, ex_out, keep)
+        return _Step(batch, stream, ev, score_host, score_short, bad_host, empty, spans, ex_out,
+                     keep)
+
+    def _launch(self, batch, stream) -> _Step:
+        """A step through the per-SV kernels (or, with tables, _launch_tables):
+        every scoring/variant row of the step decoded on the device from one
+        pinned copy of their bytes and scored in one batch."""
+        if self.table is not None:
+            return self._launch_tables(batch, stream)
+        kp, sv, model = self.kp, self.sv, self.model
+        spans, strings, explains = self._rows(batch, time.perf_counter())
+        score_host = score_short = bad_host = empty = None
+        ex_out, keep = [], []
+        with torch.cuda.stream(stream):
+            if strings:
+                xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
+                                                     min_width=kp.l)
+                empty = xs.lengths == 0
+                out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
+                E.score_rows(sv, xs, model.bias, out, stream=stream)
+                short = xs.lengths < kp.l
+                score_short = short if short.any() else None
+                score_host = self._host_buffer(stream, "score", out.shape)
+                score_host.copy_(out, non_blocking=True)
+                bad_host = self._host_buffer(stream, "bad", (len(strings),), torch.int32)
+                bad_host.copy_(dbad, non_blocking=True)
+                keep += [xs, out, dbad]
+            self._explain_calls(explains, stream, keep, ex_out)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return _Step(batch, stream, ev, score_host, score_short, bad_host, empty, spans, ex_out,
+                     keep)
 
 
 def latency_stats(reqs) -> dict:

@@ -45,6 +45,14 @@ int gkm_sync(void *stream);
 /* Zero n uint64 with a kernel (usable under stream capture). */
 int gkm_zero_u64(void *p, int64_t n, void *stream);
 
+/* ASCII DNA -> base codes: row b is raw[starts[b] .. starts[b] + lengths[b])
+ * (starts int64, lengths int32, every row's bytes in raw); codes [n][lmax]
+ * uint8 gets A=0 C=1 G=2 T=3 (either case), N and any other byte as A, A past
+ * the row; bad [n] int32 = 1 where a row holds a byte other than ACGTN/acgtn.
+ * Replaces encode_codes / to_codes over a batch (codec.py:40-137). */
+int gkm_decode_ascii(const uint8_t *raw, const int64_t *starts, const int32_t *lengths, int n,
+                     int lmax, uint8_t *codes, int32_t *bad, void *stream);
+
 /* Base codes [n][lmax] uint8 (A=0 C=1 G=2 T=3; N stored as A) -> planes and
  * reverse-complement planes (planes_rc may be NULL).
  * Replaces PackedGkmKernel.pack / pack_both (kernels/packed.py:130-187). */

@@ -58,8 +58,9 @@ def test_batch_scorer_matches_plain_calls(gpu, kt, extra, table):
 @pytest.mark.parametrize("tables", [False, True])
 def test_server_results_equal_alone_and_in_the_mix(gpu, tables):
     """Requests submitted concurrently from several threads (scoring of many
-    lengths, variants, explain), served in overlapping steps on two streams,
-    return what each gives alone (per-SV kernels, or the l-mer tables the
+    lengths, variants, explain in the three modes with rows shorter than l),
+    served in overlapping steps on two streams, return what each gives alone,
+    and empty requests their empty results (per-SV kernels, or the l-mer tables the
     server builds at start)."""
     from gkmsvm_amd import gkmexplain
     from gkmsvm_amd.serving import GkmServer
@@ -79,32 +80,36 @@ def test_server_results_equal_alone_and_in_the_mix(gpu, tables):
             ref = _queries(1, 200, 300, 200 + i)[0]
             alt = ref[:100] + ("A" if ref[100] != "A" else "C") + ref[101:]
             reqs.append(("variant", [(ref, alt)]))
-        else:
-            reqs.append(("explain", _queries(1, 300, 600, 300 + i)))
+        else:  # explain of 1-2 rows (some shorter than l: NaN), each mode
+            n = int(rng.integers(1, 3))
+            reqs.append(("explain", _queries(n, 5 if n == 2 else 300, 600, 300 + i)))
+    modes = ["hypothetical", "perturbation", "importance"]
     alone = []
-    for kind, payload in reqs:
+    for i, (kind, payload) in enumerate(reqs):
         if kind == "score":
             alone.append(m(payload).cpu().numpy().ravel())
         elif kind == "variant":
             alone.append(m.score_variants([p[0] for p in payload],
                                           [p[1] for p in payload]).cpu().numpy().ravel())
         else:
-            alone.append(gkmexplain(m, payload, mode="hypothetical").cpu().numpy())
+            alone.append(gkmexplain(m, payload, mode=modes[i % 3]).cpu().numpy())
     futs = [None] * len(reqs)
     with srv:
         def client(k):
             for i in range(k, len(reqs), 3):
                 kind, payload = reqs[i]
-                futs[i] = srv.submit(kind, payload)
+                futs[i] = srv.submit(kind, payload, mode=modes[i % 3])
         ths = [threading.Thread(target=client, args=(k,)) for k in range(3)]
         for t in ths:
             t.start()
         for t in ths:
             t.join()
         got = [f.result(timeout=120) for f in futs]
+        empty = [srv.submit(k, []).result(timeout=60) for k in ("score", "variant", "explain")]
+    assert [e.shape for e in empty] == [(0,), (0,), (0, 4, 0)]
     for (kind, _), g, a in zip(reqs, got, alone):
         if kind == "explain":
-            H.assert_close(g, a, 1e-10, "explain")
+            H.assert_close(g, a, 1e-10, "explain")  # shapes and NaN rows too
         else:
             np.testing.assert_array_equal(g, a)
 
@@ -128,24 +133,38 @@ def test_server_admission_rejects_oversized_and_small_budget_still_exact(gpu):
 
 @pytest.mark.parametrize("tables", [False, True])
 def test_server_bad_request_fails_alone(gpu, tables):
-    """A request with an empty string or a letter other than ACGTN fails with
+    """A request with an empty string or a letter other than ACGTN (incl. a
+    non-ASCII one; scoring, variant or explain in either path) fails with
     ValueError (as model(x) would) while the requests of the same step score
-    as alone; short rows give NaN."""
+    and explain as alone; short rows give NaN."""
     from gkmsvm_amd.serving import GkmServer
 
+    from gkmsvm_amd import gkmexplain
+
     m = _random_model(S=300, seed=6)
-    m.use_lmer_table = tables  # the path the server takes
+    m.use_lmer_table = m.use_explain_table = tables  # the path the server takes
     qs = _queries(6, 20, 300, 11) + ["ACGTACG"]  # the last one shorter than l
     want = m(qs).cpu().numpy().ravel()
+    ex = _queries(2, 100, 200, 12)
+    want_ex = [gkmexplain(m, [q], mode=md).cpu().numpy()
+               for q, md in zip(ex, ("hypothetical", "importance"))]
     with GkmServer(m, hbm_budget=1 << 30, tables=tables) as srv:
         srv.stop()  # queue everything first, so the bad requests share a step
         futs = [srv.submit("score", [q]) for q in qs[:3]]
-        bad = [srv.submit("score", ["ACGTXACGTACGTACGT"]), srv.submit("variant", [("", "")])]
+        bad = [srv.submit("score", ["ACGTXACGTACGTACGT"]), srv.submit("variant", [("", "")]),
+               srv.submit("score", ["ACGTACGTAC\u00e9GTACGT"]),  # non-ASCII
+               srv.submit("explain", [ex[0], "ACGTACGTACGTACGT?"]),
+               srv.submit("explain", ["ACGTNNNNacgtRACGT"], mode="importance")]
         futs += [srv.submit("score", [q]) for q in qs[3:]]
+        fex = [srv.submit("explain", [q], mode=md) for q, md in zip(ex, ("hypothetical",
+                                                                        "importance"))]
         srv.start()
         for f in bad:
             with pytest.raises(ValueError):
                 f.result(timeout=60)
         got = np.array([f.result(timeout=60)[0] for f in futs])
+        got_ex = [f.result(timeout=60) for f in fex]
     np.testing.assert_array_equal(got, want)
     assert np.isnan(got[-1])
+    for g, w in zip(got_ex, want_ex):
+        H.assert_close(g, w, 1e-10, "explain next to bad requests")

@@ -84,3 +84,35 @@ def test_deltasvm_weight_file_scores(gpu, tmp_path):
     got = d(codes, lengths=lens).cpu().numpy().ravel()
     want = O.deltasvm_score(d.weights, 10, k, codes, lens, include_rc=d.include_rc)
     H.assert_close(got, want, 1e-12, "deltasvm file")
+
+
+def test_strings_decoded_on_the_device(gpu):
+    """DeviceSeqs.from_strings (the strings' bytes decoded by gkm_decode_ascii)
+    gives the codes, planes and windows of encode_batch + DeviceSeqs, and flags
+    exactly the rows encode_batch flags for a letter other than ACGTN (either
+    case, non-ASCII included)."""
+    import torch
+
+    from gkmsvm_amd import engine as E
+    from gkmsvm_amd.codec import encode_batch
+
+    rng = np.random.default_rng(8)
+    letters = list("ACGTacgtNn")
+    strs = ["".join(rng.choice(letters, int(rng.integers(1, 700)))) for _ in range(40)]
+    strs += ["", "A", "ACGTACGTACG", "acgtXacgt", "ACGTéACGT", "ACGT ACGT", "NNNNNNNNNNNN",
+             "ACGU", "@ACGT", "acgt[", "ACGTacgt" * 300]
+    codes, lens, bad = encode_batch(strs, 11)
+    xs, dbad = E.DeviceSeqs.from_strings(strs, gpu, rc=True, min_width=11)
+    ref = E.DeviceSeqs(codes, lens, gpu, rc=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(xs.lengths, lens)
+    assert xs.lmax == codes.shape[1]
+    np.testing.assert_array_equal((dbad.cpu().numpy() != 0) | (lens == 0), bad)
+    ok = ~bad
+    np.testing.assert_array_equal(xs.codes.cpu().numpy()[ok], codes[ok])
+    np.testing.assert_array_equal(xs.planes.cpu().numpy()[ok], ref.planes.cpu().numpy()[ok])
+    np.testing.assert_array_equal(xs.planes_rc.cpu().numpy()[ok], ref.planes_rc.cpu().numpy()[ok])
+    fw, rw = xs.windows(11)
+    fr, rr = ref.windows(11)
+    np.testing.assert_array_equal(fw.cpu().numpy()[ok], fr.cpu().numpy()[ok])
+    np.testing.assert_array_equal(rw.cpu().numpy()[ok], rr.cpu().numpy()[ok])
