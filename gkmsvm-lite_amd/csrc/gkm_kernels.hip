@@ -1179,21 +1179,28 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
         for (int k = 0; k < INDEX_THREADS / WAVE; ++k)
             carried += red[k];
+        // partials: the low bytes of every count [nchunks][B][S][nm], then the
+        // high bytes (written by blocks that took the 16-bit walk only), then
+        // one flag per block [nchunks][B][n_tiles] (1: high bytes written)
+        const int64_t n_all = (int64_t)a.nchunks * a.B * a.S * a.nm;
+        uint8_t *lo = (uint8_t *)a.partial + off;
+        uint8_t *hi = lo + n_all;
+        uint8_t *flag = (uint8_t *)a.partial + 2 * n_all +
+                        ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id;
         if (carried == 0) {
-            uint16_t *dst = (uint16_t *)a.partial + off;
             if (a.nm == 4) {
-                uint2 *d2 = (uint2 *)dst;
-                for (int s = threadIdx.x; s < ns; s += INDEX_THREADS) {
-                    const uint32_t w = cnt[s];
-                    d2[s] = make_uint2((w & 0xFFu) | ((w & 0xFF00u) << 8),
-                                       ((w >> 16) & 0xFFu) | ((w >> 8) & 0xFF0000u));
-                }
+                // the word of an SV is its 4 counts in level order: a plain copy
+                uint32_t *d4 = (uint32_t *)lo;
+                for (int s = threadIdx.x; s < ns; s += INDEX_THREADS)
+                    d4[s] = cnt[s];
             } else {
                 for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
                     const int sid = i / a.nm, lv = i - sid * a.nm;
-                    dst[i] = (uint16_t)((cnt[(lv >> 2) * ns + sid] >> (8 * (lv & 3))) & 0xFFu);
+                    lo[i] = (uint8_t)(cnt[(lv >> 2) * ns + sid] >> (8 * (lv & 3)));
                 }
             }
+            if (threadIdx.x == 0)
+                *flag = 0;
             return;
         }
         // a count reached 256: the chunk again with 16-bit counts, sub_tile
@@ -1211,8 +1218,16 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 atomicAdd(&cnt[(lv >> 1) * n + r], 1u << (16 * (lv & 1)));
             });
             __syncthreads();
-            write16(s0, n);
+            for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS) {
+                const int sid = i / a.nm, lv = i - sid * a.nm;
+                const uint32_t c = (cnt[(lv >> 1) * n + sid] >> (16 * (lv & 1))) & 0xFFFFu;
+                const int64_t e = (int64_t)s0 * a.nm + i;
+                lo[e] = (uint8_t)c;
+                hi[e] = (uint8_t)(c >> 8);
+            }
         }
+        if (threadIdx.x == 0)
+            *flag = 1;
         return;
     } else {
         // counts laid out [word][SV], so that a wave's atomics spread over the banks
@@ -1565,6 +1580,99 @@ __global__ __launch_bounds__(256) void k_index_reduce4(const T *__restrict__ par
     hist[i + 3] += v3;
 }
 
+// The same sum over the byte partials of the 8-bit count kernel: low bytes,
+// plus the high bytes of the blocks (chunk, row, SV tile) whose flag is set;
+// 4 consecutive counts per thread (one 4-byte load per chunk when n % 4 == 0).
+__global__ __launch_bounds__(256) void k_index_reduce8(const uint8_t *__restrict__ partial,
+                                                       int64_t n, int64_t row_n, int nm, int tile,
+                                                       int n_tiles, int B,
+                                                       const int32_t *__restrict__ xlen, int l,
+                                                       int chunk_windows, int nchunks,
+                                                       unsigned long long *__restrict__ hist)
+{
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= n)
+        return;
+    const int64_t n_all = (int64_t)nchunks * n;
+    const uint8_t *hi = partial + n_all;
+    const uint8_t *flags = partial + 2 * n_all;
+    // each count's flag column (row, SV tile) and number of chunks (its row's)
+    int fr[4], uk[4];
+    int um = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        fr[k] = 0;
+        uk[k] = 0;
+        const int64_t e = i + k;
+        if (e < n) {
+            const int64_t r = e / row_n;
+            const int sv = (int)((e - r * row_n) / nm);
+            const int wx = xlen[r] - l + 1;
+            fr[k] = (int)r * n_tiles + sv / tile;
+            uk[k] = wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
+            um = max(um, uk[k]);
+        }
+    }
+    const bool vec = (n & 3) == 0;
+    unsigned long long v[4] = {0ull, 0ull, 0ull, 0ull};
+    if (vec && fr[0] == fr[3] && uk[0] == uk[3]) {
+        // the 4 counts share a row and an SV tile (nearly always): one flag
+        // and one 4-byte load per chunk, 4 chunks in flight
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // < 2^32: at most 2^16 per chunk
+        int c = 0;
+        for (; c < um; c++) {
+            const uchar4 p = *(const uchar4 *)(partial + (int64_t)c * n + i);
+            uint32_t x0 = p.x, x1 = p.y, x2 = p.z, x3 = p.w;
+            if (flags[(int64_t)c * B * n_tiles + fr[0]]) {
+                const uchar4 h = *(const uchar4 *)(hi + (int64_t)c * n + i);
+                x0 |= (uint32_t)h.x << 8;
+                x1 |= (uint32_t)h.y << 8;
+                x2 |= (uint32_t)h.z << 8;
+                x3 |= (uint32_t)h.w << 8;
+            }
+            s0 += x0;
+            s1 += x1;
+            s2 += x2;
+            s3 += x3;
+            if ((c & 0xFFFF) == 0xFFFF) {  // flush before 2^16 chunks could overflow
+                v[0] += s0, v[1] += s1, v[2] += s2, v[3] += s3;
+                s0 = s1 = s2 = s3 = 0;
+            }
+        }
+        v[0] += s0, v[1] += s1, v[2] += s2, v[3] += s3;
+        um = 0;  // done
+    }
+    for (int c = 0; c < um; c++) {
+        const uint8_t *pc = partial + (int64_t)c * n + i;
+        uint32_t pv[4];
+        if (vec) {
+            const uchar4 p = *(const uchar4 *)pc;
+            pv[0] = p.x;
+            pv[1] = p.y;
+            pv[2] = p.z;
+            pv[3] = p.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                pv[k] = c < uk[k] ? pc[k] : 0u;
+        }
+        const uint8_t *fc = flags + (int64_t)c * B * n_tiles;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (c < uk[k]) {
+                uint32_t x = pv[k];
+                if (fc[fr[k]])
+                    x |= (uint32_t)hi[(int64_t)c * n + i + k] << 8;
+                v[k] += x;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + k < n)
+            hist[i + k] += v[k];
+}
+
 GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                            const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
                            const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
@@ -1633,22 +1741,32 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     GKM_CHECK_LAUNCH("gkm_index_hist");
     if (hist == nullptr)
         return 0;  // the caller sums the partials with gkm_index_reduce
-    return gkm_index_reduce(partial, pack16 ? 2 : 4, xlen, B, Wx, l, nm, S, chunk_windows, hist,
-                            stream);
+    return gkm_index_reduce(partial, pack == 4 ? 1 : pack16 ? 2 : 4, xlen, B, Wx, l, nm, S, tile,
+                            chunk_windows, hist, stream);
 }
 
 GKM_API int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B,
-                             int Wx, int l, int nm, int S, int chunk_windows, void *hist,
+                             int Wx, int l, int nm, int S, int tile, int chunk_windows, void *hist,
                              void *stream)
 {
-    if (count_bytes != 2 && count_bytes != 4)
-        return fail(-1, "gkm_index_reduce: count_bytes must be 2 or 4, got %d", count_bytes);
-    if (chunk_windows < 1)
-        return fail(-1, "gkm_index_reduce: chunk_windows must be >= 1, got %d", chunk_windows);
+    if (count_bytes != 1 && count_bytes != 2 && count_bytes != 4)
+        return fail(-1, "gkm_index_reduce: count_bytes must be 1, 2 or 4, got %d", count_bytes);
+    if (chunk_windows < 1 || tile < 1)
+        return fail(-1, "gkm_index_reduce: chunk_windows and tile must be >= 1 (%d, %d)",
+                    chunk_windows, tile);
     if (B <= 0 || Wx <= 0 || S <= 0 || nm <= 0)
         return 0;
     const int nchunks = (Wx + chunk_windows - 1) / chunk_windows;
     const int64_t n = (int64_t)B * S * nm;
+    if (count_bytes == 1) {
+        const int64_t blocks = ((n + 3) / 4 + 255) / 256;
+        hipLaunchKernelGGL(k_index_reduce8, dim3((unsigned)blocks), dim3(256), 0, S_(stream),
+                           (const uint8_t *)partial, n, (int64_t)S * nm, nm, tile,
+                           (S + tile - 1) / tile, B, xlen, l, chunk_windows, nchunks,
+                           (unsigned long long *)hist);
+        GKM_CHECK_LAUNCH("gkm_index_reduce");
+        return 0;
+    }
     if (n % 4 == 0 && (int64_t)S * nm >= 4) {
         const int64_t blocks = (n / 4 + 255) / 256;
         if (count_bytes == 2)

@@ -40,7 +40,7 @@ SIGNATURES = {
                           P],
     "gkm_index_hist": [P, P, P, I, I, P, P, P, P, P, P, I, I, I, P, I, I, I, I, I, I64, I64,
                        P, P, P],
-    "gkm_index_reduce": [P, I, P, I, I, I, I, I, I, P, P],
+    "gkm_index_reduce": [P, I, P, I, I, I, I, I, I, I, P, P],
     "gkm_ism": [P, P, P, I, I, I, P, P, P, I, I, I, I, P, I, I, P, P, P, P, P, I, I, D, P,
                 P],
     "gkm_lmer_weights": [P, P, P, P, I, P, P],

@@ -504,8 +504,13 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     elif idx.pack16:  # nor the 16-bit ones: a window meets an SV window at most once
         cw = max(1, min(cw, 65535 // max(1, idx.max_sv_windows)))
     nchunks = -(-Wx // cw)
-    partial = torch.empty((nchunks, B, S, kp.nm), device=kp.device,
-                          dtype=torch.int16 if idx.pack16 else torch.int32)
+    if idx.pack == 4:  # bytes: low, high (fallback blocks), block flags
+        n_all = nchunks * B * S * kp.nm
+        partial = torch.empty(2 * n_all + nchunks * B * idx.n_tiles, device=kp.device,
+                              dtype=torch.uint8)
+    else:
+        partial = torch.empty((nchunks, B, S, kp.nm), device=kp.device,
+                              dtype=torch.int16 if idx.pack16 else torch.int32)
     wt, _ = kp.wtab(xs.lmax)
     # each row's windows in l-mer order (PAD last): a block's windows then share
     # high digits, and so do their neighbours' offsets -- cache lines reused
@@ -523,8 +528,8 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
                idx.n_masks, kp.l,
                kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, {1: 0, 2: 1, 4: 2}[idx.pack], col_min, x0,
                _p(partial), None, stream_handle(stream)))
-    _lib.call("gkm_index_reduce", _p(partial), partial.element_size(), _p(xs.len[x0:x1]), B, Wx,
-              kp.l, kp.nm, S, cw, _p(out), stream_handle(stream))
+    _lib.call("gkm_index_reduce", _p(partial), {1: 4, 2: 2, 4: 1}[idx.pack], _p(xs.len[x0:x1]),
+              B, Wx, kp.l, kp.nm, S, idx.tile, cw, _p(out), stream_handle(stream))
     return out
 
 

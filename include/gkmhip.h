@@ -187,6 +187,10 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * 16-bit counts, two per LDS word; 2 = 8-bit counts, four per LDS word, with
  * an exact fallback -- a block whose byte sums show a carry walks its chunk
  * again with 16-bit counts, tile * ceil(nm/4) / ceil(nm/2) SVs per pass.
+ * With 2 the partials are bytes: [nchunks][B][S][nm] low bytes, the same
+ * again for the high bytes (written by fallback blocks only), then
+ * [nchunks][B][n_tiles] flags (1: that block wrote high bytes) -- 2 * counts
+ * + nchunks * B * n_tiles bytes in all.
  * l <= 13, B <= 65535, tile * ceil(nm / p) * 4 bytes <= 160 KiB (p = 1, 2, 4
  * counts per word).  hist NULL: only the partials (see gkm_index_reduce).
  * LmerIndex.pairwise (kernels/lmer_index.py:239-268, 400-441;
@@ -198,14 +202,15 @@ int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xle
                    int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
                    void *partial, void *hist, void *stream);
 
-/* hist [B][S][nm] uint64 = the sum over window chunks of gkm_index_hist's
- * partial counts (count_bytes 2: uint16, 4: uint32), chunks past a row's last
- * window skipped.  gkm_index_hist runs it itself unless its hist is NULL;
- * called on its own it lets the caller time the index query and its sum apart.
- * The second half of LmerIndex.pairwise (kernels/lmer_index.py:239-268), which
- * accumulates its counts in place. */
+/* hist [B][S][nm] uint64 += the sum over window chunks of gkm_index_hist's
+ * partial counts, chunks past a row's last window skipped.  count_bytes 4:
+ * uint32, 2: uint16, 1: the 8-bit kernel's bytes (low bytes, high bytes, block
+ * flags; S SVs in tiles of `tile`).  gkm_index_hist runs it itself unless its
+ * hist is NULL; called on its own it lets the caller time the index query and
+ * its sum apart.  The second half of LmerIndex.pairwise
+ * (kernels/lmer_index.py:239-268), which accumulates its counts in place. */
 int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B, int Wx,
-                     int l, int nm, int S, int chunk_windows, void *hist, void *stream);
+                     int l, int nm, int S, int tile, int chunk_windows, void *hist, void *stream);
 
 /* ---- Exact l-mer tables of non-RBF models (gkm_tables.hip) ----------------
  * V, table  [4^l] float64 indexed by the packed l-mer (base k in bits 2k).
