@@ -100,7 +100,8 @@ class BatchScorer:
         cands = []
         if self.kp.l <= E.INDEX_MAX_L and self.kp.dmax >= 0:
             nt = self.sv.index().n_tiles
-            cands += [_Plan(True, target_blocks=t) for t in (512 * nt, 1024 * nt, 16384)]
+            cands += [_Plan(True, target_blocks=E.index_blocks([Lb] * self.rows, self.kp.l, nt, c))
+                      for c in (16, 32, 64)]
         norient = 2 if self.kp.rc else 1
         cands += [_Plan(False, shape=E.launch_shape(self.rows, self.sv.S, norient))]
         cands += [_Plan(False, shape=s) for s in ((1, 1), (2, 1), (1, 2), (4, 1))]

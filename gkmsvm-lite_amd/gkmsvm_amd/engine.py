@@ -449,7 +449,7 @@ def index_tile(nm: int, pack: int | bool = 1) -> int:
     return min(65536, max(256, (36864 // words) // 256 * 256))
 
 
-MAX_PARTIAL_BYTES = 2 << 30
+MAX_PARTIAL_BYTES = 8 << 30  # per-chunk index counts of one launch (rows per launch)
 LREC_IDS = 28                # ids held in a 64-byte l-mer record
 LREC_MAX_TILES = 4           # tiles an l-mer record covers
 RECORD_IDS = 12              # ids held in a bucket record
@@ -468,8 +468,8 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     _lib.call("gkm_zero_u64", _p(out), out.numel(), stream_handle(stream))
     fwd, _ = xs.windows(kp.l, stream)
     Wx = fwd.shape[1]
-    # rows per launch: the per-chunk partials of the smallest chunk (16 windows,
-    # or the 16-bit cap) must fit the budget
+    # rows per launch: the per-chunk partials of the smallest chunk (16
+    # windows, or the 16-bit cap) must fit the budget
     min_cw = 16 if not idx.pack16 else max(1, min(16, 65535 // max(1, idx.max_sv_windows)))
     per_row = -(-Wx // min_cw) * S * kp.nm * (2 if idx.pack16 else 4)
     step = max(1, min(B, MAX_PARTIAL_BYTES // max(1, per_row), 65535))
@@ -480,13 +480,30 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     return out
 
 
-def index_blocks(x_lengths, l: int, n_tiles: int = 1) -> int:
-    """Default block target of an index launch (measured on 4-105 row batches
-    against 20k and 72k SVs): rows of very different lengths -> the finest
-    chunks (16 windows), which balance the CUs; rows of one length -> fewer,
-    fuller blocks, 512 per SV tile."""
+INDEX_CHUNK_UNIFORM = 32  # query windows per block, rows of one length
+INDEX_CHUNK_RAGGED = 16   # rows of very different lengths
+INDEX_CHUNK_MIN = 16      # the finest chunk: one window per wave of a block
+INDEX_MIN_BLOCKS = 128    # finer chunks below this many blocks (256 CUs)
+
+
+def index_blocks(x_lengths, l: int, n_tiles: int = 1, chunk: int | None = None) -> int:
+    """Block target of an index launch that gives chunks of about `chunk`
+    query windows -- by default (measured on 1-105 row batches of 50-2000 bp
+    against 20k and 72k SVs) 32 for rows of one length, 16 when the launch
+    would have fewer than INDEX_MIN_BLOCKS blocks (a single 1 kb query: 0.57
+    -> 0.35 ms), and 16 for rows of very different lengths, which balances
+    the CUs.  Finer chunks would leave waves of a block idle."""
     wx = np.clip(np.asarray(x_lengths, dtype=np.int64) - l + 1, 0, None)
-    return 16384 if wx.max() > 1.5 * max(1.0, wx.mean()) else 512 * max(1, n_tiles)
+    per = max(1, n_tiles) * len(wx)
+    w = max(1, int(wx.max()))
+    if chunk is None:
+        if w > 1.5 * max(1.0, wx.mean()):
+            chunk = INDEX_CHUNK_RAGGED
+        else:
+            chunk = INDEX_CHUNK_UNIFORM
+            while chunk > INDEX_CHUNK_MIN and per * -(-w // chunk) < INDEX_MIN_BLOCKS:
+                chunk //= 2
+    return per * max(1, -(-w // chunk))
 
 
 def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
@@ -496,7 +513,7 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     fwd, _ = xs.windows(kp.l, stream)
     Wx = fwd.shape[1]
     wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
-    nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)), -(-wx_max // 16)))
+    nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)), -(-wx_max // INDEX_CHUNK_MIN)))
     cw = -(-wx_max // nchunks)
     if idx.vals is not None:  # per-block uint32 counts must not overflow
         cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
