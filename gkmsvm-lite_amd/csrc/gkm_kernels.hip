@@ -951,8 +951,9 @@ struct IndexArgs {
 // the first ids and weights) put both in one 32-byte load, and the n_tiles
 // records of an l-mer share a line, which the tile blocks of a chunk fetch
 // together (same XCD, see below).  One block fills a CU's LDS, so a CU runs 16
-// waves; each wave loads the next group's records (or bounds) before walking
-// the current group's buckets.  When no bucket of the group holds more than
+// waves; each wave keeps the records (or bounds) of its next two groups in
+// flight while it walks the current group's buckets (two register buffers,
+// the loop unrolled by two, so a buffer is read two steps after its load).  When no bucket of the group holds more than
 // INDEX_WALK_MAX_C entries each lane walks its own bucket, the ids past the
 // record INDEX_WALK per round with the round's loads issued together.  Larger
 // buckets (repeats) are walked wave-balanced: the wave's buckets laid end to
@@ -997,10 +998,13 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     // bucket is [e0, e1).
     auto walk = [&](auto count) {
         const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
-        int ni = i0 + wave, ng = 0;  // the pair being fetched
-        uint4 nr[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count) in nr[.][0]
-        int nlv[INDEX_PROBES];
-        auto fetch = [&]() {
+        // two (window, mask group) pairs' records in flight: the pair walked
+        // now was fetched two steps ago, into the buffer (A or B) it is read from
+        int ci = i0 + wave, cg = 0;  // the pair being walked
+        int ni = ci, ng = 0;         // the pair being fetched
+        uint4 nra[INDEX_PROBES][2], nrc[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count)
+        int nlva[INDEX_PROBES], nlvc[INDEX_PROBES];
+        auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
             const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
@@ -1021,32 +1025,37 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                     }
                 }
             }
+            if (++ng == n_groups) {
+                ng = 0;
+                ni += INDEX_THREADS / WAVE;
+            }
         };
-        fetch();
-        while (ni < i1) {
-            const int i = ni;
+        fetch(nra, nlva);
+        fetch(nrc, nlvc);
+        auto step = [&](uint4 (&nrb)[INDEX_PROBES][2], int (&nlvb)[INDEX_PROBES]) {
+            const int i = ci;
             uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
             int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
             bool small = true;
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
-                e0[p] = nr[p][0].x;
-                c[p] = (int)nr[p][0].y;
-                lv[p] = nlv[p];
+                e0[p] = nrb[p][0].x;
+                c[p] = (int)nrb[p][0].y;
+                lv[p] = nlvb[p];
                 n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
                 small = small && c[p] <= INDEX_WALK_MAX_C;
-                rid[p][0] = nr[p][0].z;
-                rid[p][1] = nr[p][0].w;
-                rid[p][2] = nr[p][1].x;
-                rid[p][3] = nr[p][1].y;
-                rid[p][4] = nr[p][1].z;
-                rid[p][5] = nr[p][1].w;
+                rid[p][0] = nrb[p][0].z;
+                rid[p][1] = nrb[p][0].w;
+                rid[p][2] = nrb[p][1].x;
+                rid[p][3] = nrb[p][1].y;
+                rid[p][4] = nrb[p][1].z;
+                rid[p][5] = nrb[p][1].w;
             }
-            if (++ng == n_groups) {
-                ng = 0;
-                ni += INDEX_THREADS / WAVE;
+            fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
+            if (++cg == n_groups) {
+                cg = 0;
+                ci += INDEX_THREADS / WAVE;
             }
-            fetch();
             const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
             const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
             if (__ballot(!small) == 0) {
@@ -1086,7 +1095,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                             if (kb + k >= n_in[p] && kb + k < c[p])
                                 count(sid[p][k], lv[p], PACK > 1 ? 1u : aw * wv[p][k]);
                 }
-                continue;
+                return;
             }
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
@@ -1132,6 +1141,12 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                             count(sid[u], plv[u], PACK > 1 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
                 }
             }
+        };
+        while (ci < i1) {
+            step(nra, nlva);
+            if (ci >= i1)
+                break;
+            step(nrc, nlvc);
         }
     };
     // 16-bit counts [(nm + 1) / 2][n] of n SVs from SV s0 of the tile out as
