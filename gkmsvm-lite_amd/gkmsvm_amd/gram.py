@@ -10,6 +10,9 @@ from . import engine as E
 from .inputs import as_batch, full_lengths
 
 
+GRAM_DEVICE_BYTES = 32 << 30  # a Gram up to this size is assembled in HBM
+
+
 def _gram_index(kp, x_lengths, ys, force: bool = False):
     """An l-mer index of the columns when it is the cheaper path (the cost
     model of SVModel.use_index_for, the columns as SVs) or forced, else None
@@ -79,7 +82,13 @@ def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = F
     dy = dx if symmetric else (E.self_kraw(kp, ys) if kp.normalize
                                else torch.ones(M, dtype=torch.float64, device=dev))
     step = E.rows_per_chunk(M, kp.nm)
-    sorted_g = np.empty((N, M), dtype=np.float64)
+    # the whole sorted Gram in HBM when it fits (mirrored and put back in input
+    # order on the device, then one copy to the host); else row blocks to host
+    on_device = N * M * 8 <= GRAM_DEVICE_BYTES
+    if on_device:
+        g_dev = torch.empty((N, M), dtype=torch.float64, device=dev)
+    else:
+        sorted_g = np.empty((N, M), dtype=np.float64)
     tile = None
     rows = range(0, N, step)
     if verbose:
@@ -92,13 +101,29 @@ def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = F
             hist = E.index_hist(kp, index, xs, r0, r1, M, col_min=0 if symmetric else -1)
         else:
             hist = E.cross_hist(kp, xs, r0, r1, ys, groups, col_min=0 if symmetric else -1)
-        if tile is None or tile.shape[0] != r1 - r0:
+        if on_device:
+            tile = g_dev[r0:r1]
+        elif tile is None or tile.shape[0] != r1 - r0:
             tile = torch.empty((r1 - r0, M), dtype=torch.float64, device=dev)
         _lib.call("gkm_combine_gram", E._p(hist), r1 - r0, M, kp.nm, E._p(kp.tab),
                   E._p(dx[r0:r1]), E._p(dy), int(kp.normalize), int(kp.rbf), kp.gamma,
                   E._p(tile), st)
-        sorted_g[r0:r1] = tile.cpu().numpy()
+        if not on_device:
+            sorted_g[r0:r1] = tile.cpu().numpy()
         del hist
+    if on_device:
+        if symmetric:  # upper triangle (with the diagonal) mirrored: exactly symmetric
+            g_dev = torch.triu(g_dev)
+            g_dev += torch.triu(g_dev, 1).T
+        if not (np.array_equal(ox, np.arange(N)) and np.array_equal(oy, np.arange(M))):
+            inv_x = torch.from_numpy(np.argsort(ox)).to(dev)
+            inv_y = torch.from_numpy(np.argsort(oy)).to(dev)
+            g_dev = g_dev.index_select(0, inv_x).index_select(1, inv_y)
+        if gram.flags.c_contiguous and gram.dtype == np.float64:
+            torch.from_numpy(gram).copy_(g_dev)
+        else:
+            gram[...] = g_dev.cpu().numpy()
+        return gram
     if symmetric:
         upper = np.triu(sorted_g)
         sorted_g = upper + np.triu(upper, 1).T

@@ -319,7 +319,8 @@ def test_score_variants_kernel(gpu):
 def test_gram_matches_oracle(gpu, kt):
     """Symmetric Gram (upper triangle computed, mirrored, exactly symmetric) and
     a rectangular one, ragged lengths, through the pairwise kernel and the
-    l-mer index, against the oracle."""
+    l-mer index, against the oracle; assembled in HBM or on the host, the same
+    bits."""
     from gkmsvm_amd import compute_gram
     from gkmsvm_amd.kernels import build_kernel
 
@@ -331,12 +332,23 @@ def test_gram_matches_oracle(gpu, kt):
     yc, yl = _rand_codes(rng, 70, 30, 90)
     want = O.gram(okern, xc, xl)
     want2 = O.gram(okern, xc, xl, yc, yl)
+    from gkmsvm_amd import gram as G
+
     for use_index in (False, True):
         g = compute_gram(kern, xc, lengths=xl, use_index=use_index)
         assert (g == g.T).all()
         H.assert_close(g, want, 1e-10, f"symmetric, index={use_index}")
         g2 = compute_gram(kern, xc, yc, lengths=xl, y_lengths=yl, use_index=use_index)
         H.assert_close(g2, want2, 1e-10, f"rectangular, index={use_index}")
+        # the Gram assembled in HBM (mirror, input order) equals the host assembly
+        cap = G.GRAM_DEVICE_BYTES
+        G.GRAM_DEVICE_BYTES = 0
+        try:
+            np.testing.assert_array_equal(compute_gram(kern, xc, lengths=xl, use_index=use_index), g)
+            np.testing.assert_array_equal(
+                compute_gram(kern, xc, yc, lengths=xl, y_lengths=yl, use_index=use_index), g2)
+        finally:
+            G.GRAM_DEVICE_BYTES = cap
 
 
 def test_plain_scoring_config_against_index_oracle(gpu):
