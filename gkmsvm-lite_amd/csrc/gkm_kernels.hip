@@ -1828,7 +1828,8 @@ static cross_fn pick_cross_w(bool weighted)
 template <int LT>
 static cross_fn pick_cross_n(int nm, bool weighted)
 {
-    return nm <= 4 ? pick_cross_w<LT, 4>(weighted) : pick_cross_w<LT, 16>(weighted);
+    return nm <= 4 ? pick_cross_w<LT, 4>(weighted)
+         : nm <= 8 ? pick_cross_w<LT, 8>(weighted) : pick_cross_w<LT, 16>(weighted);
 }
 static cross_fn pick_cross(int l, int nm, bool weighted)
 {
@@ -1848,7 +1849,8 @@ static self_fn pick_self_w(bool weighted)
 template <int LT>
 static self_fn pick_self_n(int nm, bool weighted)
 {
-    return nm <= 4 ? pick_self_w<LT, 4>(weighted) : pick_self_w<LT, 16>(weighted);
+    return nm <= 4 ? pick_self_w<LT, 4>(weighted)
+         : nm <= 8 ? pick_self_w<LT, 8>(weighted) : pick_self_w<LT, 16>(weighted);
 }
 static self_fn pick_self(int l, int nm, bool weighted)
 {
@@ -1872,7 +1874,8 @@ static sweep_fn pick_sweep_m(int mode)
 template <int LT>
 static sweep_fn pick_sweep_n(int nm, int mode)
 {
-    return nm <= 4 ? pick_sweep_m<LT, 4>(mode) : pick_sweep_m<LT, 16>(mode);
+    return nm <= 4 ? pick_sweep_m<LT, 4>(mode)
+         : nm <= 8 ? pick_sweep_m<LT, 8>(mode) : pick_sweep_m<LT, 16>(mode);
 }
 static sweep_fn pick_sweep(int l, int nm, int mode)
 {
