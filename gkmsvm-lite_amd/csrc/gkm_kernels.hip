@@ -1874,7 +1874,8 @@ static sweep_fn pick_sweep_m(int mode)
 template <int LT>
 static sweep_fn pick_sweep_n(int nm, int mode)
 {
-    return nm <= 4 ? pick_sweep_m<LT, 4>(mode)
+    // 5 levels: ISM at the default d = 3 (a mutation moves a pair by one level)
+    return nm <= 4 ? pick_sweep_m<LT, 4>(mode) : nm == 5 ? pick_sweep_m<LT, 5>(mode)
          : nm <= 8 ? pick_sweep_m<LT, 8>(mode) : pick_sweep_m<LT, 16>(mode);
 }
 static sweep_fn pick_sweep(int l, int nm, int mode)
