@@ -236,6 +236,30 @@ def roofline_of(summary: dict) -> dict:
     return out
 
 
+def ism_only(args):
+    """ISM of the -t 5 RBF configuration (50,000 SVs x 500 bp, gamma 2, Laplace
+    coefficients; 1 kb queries, 3,000 mutants each): one query, --launches
+    times after a warm-up, for rocprofv3 profiles of the sweep kernel."""
+    import torch
+
+    from gkmsvm_amd import GkmSVM, ism
+
+    rng = np.random.default_rng(8)
+    sv4 = _gc_seqs(rng, 50_000, 500, 0.5)
+    m4 = GkmSVM(sv4, rng.laplace(0, 0.5, 50_000), 0.0, 5,
+                {**PARAMS, "gamma": 2.0, "M": 50, "H": 50.0})
+    m4.rocm()
+    q = ["".join("ACGT"[c] for c in rng.integers(0, 4, 1000))]
+    ism(m4, q).cpu()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.launches):
+        ism(m4, q).cpu()
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "ism", "launches": args.launches,
+                      "ms_per_query": (time.perf_counter() - t) / args.launches * 1e3}), flush=True)
+
+
 def kernel_only(args):
     """Run one hot kernel on the serving shape (16 x 1 kb queries against the
     72,000 resident SVs) --launches times: for rocprofv3 profiles."""
@@ -253,6 +277,8 @@ def kernel_only(args):
         return
     if args.mode == "peaks":
         return peaks(args)
+    if args.mode == "ism":
+        return ism_only(args)
     model, _, _ = make_model()
     sv = model._svs()
     kp = model._params()
@@ -461,10 +487,11 @@ def main():
     ap.add_argument("--no-tables", action="store_true",
                     help="skip the l-mer table runs (profiles of the per-SV kernels)")
     ap.add_argument("--mode", default="serve",
-                    choices=["serve", "cross", "index", "table", "copy", "peaks", "configs"],
+                    choices=["serve", "cross", "index", "table", "copy", "peaks", "ism",
+                             "configs"],
                     help="serve: the benchmark; cross/index/table/copy: one kernel family on "
-                         "the serving shape, for profiling; "
-                         "configs: bounded timings of the other configurations")
+                         "the serving shape, for profiling; ism: the -t 5 ISM sample, for "
+                         "profiling; configs: bounded timings of the other configurations")
     ap.add_argument("--c3-queries", type=int, default=4096)
     ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)
