@@ -964,7 +964,7 @@ struct IndexArgs {
 // congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
 // chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk, and chunks past
 // the row's last window, exit at once.
-template <int PACK, int RK>
+template <int PACK, int RK, bool W16 = false>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                     for (int k = 0; k < record_ids<RK>(); ++k)
                         if (k < c[p])
                             count((int)rec_id(rid[p], k), lv[p],
-                                  PACK > 1 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
+                                  PACK > 1 && !W16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
                 int kmin = n_in[0];
 #pragma unroll
                 for (int p = 1; p < INDEX_PROBES; ++p)
@@ -1085,7 +1085,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                             const bool in = kb + k >= n_in[p] && kb + k < c[p];
                             const uint32_t e = e0[p] + (uint32_t)(kb + k);
                             sid[p][k] = in ? (int)a.ids[e] : 0;
-                            if (PACK == 1)
+                            if (PACK == 1 || W16)
                                 wv[p][k] = in && a.vals ? a.vals[e] : 1u;
                         }
 #pragma unroll
@@ -1093,7 +1093,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
                         for (int k = 0; k < INDEX_WALK; ++k)
                             if (kb + k >= n_in[p] && kb + k < c[p])
-                                count(sid[p][k], lv[p], PACK > 1 ? 1u : aw * wv[p][k]);
+                                count(sid[p][k], lv[p], PACK > 1 && !W16 ? 1u : aw * wv[p][k]);
                 }
                 return;
             }
@@ -1138,7 +1138,8 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
                     for (int u = 0; u < INDEX_PASSES; ++u)
                         if (sid[u] >= 0)
-                            count(sid[u], plv[u], PACK > 1 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
+                            count(sid[u], plv[u],
+                                  PACK > 1 && !W16 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
                 }
             }
         };
@@ -1168,7 +1169,64 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             }
         }
     };
-    if constexpr (PACK == 4) {
+    if constexpr (W16) {
+        // weighted, two 16-bit counts per word: an add (at most 255^2 < 2^16)
+        // that carries out of its field lowers the sum of the block's fields
+        // below the weight added (by 65,535, or 65,536 out of the word), so the
+        // block compares the two (64-bit sums) and, where they differ, walks
+        // its chunk again with 32-bit counts, sub SVs per pass
+        unsigned long long adds = 0;
+        walk([&](int sid, int lv, uint32_t add) {
+            if (sid < lo_col)
+                return;
+            adds += add;
+            atomicAdd(&cnt[(lv >> 1) * ns + sid], add << (16 * (lv & 1)));
+        });
+        __syncthreads();
+        unsigned long long d = adds;
+        for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS) {
+            const uint32_t w = cnt[i];
+            d -= (unsigned long long)((w & 0xFFFFu) + (w >> 16));
+        }
+        __shared__ unsigned long long red64[INDEX_THREADS / WAVE];
+        for (int o = WAVE / 2; o > 0; o >>= 1)
+            d += __shfl_xor(d, o, WAVE);
+        if (lane == 0)
+            red64[wave] = d;
+        __syncthreads();
+        unsigned long long carried = 0;
+#pragma unroll
+        for (int k = 0; k < INDEX_THREADS / WAVE; ++k)
+            carried += red64[k];
+        uint32_t *dst = (uint32_t *)a.partial + off;
+        if (carried == 0) {
+            for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
+                const int sid = i / a.nm, lv = i - sid * a.nm;
+                dst[i] = (cnt[(lv >> 1) * ns + sid] >> (16 * (lv & 1))) & 0xFFFFu;
+            }
+            return;
+        }
+        const int sub = (int)((int64_t)a.tile * ((a.nm + 1) / 2) / a.nm);  // LDS words / nm
+        for (int s0 = 0; s0 < ns; s0 += sub) {
+            const int n = min(sub, ns - s0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS)
+                cnt[i] = 0u;
+            __syncthreads();
+            walk([&](int sid, int lv, uint32_t add) {
+                const int r = sid - s0;
+                if (sid < lo_col || (unsigned)r >= (unsigned)n)
+                    return;
+                atomicAdd(&cnt[lv * n + r], add);
+            });
+            __syncthreads();
+            for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS) {
+                const int sid = i / a.nm, lv = i - sid * a.nm;
+                dst[(int64_t)s0 * a.nm + i] = cnt[lv * n + sid];
+            }
+        }
+        return;
+    } else if constexpr (PACK == 4) {
         uint32_t adds = 0;
         walk([&](int sid, int lv, uint32_t) {
             if (sid < lo_col)
@@ -1709,9 +1767,11 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
         return 0;
     if (pack_counts < 0 || pack_counts > 2)
         return fail(-1, "gkm_index_hist: pack_counts must be 0, 1 or 2, got %d", pack_counts);
-    // counts per LDS word: 1 (uint32, always with weights), 2 (16-bit), 4 (8-bit)
-    const int pack = vals != nullptr ? 1 : pack_counts == 2 ? 4 : pack_counts == 1 ? 2 : 1;
-    const bool pack16 = pack > 1;
+    // counts per LDS word: 1 (uint32), 2 (16-bit), 4 (8-bit); weighted
+    // indexes take 16-bit counts (with the carry check) when pack_counts is 1
+    const bool w16 = vals != nullptr && pack_counts == 1;
+    const int pack = vals != nullptr ? (w16 ? 2 : 1) : pack_counts == 2 ? 4 : pack_counts == 1 ? 2 : 1;
+    const bool pack16 = pack > 1 && !w16;  // uint16 partials
     const size_t words = (size_t)tile * ((nm + pack - 1) / pack);
     size_t lds = words * sizeof(uint32_t);
     if (lds > 160 * 1024)
@@ -1743,7 +1803,8 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     const int rk = records == nullptr ? 0 : vals ? 2 : 1;  // 32-byte records per key
     a.records = records;
     typedef void (*tile_fn)(IndexArgs);
-    const tile_fn kfn = pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
+    const tile_fn kfn = w16       ? (rk ? k_index_tile<2, 2, true> : k_index_tile<2, 0, true>)
+                      : pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
                       : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
                       : rk == 2   ? k_index_tile<1, 2>
                       : rk == 1   ? k_index_tile<1, 1> : k_index_tile<1, 0>;

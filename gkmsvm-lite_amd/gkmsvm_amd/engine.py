@@ -342,7 +342,12 @@ class DeviceIndex:
         nb = 2 if kp.rc else 1
         sv_windows = int(np.clip(seqs.lengths - l + 1, 0, None).max()) * nb if S else 0
         self.pack16 = not kp.weighted and 0 < sv_windows < 65536 and count_bits != 32
-        self.pack = index_pack(kp.nm, self.pack16) if count_bits != 16 else 2 if self.pack16 else 1
+        if kp.weighted:  # 16-bit weighted counts (carry-checked, 32-bit fallback): uint32 partials
+            self.pack = 1 if count_bits == 32 else 2
+        elif not self.pack16:
+            self.pack = 1
+        else:
+            self.pack = 2 if count_bits == 16 else index_pack(kp.nm, True)
         self.tile = tile or min(index_tile(kp.nm, self.pack), max(256, -(-S // 256) * 256))
         self.n_tiles = (S + self.tile - 1) // self.tile
         fwd, rcw = seqs.windows(l, stream)
@@ -430,10 +435,11 @@ class DeviceIndex:
 
 
 def index_pack(nm: int, unweighted: bool) -> int:
-    """Counts per 32-bit LDS word of the index query: 1 (uint32: weighted), 2
-    (16-bit) or 4 (8-bit, where that takes fewer words than 16-bit: nm >= 3)."""
+    """Counts per 32-bit LDS word of the index query: 2 (16-bit; weighted
+    counts with the kernel's carry check and 32-bit fallback) or 4 (8-bit,
+    unweighted, where that takes fewer words than 16-bit: nm >= 3)."""
     if not unweighted:
-        return 1
+        return 2
     return 4 if (nm + 3) // 4 < (nm + 1) // 2 else 2
 
 
@@ -545,7 +551,8 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
                idx.n_masks, kp.l,
                kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, {1: 0, 2: 1, 4: 2}[idx.pack], col_min, x0,
                _p(partial), None, stream_handle(stream)))
-    _lib.call("gkm_index_reduce", _p(partial), {1: 4, 2: 2, 4: 1}[idx.pack], _p(xs.len[x0:x1]),
+    fmt = 1 if idx.pack == 4 else 2 if idx.pack16 else 4  # bytes, uint16, uint32 partials
+    _lib.call("gkm_index_reduce", _p(partial), fmt, _p(xs.len[x0:x1]),
               B, Wx, kp.l, kp.nm, S, idx.tile, cw, _p(out), stream_handle(stream))
     return out
 

@@ -182,8 +182,11 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * probe; masks / levels
  * [n_masks] the XOR neighbourhood of mismatch counts < nm with non-zero
  * weight.  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
- * counts, uint16 with pack_counts, else uint32.  pack_counts (unweighted
- * only; the caller keeps chunk_windows * windows per SV below 2^16): 1 =
+ * counts, uint16 with pack_counts, else uint32.  With vals (weighted),
+ * pack_counts 1 = two 16-bit counts per LDS word with the same kind of exact
+ * check (field sums against the weight added; a block that carried walks its
+ * chunk again with 32-bit counts), partials uint32 either way.  pack_counts
+ * (unweighted; the caller keeps chunk_windows * windows per SV below 2^16): 1 =
  * 16-bit counts, two per LDS word; 2 = 8-bit counts, four per LDS word, with
  * an exact fallback -- a block whose byte sums show a carry walks its chunk
  * again with 16-bit counts, tile * ceil(nm/4) / ceil(nm/2) SVs per pass.

@@ -415,7 +415,8 @@ def test_index_histograms_equal_brute_force(gpu, seed):
 def test_index_mixed_bucket_sizes(gpu):
     """Buckets of a few entries (walked by their lanes) and of thousands (poly-A
     and dinucleotide repeats, walked wave-balanced) in the same launch give the
-    brute-force integers, weighted and unweighted."""
+    brute-force integers, weighted (16-bit counts with the 32-bit fallback, and
+    32-bit counts) and unweighted."""
     from gkmsvm_amd import engine as E
 
     rng = np.random.default_rng(17)
@@ -438,6 +439,12 @@ def test_index_mixed_bucket_sizes(gpu):
         b = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+        if kt == 4:  # weighted: 16-bit counts (poly-A pairs carry: the 32-bit walk) and 32-bit
+            assert idx.pack == 2
+            w32 = E.DeviceIndex(s.kp, s.seqs, count_bits=32)
+            assert w32.pack == 1
+            c = E.index_hist(s.kp, w32, xs, 0, xs.n, s.S)
+            np.testing.assert_array_equal(c.cpu().numpy(), b.cpu().numpy())
 
 
 @pytest.mark.parametrize("d", [1, 2, 3, 4, 5])
