@@ -104,6 +104,43 @@ def test_cross_and_self_histograms_bit_exact(gpu, seed):
                                   O.self_hist(okern, O.pack_both(okern, yc, yl)))
 
 
+@pytest.mark.parametrize("d", [4, 5, 6, 7])
+def test_histograms_and_ism_5_to_8_levels(gpu, d):
+    """-t 0 with d = 4..7 (nm = 5..8 levels: the 5- and 8-level kernel
+    instances): pairwise and self histograms equal the oracle's counts, and
+    ISM (nm + 1 levels) equals rescoring every mutant."""
+    from gkmsvm_amd import engine as E
+    from gkmsvm_amd import ism
+    from gkmsvm_amd.kernels import build_kernel
+
+    rng = np.random.default_rng(60 + d)
+    p = {"L": 12, "k": 12 - d, "d": d}
+    kern = build_kernel(0, p)
+    okern = O.Kern.make(0, p)
+    xc, xl = _rand_codes(rng, 4, 30, 120)
+    yc, yl = _rand_codes(rng, 70, 20, 160)
+    kp = E.KernelParams(kern, gpu)
+    assert kp.nm == d + 1
+    xs = E.DeviceSeqs(xc, xl, gpu, rc=kp.rc)
+    ys = E.DeviceSeqs(yc, yl, gpu, rc=kp.rc)
+    groups = torch.from_numpy(E.group_bounds(yl)).to(gpu)
+    got = E.cross_hist(kp, xs, 0, xs.n, ys, groups).cpu().numpy().view(np.uint64)
+    np.testing.assert_array_equal(
+        got, O.pair_hist(okern, O.pack_both(okern, xc, xl), O.pack_both(okern, yc, yl)))
+    from gkmsvm_amd import _lib
+
+    hist = torch.zeros((ys.n, kp.nm), dtype=torch.int64, device=gpu)
+    _lib.call("gkm_self_hist", E._p(ys.planes), E._p(ys.planes_rc), E._p(ys.len), ys.n, ys.nw,
+              kp.l, kp.nm, None, 0, 3, E._p(hist), None)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hist.cpu().numpy().view(np.uint64),
+                                  O.self_hist(okern, O.pack_both(okern, yc, yl)))
+    m = _model(0, p, yc, yl, rng.normal(size=len(yl)), 0.1)
+    qc, ql = _rand_codes(rng, 2, 40, 60)
+    got = ism(m, qc, lengths=ql).cpu().numpy()
+    H.assert_close(got, O.ism(_oracle_model(m), qc, ql), 1e-9, f"ISM d={d}")
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_upper_triangle_columns(gpu, seed):
     """col_min restricts row b to columns >= b + row_base; the rest stay 0."""
