@@ -154,6 +154,30 @@ def test_launch_shapes_fit_grid_limits():
     assert g.tolist() == [[5, 100], [100, 100]]
 
 
+def test_index_launch_rules():
+    """The l-mer index query's counts per LDS word, tile sizes and window chunks:
+    8-bit where that saves words (nm >= 3), 16-bit otherwise and for weighted
+    counts, tiles filling 144 KiB; chunks of 32 windows for rows of one length
+    (16 under 128 blocks), 16 for rows of very different lengths."""
+    from gkmsvm_amd import engine as E
+
+    assert [E.index_pack(nm, True) for nm in (1, 2, 3, 4, 5, 8, 9)] == [2, 2, 4, 4, 4, 4, 4]
+    assert E.index_pack(4, False) == 2
+    assert E.index_tile(4, 4) == 36864 and E.index_tile(4, 2) == 18432
+    assert E.index_tile(4, 1) == 9216 and E.index_tile(4, True) == 36864
+    assert E.index_tile(4, False) == 18432 and E.index_tile(1, 4) == 36864
+    l = 11
+    # 16 x 1 kb against 2 tiles: chunks of 32 windows -> 31 per row
+    assert E.index_blocks(np.full(16, 1000), l, 2) == 2 * 16 * 31
+    # a single 1 kb query: under 128 blocks at 32 windows, so chunks of 16
+    assert E.index_blocks(np.full(1, 1000), l, 2) == 2 * 1 * 62
+    # ragged rows (max > 1.5 x mean): chunks of 16
+    lens = np.array([60, 2000, 100, 150])
+    assert E.index_blocks(lens, l, 2) == 2 * 4 * -(-(2000 - l + 1) // 16)
+    # an explicit chunk
+    assert E.index_blocks(np.full(8, 300), l, 3, 64) == 3 * 8 * -(-290 // 64)
+
+
 # -- the C-ABI library -------------------------------------------------------------
 
 
