@@ -91,7 +91,8 @@ class GkmServer:
     """Serve one model; see the module docstring."""
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
-                 streams: int = 2, max_explain: int = 64, tables: bool = True):
+                 streams: int = 2, max_explain: int = 64, tables: bool = True,
+                 history: int = 1 << 16):
         model.rocm()
         self.model = model
         self.sv = model._svs()
@@ -124,7 +125,9 @@ class GkmServer:
         self._stop = False
         self._thread = self._harvester = None
         self._launcher_done = False
-        self.completed: list = []
+        # the last `history` completed requests, for latency_stats (bounded so
+        # a long-running server does not hold every request it served)
+        self.completed: deque = deque(maxlen=int(history))
 
     def _build_explain(self, pert: bool):
         from .explain import explain_tables
@@ -258,8 +261,13 @@ class GkmServer:
                         return
                     self._lock.wait()
                 st = self._inflight[0]
-            st.done.synchronize()
-            self._deliver(st)
+            try:
+                st.done.synchronize()
+                self._deliver(st)
+            except Exception as exc:  # a device error or a delivery bug fails the
+                for r in st.reqs:     # step's requests, not the server
+                    if not r.future.done():
+                        r.future.set_exception(exc)
             with self._lock:
                 self._inflight.pop(0)
                 self._free.append(st.stream)

@@ -168,3 +168,30 @@ def test_server_bad_request_fails_alone(gpu, tables):
     assert np.isnan(got[-1])
     for g, w in zip(got_ex, want_ex):
         H.assert_close(g, w, 1e-10, "explain next to bad requests")
+
+
+def test_server_survives_a_failed_step(gpu):
+    """An error while a step's results are handed out fails that step's
+    requests; the server goes on serving, and its history of completed
+    requests stays within its bound."""
+    from gkmsvm_amd.serving import GkmServer
+
+    m = _random_model(S=300, seed=7)
+    m.use_lmer_table = m.use_explain_table = False
+    qs = _queries(4, 50, 300, 13)
+    want = m(qs).cpu().numpy().ravel()
+    with GkmServer(m, hbm_budget=1 << 30, tables=False, history=2) as srv:
+        deliver, calls = srv._deliver, []
+
+        def failing_once(st):
+            calls.append(1)
+            if len(calls) == 1:
+                raise RuntimeError("delivery failed")
+            deliver(st)
+
+        srv._deliver = failing_once
+        with pytest.raises(RuntimeError, match="delivery failed"):
+            srv.submit("score", qs[:1]).result(timeout=60)
+        got = np.array([srv.submit("score", [q]).result(timeout=60)[0] for q in qs])
+    np.testing.assert_array_equal(got, want)
+    assert len(srv.completed) == 2
