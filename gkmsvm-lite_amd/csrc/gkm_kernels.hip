@@ -1374,9 +1374,19 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
     double *res = (double *)smem;          // [4][span]
     double *stab = res;                    // set below, after res
     const int l = LT > 0 ? LT : a.l;
-    const int b = blockIdx.y;
+    // Block order: the XCDs take blocks round-robin by id (block p runs on XCD
+    // p % 8), so XCD x is given the consecutive (row, chunk) pairs
+    // k = x * per + p / 8 (row-major): its L2 then holds the coefficients of
+    // about B / 8 rows instead of every row's.  Ids past the last pair exit.
+    const int nx = (a.Wx + a.chunk_windows - 1) / a.chunk_windows;  // chunks per row
+    const int64_t n_pairs = (int64_t)nx * a.B;
+    const int64_t per = (n_pairs + 7) / 8;
+    const int64_t k = (int64_t)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (k >= n_pairs)
+        return;
+    const int b = (int)(k / nx);
     const int wx = a.xlen[b] - l + 1;
-    const int i0 = blockIdx.x * a.chunk_windows;
+    const int i0 = (int)(k - (int64_t)b * nx) * a.chunk_windows;
     const int i1 = min(wx, i0 + a.chunk_windows);
     const int span = a.chunk_windows + LM - 1;  // positions i0 .. i0 + span - 1
     stab = res + 4 * (size_t)span;
@@ -2323,7 +2333,12 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
     size_t lds = (4 * ((size_t)chunk_windows + lm - 1) + 32) * sizeof(double);
     if (lds > 64 * 1024)
         return fail(-2, "gkm_index_explain: chunk_windows %d too large", chunk_windows);
-    dim3 grid((unsigned)((Wx + chunk_windows - 1) / chunk_windows), (unsigned)B);
+    // one dimension, 8 ids per XCD slot (the kernel maps ids to (row, chunk))
+    const int64_t n_pairs = (int64_t)((Wx + chunk_windows - 1) / chunk_windows) * B;
+    if (n_pairs > ((int64_t)1 << 30))
+        return fail(-1, "gkm_index_explain: %lld blocks exceed the launch limit",
+                    (long long)n_pairs);
+    dim3 grid((unsigned)(8 * ((n_pairs + 7) / 8)));
     typedef void (*ixexp_fn)(IndexExplainArgs);
     const int rk = record_kind(records, vals, n_tiles);
 #define GKM_IXEXP(LT) (rk == 3 ? k_index_explain<LT, 3> : rk == 2 ? k_index_explain<LT, 2> \
