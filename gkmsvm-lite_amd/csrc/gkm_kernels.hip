@@ -920,9 +920,10 @@ struct IndexArgs {
     const uint32_t *offsets; // [4^l * n_tiles + 1], key = lmer * n_tiles + tile
     const uint16_t *ids;     // [entries] SV of each entry, relative to its tile
     const uint8_t *vals;     // [entries] window weight of each entry, or NULL
-    const uint32_t *records; // [keys][8] bucket records (RK 1 or 2), or NULL
-    const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood (sorted)
-    const uint8_t *levels;   // [n_masks] mismatch count of each mask
+    const uint32_t *records; // [record_tables][keys][8] bucket records (RK 1 or 2), or NULL
+    const uint32_t *masks;   // [n_masks] XOR masks of the neighbourhood
+    const uint8_t *levels;   // [n_masks] mismatch count of each mask | its record table << 4
+    int record_tables;       // tables in records (table q: digits 0 and q of the key swapped)
     const uint8_t *wtab;     // weight by distance, or NULL
     void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16)
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
@@ -1012,13 +1013,24 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
                 nlv[p] = 0;
                 if (w < nl && t < a.n_masks) {
-                    const uint32_t key = (w ^ a.masks[t]) * (uint32_t)a.n_tiles + (uint32_t)tile_id;
-                    nlv[p] = a.levels[t];
+                    const uint32_t v = w ^ a.masks[t];
+                    const uint32_t lvb = a.levels[t];  // level | record table << 4
+                    nlv[p] = (int)(lvb & 15u);
                     if (RK) {
-                        const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
+                        // record table q holds the records with digits 0 and q of
+                        // the l-mer swapped, so the two masks of a pair (differing
+                        // in bit 0 of digit q) read one 64-byte-aligned pair of
+                        // records, a 128-byte line, in one instruction
+                        const uint32_t q = min(lvb >> 4, (uint32_t)(a.record_tables - 1));
+                        const uint32_t x = (v ^ (v >> (2u * q))) & 3u;
+                        const uint32_t vq = v ^ x ^ (x << (2u * q));
+                        const size_t rk = (size_t)q * ((size_t)nl * (uint32_t)a.n_tiles) +
+                                          (size_t)vq * (uint32_t)a.n_tiles + (uint32_t)tile_id;
+                        const uint4 *r = (const uint4 *)(a.records + rk * 8);
                         nr[p][0] = r[0];
                         nr[p][1] = r[1];
                     } else {
+                        const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
                         const bounds2 r = *(const bounds2 *)(a.offsets + key);
                         nr[p][0].x = r.x;
                         nr[p][0].y = r.y - r.x;
@@ -1758,10 +1770,11 @@ __global__ __launch_bounds__(256) void k_index_reduce8(const uint8_t *__restrict
 
 GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                            const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                           const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
-                           int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
-                           int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
-                           void *partial, void *hist, void *stream)
+                           const uint32_t *records, int record_tables, const uint32_t *masks,
+                           const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                           int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
+                           int64_t col_min, int64_t row_base, void *partial, void *hist,
+                           void *stream)
 {
     int rc = check_l_nm(l, nm);
     if (rc)
@@ -1777,6 +1790,9 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
         return 0;
     if (pack_counts < 0 || pack_counts > 2)
         return fail(-1, "gkm_index_hist: pack_counts must be 0, 1 or 2, got %d", pack_counts);
+    if (records != nullptr && (record_tables < 1 || record_tables > l))
+        return fail(-1, "gkm_index_hist: record_tables must be 1..l (%d), got %d", l,
+                    record_tables);
     // counts per LDS word: 1 (uint32), 2 (16-bit), 4 (8-bit); weighted
     // indexes take 16-bit counts (with the carry check) when pack_counts is 1
     const bool w16 = vals != nullptr && pack_counts == 1;
@@ -1795,6 +1811,7 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.vals = vals;
     a.masks = masks;
     a.levels = levels;
+    a.record_tables = records != nullptr ? record_tables : 1;
     a.wtab = wtab;
     a.partial = partial;
     a.B = B;

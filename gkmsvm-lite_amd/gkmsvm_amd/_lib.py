@@ -38,7 +38,7 @@ SIGNATURES = {
     "gkm_explain": [I, P, P, I, I, I, P, P, P, I, I, I, I, I, P, I, I, P, P, P, P, P],
     "gkm_index_explain": [I, P, P, I, I, I, P, P, P, P, P, P, I, I, P, I, I, I, I, P, P, P, P,
                           P],
-    "gkm_index_hist": [P, P, P, I, I, P, P, P, P, P, P, I, I, I, P, I, I, I, I, I, I64, I64,
+    "gkm_index_hist": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, I64, I64,
                        P, P, P],
     "gkm_index_reduce": [P, I, P, I, I, I, I, I, I, I, P, P],
     "gkm_ism": [P, P, P, I, I, I, P, P, P, I, I, I, I, P, I, I, P, P, P, P, P, I, I, D, P,

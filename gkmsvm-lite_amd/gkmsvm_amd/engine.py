@@ -311,6 +311,40 @@ def neighbor_masks(l: int, table, dmax: int):
     return masks, levels
 
 
+def paired_masks(masks: np.ndarray, levels: np.ndarray, tables: int):
+    """The index query's order of the neighbourhood masks: pairs of masks that
+    differ in bit 0 of one digit q < tables, each pair at lanes 2k, 2k + 1 of a
+    wave (so both probes read one 128-byte line of record table q, which swaps
+    the l-mer's digits 0 and q), then the unpaired masks; levels[t] | q << 4.
+    A greedy matching, masks with the fewest free partners first (l = 11, d = 3:
+    2,432 pairs of the 4,984 masks)."""
+    n = len(masks)
+    where = {int(m): i for i, m in enumerate(masks)}
+    adj = [[(where[int(m) ^ (1 << (2 * q))], q) for q in range(tables)
+            if int(m) ^ (1 << (2 * q)) in where] for m in masks]
+    mate = [-1] * n
+    table = [0] * n
+    for i in sorted(range(n), key=lambda i: (len(adj[i]), int(masks[i]))):
+        if mate[i] >= 0:
+            continue
+        best = None
+        for j, q in adj[i]:
+            if mate[j] < 0:
+                free = sum(1 for k, _ in adj[j] if mate[k] < 0)
+                if best is None or (free, q) < best[0]:
+                    best = ((free, q), j, q)
+        if best is not None:
+            j, q = best[1], best[2]
+            mate[i], mate[j] = j, i
+            table[i] = table[j] = q
+    pairs = sorted({(min(i, mate[i]), max(i, mate[i])) for i in range(n) if mate[i] >= 0},
+                   key=lambda pr: (table[pr[0]], int(masks[pr[0]])))
+    sel = np.array([k for pr in pairs for k in pr] + [i for i in range(n) if mate[i] < 0],
+                   dtype=np.int64)
+    lv = (levels.astype(np.uint8) | (np.array(table, dtype=np.uint8) << 4))[sel]
+    return masks[sel], lv
+
+
 # Throughput of the two scoring paths on one MI355X (measured, l=11 d=3, 300-1000
 # bp rows, 500-72,000 SVs), for choosing between them: window-pair comparisons
 # per second of gkm_cross_hist; (window, mask, SV tile) probes and index
@@ -417,7 +451,27 @@ class DeviceIndex:
             self.lmer_records = r32
             del rec, lmer, rank, sel, cnt
         del ks
+        # record tables 1..q: the bucket records again with the l-mer's digits 0
+        # and q swapped, so a pair of masks differing in bit 0 of digit q reads
+        # one 128-byte line (with at most 2 tiles an l-mer pair's records of a
+        # tile share one); tables beyond RECORD_TABLES_MAX_BYTES are not built
+        self.record_tables = 1
+        if self.records is not None and self.n_tiles <= 2:
+            tables = int(min(l, max(1, RECORD_TABLES_MAX_BYTES // (nk * 32))))
+            if tables > 1:
+                base = self.records.view(nl, self.n_tiles * 8)
+                allr = torch.empty((tables, nl, self.n_tiles * 8), dtype=torch.int32, device=dev)
+                allr[0] = base
+                lm = torch.arange(nl, device=dev, dtype=torch.int64)
+                for q in range(1, tables):
+                    x = (lm ^ (lm >> (2 * q))) & 3
+                    allr[q] = base[lm ^ x ^ (x << (2 * q))]
+                del lm, x, base
+                self.records = allr.view(tables * nk, 8)  # table 0 first: key order
+                self.record_tables = tables
         masks, levels = neighbor_masks(l, kp.h, kp.dmax)
+        if self.record_tables > 1:
+            masks, levels = paired_masks(masks, levels, self.record_tables)
         self.masks = torch.from_numpy(masks.view(np.int32)).to(dev)
         self.levels = torch.from_numpy(levels).to(dev)
         self.n_masks = len(masks)
@@ -461,6 +515,7 @@ LREC_MAX_TILES = 4           # tiles an l-mer record covers
 RECORD_IDS = 12              # ids held in a bucket record
 RECORD_IDS_W = 8             # (id, weight) pairs held in a record of a weighted index
 RECORD_MAX_BYTES = 4 << 30   # larger record tables (l >= 13 with many tiles) are not built
+RECORD_TABLES_MAX_BYTES = 4 << 30  # all the digit-swapped record tables together
 
 
 def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: int, S: int, *,
@@ -547,7 +602,8 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
            lambda: _lib.call(
                "gkm_index_hist", _p(swin), _p(spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
-               _p(idx.ids), _p(idx.vals), _p(idx.records), _p(idx.masks), _p(idx.levels),
+               _p(idx.ids), _p(idx.vals), _p(idx.records), idx.record_tables, _p(idx.masks),
+               _p(idx.levels),
                idx.n_masks, kp.l,
                kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, {1: 0, 2: 1, 4: 2}[idx.pack], col_min, x0,
                _p(partial), None, stream_handle(stream)))

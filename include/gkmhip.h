@@ -176,12 +176,16 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * index is a CSR over keys lmer * n_tiles + tile (offsets [4^l n_tiles + 1]
  * uint32; ids uint16 SV within its tile of `tile` SVs; vals uint8 window
  * weights or NULL) of every SV window's l-mer, forward and RC; records: NULL
- * or [4^l n_tiles][8] uint32 bucket records, one per key: start, count, the
- * bucket's first 12 ids (uint16), or with vals its first 8 ids and their uint8
- * weights -- one 32-byte load per
- * probe; masks / levels
- * [n_masks] the XOR neighbourhood of mismatch counts < nm with non-zero
- * weight.  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
+ * or [record_tables][4^l n_tiles][8] uint32 bucket records, one per key:
+ * start, count, the bucket's first 12 ids (uint16), or with vals its first 8
+ * ids and their uint8 weights -- one 32-byte load per probe; table q (1 <=
+ * record_tables <= l) holds them at the key of the l-mer with its digits 0
+ * and q swapped (table 0: in key order).  masks / levels [n_masks]: the XOR
+ * neighbourhood of mismatch counts < nm with non-zero weight; levels[t] =
+ * mismatch count | (record table of mask t) << 4, the table clamped to
+ * record_tables - 1 (any table gives the same counts; a table q shared by
+ * masks that differ in bit 0 of digit q puts their two records in one
+ * 128-byte line).  partial: caller scratch of ceil(Wx / chunk_windows) * B * S * nm
  * counts, uint16 with pack_counts, else uint32.  With vals (weighted),
  * pack_counts 1 = two 16-bit counts per LDS word with the same kind of exact
  * check (field sums against the weight added; a block that carried walks its
@@ -200,10 +204,10 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * kernels/lmer_index_cpu.py:23-67). */
 int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
                    const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                   const uint32_t *records, const uint32_t *masks, const uint8_t *levels,
-                   int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
-                   int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
-                   void *partial, void *hist, void *stream);
+                   const uint32_t *records, int record_tables, const uint32_t *masks,
+                   const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                   int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
+                   int64_t col_min, int64_t row_base, void *partial, void *hist, void *stream);
 
 /* hist [B][S][nm] uint64 += the sum over window chunks of gkm_index_hist's
  * partial counts, chunks past a row's last window skipped.  count_bytes 4:

@@ -178,6 +178,35 @@ def test_index_launch_rules():
     assert E.index_blocks(np.full(8, 300), l, 3, 64) == 3 * 8 * -(-290 // 64)
 
 
+def test_paired_masks():
+    """The index query's mask order: a permutation of the neighbourhood (levels
+    kept) whose leading entries pair up at lanes 2k, 2k + 1 with masks that
+    differ in bit 0 of one digit q < tables -- q in each level byte's high
+    nibble, the digit whose record table puts both probes in one 128-byte
+    line -- and the unpaired masks last."""
+    from gkmsvm_amd import engine as E
+    from gkmsvm_amd.kernels import build_kernel
+
+    kp_h = build_kernel(2, {"L": 11, "k": 7, "d": 3})._mismatch_table
+    masks, levels = E.neighbor_masks(11, kp_h, 3)
+    assert len(masks) == 4984
+    want = dict(zip(masks.tolist(), levels.tolist()))
+    for tables, n_pairs in ((11, 2432), (1, 872)):
+        pm, pl = E.paired_masks(masks, levels, tables)
+        assert sorted(pm.tolist()) == sorted(masks.tolist())
+        assert all(want[m] == (v & 15) for m, v in zip(pm.tolist(), pl.tolist()))
+        q = pl >> 4
+        assert q.max() < tables
+        k = 0
+        while k + 1 < len(pm) and (int(pm[k]) ^ int(pm[k + 1])) == 1 << (2 * int(q[k])):
+            assert q[k] == q[k + 1]
+            k += 2
+        assert k == 2 * n_pairs
+        rest = pm[k:].tolist()  # no two unpaired masks could have been paired
+        for a in rest:
+            assert not any(a ^ (1 << (2 * d)) in set(rest) for d in range(tables))
+
+
 # -- the C-ABI library -------------------------------------------------------------
 
 
