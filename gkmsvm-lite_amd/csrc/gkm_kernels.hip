@@ -1005,16 +1005,41 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         int ni = ci, ng = 0;         // the pair being fetched
         uint4 nra[INDEX_PROBES][2], nrc[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count)
         int nlva[INDEX_PROBES], nlvc[INDEX_PROBES];
+        // Loads complete in order, so a wait for a load also waits for every
+        // older one: what a fetch needs at once -- its masks and levels, its
+        // window's l-mer -- is loaded a fetch (a window) ahead, before the
+        // records then issued, and waiting for it leaves those in flight.
+        uint32_t pm[INDEX_PROBES], pl[INDEX_PROBES];  // the next fetch's masks, levels
+        auto load_masks = [&](int g) {
+#pragma unroll
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                const int t = (g * INDEX_PROBES + p) * WAVE + lane;
+                pm[p] = t < a.n_masks ? a.masks[t] : 0u;
+                pl[p] = t < a.n_masks ? a.levels[t] : 0u;
+            }
+        };
+        auto load_window = [&](int i) -> uint32_t {  // PAD past the chunk: no probes
+            return i < i1 ? a.xwin[(int64_t)b * a.Wx + i] : nl;
+        };
+        uint32_t fw = load_window(ni), fw_next = load_window(ni + INDEX_THREADS / WAVE);
+        load_masks(0);
         auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
-            const uint32_t w = ni < i1 ? a.xwin[(int64_t)b * a.Wx + ni] : nl;  // PAD: no probes
+            const uint32_t w = fw;
+            uint32_t mk[INDEX_PROBES], lb[INDEX_PROBES];
+#pragma unroll
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                mk[p] = pm[p];
+                lb[p] = pl[p];
+            }
+            load_masks(ng + 1 == n_groups ? 0 : ng + 1);
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
                 const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
                 nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
                 nlv[p] = 0;
                 if (w < nl && t < a.n_masks) {
-                    const uint32_t v = w ^ a.masks[t];
-                    const uint32_t lvb = a.levels[t];  // level | record table << 4
+                    const uint32_t v = w ^ mk[p];
+                    const uint32_t lvb = lb[p];  // level | record table << 4
                     nlv[p] = (int)(lvb & 15u);
                     if (RK) {
                         // record table q holds the records with digits 0 and q of
@@ -1040,6 +1065,8 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             if (++ng == n_groups) {
                 ng = 0;
                 ni += INDEX_THREADS / WAVE;
+                fw = fw_next;
+                fw_next = load_window(ni + INDEX_THREADS / WAVE);
             }
         };
         fetch(nra, nlva);
@@ -1063,6 +1090,24 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 rid[p][4] = nrb[p][1].z;
                 rid[p][5] = nrb[p][1].w;
             }
+            // the lane walk's first round of ids past the records, issued before
+            // the fetch below, so waiting for it leaves that fetch in flight
+            int kmin = n_in[0];
+#pragma unroll
+            for (int p = 1; p < INDEX_PROBES; ++p)
+                kmin = min(kmin, n_in[p]);
+            int sid0[INDEX_PROBES][INDEX_WALK];
+            uint32_t wv0[INDEX_PROBES][INDEX_WALK];
+#pragma unroll
+            for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                for (int k = 0; k < INDEX_WALK; ++k) {
+                    const bool in = small && kmin + k >= n_in[p] && kmin + k < c[p];
+                    const uint32_t e = e0[p] + (uint32_t)(kmin + k);
+                    sid0[p][k] = in ? (int)a.ids[e] : 0;
+                    if (PACK == 1 || W16)
+                        wv0[p][k] = in && a.vals ? a.vals[e] : 1u;
+                }
             fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
             if (++cg == n_groups) {
                 cg = 0;
@@ -1083,10 +1128,6 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                         if (k < c[p])
                             count((int)rec_id(rid[p], k), lv[p],
                                   PACK > 1 && !W16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
-                int kmin = n_in[0];
-#pragma unroll
-                for (int p = 1; p < INDEX_PROBES; ++p)
-                    kmin = min(kmin, n_in[p]);
                 for (int kb = kmin; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
                     int sid[INDEX_PROBES][INDEX_WALK];
                     uint32_t wv[INDEX_PROBES][INDEX_WALK];
@@ -1096,9 +1137,15 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                         for (int k = 0; k < INDEX_WALK; ++k) {
                             const bool in = kb + k >= n_in[p] && kb + k < c[p];
                             const uint32_t e = e0[p] + (uint32_t)(kb + k);
-                            sid[p][k] = in ? (int)a.ids[e] : 0;
-                            if (PACK == 1 || W16)
-                                wv[p][k] = in && a.vals ? a.vals[e] : 1u;
+                            if (kb == kmin) {  // loaded before the fetch
+                                sid[p][k] = sid0[p][k];
+                                if (PACK == 1 || W16)
+                                    wv[p][k] = wv0[p][k];
+                            } else {
+                                sid[p][k] = in ? (int)a.ids[e] : 0;
+                                if (PACK == 1 || W16)
+                                    wv[p][k] = in && a.vals ? a.vals[e] : 1u;
+                            }
                         }
 #pragma unroll
                     for (int p = 0; p < INDEX_PROBES; ++p)
