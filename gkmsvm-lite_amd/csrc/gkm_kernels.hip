@@ -1010,21 +1010,23 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         // window's l-mer -- is loaded a fetch (a window) ahead, before the
         // records then issued, and waiting for it leaves those in flight.
         uint32_t pm[INDEX_PROBES], pl[INDEX_PROBES];  // the next fetch's masks, levels
+        // (loads without branches around them -- past the end they read the last
+        // entry, which no probe uses -- so the compiler can count them)
         auto load_masks = [&](int g) {
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
-                const int t = (g * INDEX_PROBES + p) * WAVE + lane;
-                pm[p] = t < a.n_masks ? a.masks[t] : 0u;
-                pl[p] = t < a.n_masks ? a.levels[t] : 0u;
+                const int t = min((g * INDEX_PROBES + p) * WAVE + lane, a.n_masks - 1);
+                pm[p] = a.masks[t];
+                pl[p] = a.levels[t];
             }
         };
-        auto load_window = [&](int i) -> uint32_t {  // PAD past the chunk: no probes
-            return i < i1 ? a.xwin[(int64_t)b * a.Wx + i] : nl;
+        auto load_window = [&](int i) -> uint32_t {  // windows past the chunk: no probes
+            return a.xwin[(int64_t)b * a.Wx + min(i, a.Wx - 1)];
         };
         uint32_t fw = load_window(ni), fw_next = load_window(ni + INDEX_THREADS / WAVE);
         load_masks(0);
         auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
-            const uint32_t w = fw;
+            const uint32_t w = ni < i1 ? fw : nl;  // PAD past the chunk
             uint32_t mk[INDEX_PROBES], lb[INDEX_PROBES];
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
@@ -1035,12 +1037,15 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
                 const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
-                nr[p][0] = nr[p][1] = make_uint4(0u, 0u, 0u, 0u);
-                nlv[p] = 0;
-                if (w < nl && t < a.n_masks) {
-                    const uint32_t v = w ^ mk[p];
+                // every lane loads (a lane without a probe reads key 0 and is
+                // marked by level -1): no branch around the loads, so the
+                // compiler can count them and wait for this fetch two steps on
+                // without draining the fetches after it
+                const bool ok = w < nl && t < a.n_masks;
+                {
+                    const uint32_t v = ok ? w ^ mk[p] : 0u;
                     const uint32_t lvb = lb[p];  // level | record table << 4
-                    nlv[p] = (int)(lvb & 15u);
+                    nlv[p] = ok ? (int)(lvb & 15u) : -1;
                     if (RK) {
                         // record table q holds the records with digits 0 and q of
                         // the l-mer swapped, so the two masks of a pair (differing
@@ -1054,11 +1059,11 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                         const uint4 *r = (const uint4 *)(a.records + rk * 8);
                         nr[p][0] = r[0];
                         nr[p][1] = r[1];
-                    } else {
+                    } else {  // the bucket's bounds, its count taken when it is walked
                         const uint32_t key = v * (uint32_t)a.n_tiles + (uint32_t)tile_id;
                         const bounds2 r = *(const bounds2 *)(a.offsets + key);
                         nr[p][0].x = r.x;
-                        nr[p][0].y = r.y - r.x;
+                        nr[p][0].y = r.y;
                     }
                 }
             }
@@ -1079,8 +1084,8 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
                 e0[p] = nrb[p][0].x;
-                c[p] = (int)nrb[p][0].y;
-                lv[p] = nlvb[p];
+                c[p] = nlvb[p] < 0 ? 0 : RK ? (int)nrb[p][0].y : (int)(nrb[p][0].y - nrb[p][0].x);
+                lv[p] = max(nlvb[p], 0);
                 n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
                 small = small && c[p] <= INDEX_WALK_MAX_C;
                 rid[p][0] = nrb[p][0].z;
@@ -1102,11 +1107,13 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             for (int p = 0; p < INDEX_PROBES; ++p)
 #pragma unroll
                 for (int k = 0; k < INDEX_WALK; ++k) {
+                    // unconditional too (entry 0 where there is none; the index
+                    // holds at least one)
                     const bool in = small && kmin + k >= n_in[p] && kmin + k < c[p];
-                    const uint32_t e = e0[p] + (uint32_t)(kmin + k);
-                    sid0[p][k] = in ? (int)a.ids[e] : 0;
+                    const uint32_t e = in ? e0[p] + (uint32_t)(kmin + k) : 0u;
+                    sid0[p][k] = (int)a.ids[e];
                     if (PACK == 1 || W16)
-                        wv0[p][k] = in && a.vals ? a.vals[e] : 1u;
+                        wv0[p][k] = a.vals ? a.vals[e] : 1u;
                 }
             fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
             if (++cg == n_groups) {

@@ -451,6 +451,11 @@ class DeviceIndex:
             self.lmer_records = r32
             del rec, lmer, rank, sel, cnt
         del ks
+        # at least one entry: the query reads entry 0 where a lane has none
+        if self.ids.numel() == 0:
+            self.ids = torch.zeros(1, dtype=torch.int16, device=dev)
+            if self.vals is not None:
+                self.vals = torch.zeros(1, dtype=self.vals.dtype, device=dev)
         # record tables 1..q: the bucket records again with the l-mer's digits 0
         # and q swapped, so a pair of masks differing in bit 0 of digit q reads
         # one 128-byte line (with at most 2 tiles an l-mer pair's records of a

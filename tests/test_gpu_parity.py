@@ -574,3 +574,23 @@ def test_peak_microbenchmarks_run(gpu):
     assert int(sink.abs().sum()) == 0
     with pytest.raises(_lib.GkmError):
         _lib.call("gkm_peak_read", src.data_ptr(), 15, sink.data_ptr(), 64, E.stream_handle())
+
+
+def test_index_without_entries(gpu):
+    """An index whose SVs are all shorter than l holds no entries (the query's
+    loads then read its one padding entry): the histograms are zero, equal to
+    the pairwise kernel's."""
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(29)
+    sv = rng.integers(0, 4, (40, 9)).astype(np.uint8)
+    m = _model(2, {"L": 11, "k": 7, "d": 3}, sv, np.full(40, 9), rng.normal(size=40))
+    s = m._svs()
+    idx = s.index()
+    assert idx.n_entries == 0 and idx.ids.numel() == 1
+    qc, ql = _rand_codes(rng, 3, 20, 300)
+    xs = E.DeviceSeqs(qc, ql, gpu, rc=s.kp.rc)
+    got = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S).cpu().numpy()
+    want = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups).cpu().numpy()
+    np.testing.assert_array_equal(got, want)
+    assert not got.any()
