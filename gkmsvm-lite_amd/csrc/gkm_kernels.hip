@@ -1024,6 +1024,15 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             return a.xwin[(int64_t)b * a.Wx + min(i, a.Wx - 1)];
         };
         uint32_t fw = load_window(ni), fw_next = load_window(ni + INDEX_THREADS / WAVE);
+        // the walked window's position in its row (window weights), a window ahead
+        auto load_pos = [&](int i) -> int {
+            return a.xpos[(int64_t)b * a.Wx + min(i, a.Wx - 1)];
+        };
+        int cpos = 0, cpos_next = 0;
+        if (a.xpos) {
+            cpos = load_pos(ci);
+            cpos_next = load_pos(ci + INDEX_THREADS / WAVE);
+        }
         load_masks(0);
         auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
             const uint32_t w = ni < i1 ? fw : nl;  // PAD past the chunk
@@ -1116,11 +1125,15 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                         wv0[p][k] = a.vals ? a.vals[e] : 1u;
                 }
             fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
+            const int pos = a.xpos ? cpos : i;  // loaded a window ahead
             if (++cg == n_groups) {
                 cg = 0;
                 ci += INDEX_THREADS / WAVE;
+                if (a.xpos) {
+                    cpos = cpos_next;
+                    cpos_next = load_pos(ci + INDEX_THREADS / WAVE);
+                }
             }
-            const int pos = a.xpos ? a.xpos[(int64_t)b * a.Wx + i] : i;
             const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
             if (__ballot(!small) == 0) {
                 // lane walk: every probe's next INDEX_WALK entries per round

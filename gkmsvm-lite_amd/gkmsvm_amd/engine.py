@@ -120,6 +120,10 @@ class DeviceSeqs:
         """(fwd, rc) packed l-mer windows [n, Lmax - l + 1] int32 (uint32 bits)."""
         if l not in self._windows:
             W = self.lmax - l + 1
+            if W < 1:  # no row holds an l-mer: one column of PAD (0xFFFFFFFF)
+                fwd = torch.full((self.n, 1), -1, dtype=torch.int32, device=self.device)
+                self._windows[l] = (fwd, fwd.clone())
+                return self._windows[l]
             fwd = torch.empty((self.n, W), dtype=torch.int32, device=self.device)
             rcw = torch.empty_like(fwd)
             _lib.call("gkm_pack_windows", _p(self.codes), _p(self.len), self.n, self.lmax, l,
