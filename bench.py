@@ -201,8 +201,8 @@ HBM_KINDS = {
 # copies of known size) in rocprofv3 --pmc runs of this benchmark's headline
 # serve run, as recorded in profiles/serving_72kSV_mix_trace.txt.
 PROFILED_TRAFFIC = {
-    "k_index_tile": (15_297_278 * 2 + 1_369_615) * 1024.0,
-    "k_index_explain": (2_409_872 * 2 + 701) * 1024.0,
+    "k_index_tile": (15_538_564 * 2 + 1_369_626) * 1024.0,
+    "k_index_explain": (2_409_884 * 2 + 701) * 1024.0,
 }
 TRAFFIC_SOURCE = "profiles/serving_72kSV_mix_trace.txt (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
 
