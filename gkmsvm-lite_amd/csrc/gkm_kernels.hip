@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         // now was fetched two steps ago, into the buffer (A or B) it is read from
         int ci = i0 + wave, cg = 0;  // the pair being walked
         int ni = ci, ng = 0;         // the pair being fetched
-        uint4 nra[INDEX_PROBES][2], nrc[INDEX_PROBES][2];  // record (RK 1, 2), or (start, count)
+        uint4 nra[INDEX_PROBES][2], nrc[INDEX_PROBES][2];  // record (RK 1, 2), or (start, end)
         int nlva[INDEX_PROBES], nlvc[INDEX_PROBES];
         // Loads complete in order, so a wait for a load also waits for every
         // older one: what a fetch needs at once -- its masks and levels, its
