@@ -608,7 +608,7 @@ def main():
 
     roofline = roofline_of(summary)
     line = {
-        "metric": "query x SV kernel evals/sec (served), with p50/p99 request latency and the plain path",
+        "metric": "query\u00d7SV kernel evals/sec + p50/p99 request latency, served vs plain (1 GPU)",
         "value": served_rate,
         "unit": "query x SV kernel evals/s",
         "n_gpus": 1,
