@@ -520,22 +520,37 @@ def main():
     steps = [make_requests(args.requests, 100 + i) for i in range(n_steps)]
     rng = np.random.default_rng(5)
 
-    def served_run(server, timed_steps):
+    def served_run(server, timed_steps, events_in_timed=True):
+        """(seconds over the timed steps, kernel summary, latency at saturation,
+        per-step wall ms).  events_in_timed: the kernels' HIP events are
+        recorded during the timed steps; else (the host-bound table path,
+        where they would add ~3 % to a step) during a second pass over the
+        same steps after them."""
         for i in range(args.warmup):
             run_served(server, steps[i], args.rate, rng)
         server.completed.clear()
         timer = E.KernelTimer()
-        E.TIMER = timer
+        E.TIMER = timer if events_in_timed else None
         torch.cuda.synchronize()
+        per_step = []
         t0 = time.perf_counter()
         for s_ in timed_steps:
-            run_served(server, s_, args.rate, rng)
+            per_step.append(run_served(server, s_, args.rate, rng))
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         E.TIMER = None
         lat_sat = latency_stats(server.completed)
         server.completed.clear()
-        return el, timer.summary(), lat_sat
+        if not events_in_timed:
+            E.TIMER = timer
+            for s_ in timed_steps:
+                run_served(server, s_, args.rate, rng)
+            torch.cuda.synchronize()
+            E.TIMER = None
+            server.completed.clear()
+        ms = np.array(per_step) * 1e3
+        step_ms = {"min": float(ms.min()), "p50": float(np.median(ms)), "max": float(ms.max())}
+        return el, timer.summary(), lat_sat, step_ms
 
     # 1. The headline: the stream served through the per-SV kernels (pairwise
     # kernel / l-mer index for scoring, the explain sweep), tables off -- the
@@ -548,7 +563,7 @@ def main():
     ktimed = steps[args.warmup:args.warmup + args.steps]
     k_evals = sum(evals(s_) for s_ in ktimed)
     server_k = GkmServer(model_k, hbm_budget=16 << 30, tables=False).start()
-    served_s, summary, lat_sat = served_run(server_k, ktimed)
+    served_s, summary, lat_sat, step_ms = served_run(server_k, ktimed)
     served_rate = k_evals / served_s
     # latency at a nominal load: arrivals at a fraction of the measured capacity
     cap = args.requests * len(ktimed) / served_s
@@ -570,7 +585,8 @@ def main():
         server.start()
         ttimed = steps[args.warmup:args.warmup + args.table_steps]
         t_evals = sum(evals(s_) for s_ in ttimed)
-        served_t_s, summary_t, lat_t_sat = served_run(server, ttimed)
+        served_t_s, summary_t, lat_t_sat, step_ms_t = served_run(server, ttimed,
+                                                                 events_in_timed=False)
         cap_t = args.requests * len(ttimed) / served_t_s
         server.completed.clear()
         for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
@@ -578,7 +594,7 @@ def main():
         lat_t = latency_stats(server.completed)
         server.stop()
         served_t = {"value": t_evals / served_t_s, "ms_per_step": served_t_s / len(ttimed) * 1e3,
-                    "steps": len(ttimed), "table_build_s": build_s,
+                    "steps": len(ttimed), "step_wall_ms": step_ms_t, "table_build_s": build_s,
                     "latency_ms_at_saturation": lat_t_sat,
                     "latency_load_rate_per_s": args.load * cap_t, "latency_ms": lat_t,
                     "roofline": roofline_of(summary_t)}
@@ -626,6 +642,7 @@ def main():
                         "and gkmexplain (1 kb); served through the per-SV kernels",
             "requests_per_step": args.requests,
             "arrival_rate_per_s": args.rate,
+            "step_wall_ms": step_ms,
             "latency_ms_at_saturation": lat_sat,
             "latency_load_rate_per_s": load_rate,
             "latency_ms": lat,

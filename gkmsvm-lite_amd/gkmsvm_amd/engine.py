@@ -75,12 +75,16 @@ class DeviceSeqs:
         self._pack(rc, stream)
 
     @classmethod
-    def from_strings(cls, strings, device, *, rc: bool = True, stream=None, min_width: int = 1):
+    def from_strings(cls, strings, device, *, rc: bool = True, stream=None, min_width: int = 1,
+                     staging: torch.Tensor | None = None):
         """(DeviceSeqs of DNA strings, bad [n] int32 on the device): the
         strings' bytes, unpadded, go over in one pinned copy and
         gkm_decode_ascii writes the padded codes (width max(Lmax, min_width); N
         as A) and flags rows with a letter other than ACGTN (bad = 1; empty
-        rows are the caller's to flag, from .lengths)."""
+        rows are the caller's to flag, from .lengths).  staging: a pinned
+        uint8 host buffer to stage the bytes in when it is large enough (the
+        caller keeps it untouched until the copy on `stream` completes);
+        otherwise a new pinned buffer is allocated."""
         self = cls.__new__(cls)
         n = len(strings)
         lens = np.fromiter(map(len, strings), dtype=np.int64, count=n)
@@ -91,7 +95,10 @@ class DeviceSeqs:
         self.lengths = lens
         self.device = device
         nr = -(-len(raw) // 8) * 8
-        self._host = torch.empty(nr + 12 * n, dtype=torch.uint8, pin_memory=True)
+        if staging is not None and staging.numel() >= nr + 12 * n:
+            self._host = staging[: nr + 12 * n]
+        else:
+            self._host = torch.empty(nr + 12 * n, dtype=torch.uint8, pin_memory=True)
         hb = self._host.numpy()
         hb[: len(raw)] = np.frombuffer(raw, dtype=np.uint8)
         starts = hb[nr: nr + 8 * n].view(np.int64)
@@ -158,6 +165,7 @@ class KernelParams:
         self.weighted = base.weight_table(1) is not None
         self._wtab = None
         self._wtab_n = 0
+        self._wtab_old = []
 
     def wtab(self, max_len: int):
         """(uint8 weight-by-distance table on the device covering sequences up to
@@ -168,6 +176,10 @@ class KernelParams:
         if self._wtab is None or self._wtab_n < need:
             n = max(need, 1024)
             t = self.base.weight_table(2 * (n - 1))
+            if self._wtab is not None:
+                # a kernel queued on another stream may still read the old
+                # table: it is kept (a few KB) rather than freed
+                self._wtab_old.append(self._wtab)
             self._wtab = torch.from_numpy(np.ascontiguousarray(t)).to(self.device)
             self._wtab_n = len(t)
         return self._wtab, self._wtab_n

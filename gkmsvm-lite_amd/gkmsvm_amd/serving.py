@@ -2,7 +2,8 @@
 
 The model's SV tables, coefficients and self-kernels stay in HBM.  Clients
 submit scoring, variant (score_variants, kernel method) and GkmExplain
-requests from any thread and get a Future.  A launcher thread admits pending
+requests from any thread and get a handle (result(), done(), as a Future's).
+A launcher thread admits pending
 requests into GPU steps while earlier steps run: each step packs every
 admitted scoring/variant row into one batch (rows of any length; their bytes
 go over in one pinned copy and are decoded on the device, where rows with a
@@ -25,10 +26,11 @@ rows l lookups per position, independent of the number of SVs.
 
 from __future__ import annotations
 
+import gc
 import threading
 import time
 from collections import deque
-from concurrent.futures import Future
+from concurrent.futures import TimeoutError as FutureTimeout
 from dataclasses import dataclass
 
 import numpy as np
@@ -38,24 +40,51 @@ from . import engine as E
 from . import tables as T
 from .codec import encode_batch
 from . import _lib
-from .inputs import short_rows  # noqa: F401  (documented NaN semantics)
 
 _BAD_SEQ = "Invalid sequence (empty, or a letter other than ACGTN)"
 
+# gc.freeze() nesting across running servers (the heap alive when the first
+# one starts stays out of the collector's scans until the last one stops)
+_GC_LOCK = threading.Lock()
+_GC_FROZEN = 0
+
+
+def _gc_freeze():
+    global _GC_FROZEN
+    with _GC_LOCK:
+        if _GC_FROZEN == 0:
+            gc.collect()
+            gc.freeze()
+        _GC_FROZEN += 1
+
+
+def _gc_unfreeze():
+    global _GC_FROZEN
+    with _GC_LOCK:
+        _GC_FROZEN -= 1
+        if _GC_FROZEN == 0:
+            gc.unfreeze()
+
 
 class Request:
-    """One client request: kind "score" | "variant" | "explain"; seqs [str] or,
-    for variants, [(ref, alt)]; its Future, timestamps, and the rows, longest
-    sequence and HBM bytes it takes (computed once)."""
+    """One client request -- kind "score" | "variant" | "explain"; seqs [str]
+    or, for variants, [(ref, alt)]; timestamps; the rows, longest sequence and
+    HBM bytes it takes (computed once) -- and the handle submit() returns:
+    result(timeout), exception(timeout), done() and add_done_callback(fn) as
+    on a concurrent.futures.Future.  A request owns no lock: every waiter
+    sleeps on its server's delivery condition, which the server notifies once
+    per step after setting all of the step's results."""
 
-    __slots__ = ("kind", "seqs", "mode", "future", "t_submit", "t_admit", "t_done",
-                 "n_rows", "lmax", "nbytes")
+    __slots__ = ("kind", "seqs", "mode", "t_submit", "t_admit", "t_done", "n_rows", "lmax",
+                 "nbytes", "_cond", "_done", "_value", "_error", "_callbacks")
 
-    def __init__(self, kind: str, seqs: list, mode: str = "hypothetical"):
+    def __init__(self, kind: str, seqs: list, mode: str = "hypothetical", cond=None):
         self.kind = kind
         self.seqs = seqs
         self.mode = mode
-        self.future = Future()
+        self._cond = threading.Condition() if cond is None else cond
+        self._done = False
+        self._value = self._error = self._callbacks = None
         self.t_submit = self.t_admit = self.t_done = 0.0
         if kind == "variant":
             self.n_rows = 2 * len(seqs)
@@ -70,6 +99,39 @@ class Request:
 
     def max_len(self) -> int:
         return self.lmax
+
+    # -- the result handle ---------------------------------------------------------
+
+    def done(self) -> bool:
+        return self._done
+
+    def _wait(self, timeout):
+        if not self._done:
+            with self._cond:
+                if not self._cond.wait_for(lambda: self._done, timeout):
+                    raise FutureTimeout(f"no result within {timeout} s")
+
+    def result(self, timeout: float | None = None):
+        """The request's result (scores [n] float64, variant effects [n], or
+        attributions [n, 4, L]); raises the request's error."""
+        self._wait(timeout)
+        if self._error is not None:
+            raise self._error
+        return self._value
+
+    def exception(self, timeout: float | None = None):
+        self._wait(timeout)
+        return self._error
+
+    def add_done_callback(self, fn) -> None:
+        """fn(request) once the result is set (at once when it already is)."""
+        with self._cond:
+            if not self._done:
+                if self._callbacks is None:
+                    self._callbacks = []
+                self._callbacks.append(fn)
+                return
+        fn(self)
 
 
 @dataclass
@@ -92,7 +154,18 @@ class GkmServer:
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
                  streams: int = 2, max_explain: int = 64, tables: bool = True,
-                 history: int = 1 << 16):
+                 history: int = 1 << 16, stage_len: int = 2048, freeze_gc: bool = True):
+        """hbm_budget: device bytes a step may hold; max_rows / max_explain:
+        scoring+variant rows and explain requests per step; streams: steps in
+        flight; tables: build the exact l-mer tables at start where the model
+        allows them; history: completed requests kept for latency_stats;
+        stage_len: pinned host buffers are sized at start for steps of rows up
+        to this many bases (longer rows grow them); freeze_gc: while the
+        server runs, the objects alive when it starts are moved out of the
+        cyclic collector's generations (gc.freeze, process-wide), so a full
+        collection scans only what was allocated since -- without it, a
+        collection over the interpreter's whole heap (tens of ms with torch
+        loaded) lands inside some step."""
         model.rocm()
         self.model = model
         self.sv = model._svs()
@@ -116,18 +189,60 @@ class GkmServer:
         self.max_explain = int(max_explain)
         self._streams = [torch.cuda.Stream(device=self.kp.device) for _ in range(streams)]
         self._free = list(self._streams)
-        # pinned result buffers, one set per stream: a stream's step is
-        # delivered (its results copied out) before the stream takes a new step
+        # pinned staging and result buffers, one set per stream: a stream's
+        # step is delivered (its results copied out) before the stream takes a
+        # new step; sized here so no step allocates pinned memory
         self._pinned = {id(st): {} for st in self._streams}
+        self._presize(int(stage_len))
+        self._prepare(int(stage_len))
+        # the tables and caches above were written on the caller's current
+        # stream, which the server's streams do not wait on: finish them now
+        torch.cuda.current_stream(self.kp.device).synchronize()
+        self.freeze_gc = bool(freeze_gc)
+        self._froze = False
         self._pending: deque = deque()
         self._inflight: list = []
         self._lock = threading.Condition()
+        # what every request's waiters sleep on (notified once per step)
+        self._delivered = threading.Condition(threading.Lock())
         self._stop = False
         self._thread = self._harvester = None
         self._launcher_done = False
-        # the last `history` completed requests, for latency_stats (bounded so
-        # a long-running server does not hold every request it served)
+        # (kind, t_submit, t_admit, t_done) of the last `history` completed
+        # requests, for latency_stats: tuples of atoms, which the cyclic
+        # collector stops tracking, rather than the requests themselves
         self.completed: deque = deque(maxlen=int(history))
+
+    def _presize(self, stage_len: int):
+        """Allocate each stream's pinned buffers for a full step of rows up to
+        stage_len bases: staged string bytes, scores, explain outputs, flags."""
+        rows = self.max_rows + self.max_explain
+        for st in self._streams:
+            self._host_buffer(st, "stage", (rows * stage_len + 12 * rows + 8,), torch.uint8)
+            self._host_buffer(st, "out", (self.max_rows + self.max_explain * 4 * stage_len,))
+            self._host_buffer(st, "bad", (rows,), torch.int32)
+            if self.table is None:
+                self._host_buffer(st, "score", (self.max_rows,))
+            for i in range(2):
+                self._host_buffer(st, f"explain{i}", (self.max_explain * 4 * stage_len,))
+
+    def _prepare(self, stage_len: int):
+        """Build the device state that steps would otherwise build lazily on
+        a server stream, where another stream could read it half-written: the
+        window-weight table, the SVs' l-mer index and the explain
+        neighbourhoods the per-SV kernels use."""
+        from .explain import _explain_masks, explain_tables
+
+        kp, sv = self.kp, self.sv
+        kp.wtab(stage_len)
+        if self.table is not None or self.model.use_lmer_index is False:
+            return
+        if sv.index() is None:
+            return
+        for pert in (False, True):
+            ta, tb, cmax = explain_tables(kp.base, pert)
+            if cmax >= 0:
+                _explain_masks(sv, ta, tb, cmax, pert)
 
     def _build_explain(self, pert: bool):
         from .explain import explain_tables
@@ -141,27 +256,52 @@ class GkmServer:
 
     # -- client side ----------------------------------------------------------------
 
-    def submit(self, kind: str, seqs, mode: str = "hypothetical") -> Future:
+    def submit(self, kind: str, seqs, mode: str = "hypothetical") -> Request:
+        """Queue a request; returns its handle (see Request)."""
         if kind not in ("score", "variant", "explain"):
             raise ValueError(f"unknown request kind {kind!r}")
-        req = Request(kind, list(seqs), mode)
+        req = Request(kind, list(seqs), mode, self._delivered)
         req.t_submit = time.perf_counter()
         if req.n_rows == 0:  # nothing to compute: the empty result at once
-            req.t_admit = req.t_done = req.t_submit
-            req.future.set_result(np.empty((0, 4, 0) if kind == "explain" else 0))
-            return req.future
+            req.t_admit = req.t_submit
+            self._finish([(req, np.empty((0, 4, 0) if kind == "explain" else 0), None)])
+            return req
         need = req.nbytes = self._bytes(req)
         if need > self.hbm_budget:
-            req.future.set_exception(MemoryError(
-                f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))
-            return req.future
+            self._finish([(req, None, MemoryError(
+                f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))])
+            return req
         with self._lock:
             self._pending.append(req)
             self._lock.notify_all()
-        return req.future
+        return req
+
+    def _finish(self, items):
+        """Set results [(request, value, error)] (requests already done are
+        skipped), record the completed ones, wake every waiter once, then run
+        the requests' done-callbacks."""
+        now = time.perf_counter()
+        called = []
+        with self._delivered:
+            for req, value, error in items:
+                if req._done:
+                    continue
+                req._value, req._error, req.t_done = value, error, now
+                req._done = True
+                if error is None and req.n_rows:
+                    self.completed.append((req.kind, req.t_submit, req.t_admit, now))
+                if req._callbacks:
+                    called.append(req)
+            self._delivered.notify_all()
+        for req in called:
+            for fn in req._callbacks:
+                fn(req)
 
     def start(self):
         if self._thread is None:
+            if self.freeze_gc and not self._froze:
+                _gc_freeze()
+                self._froze = True
             self._stop = False
             self._launcher_done = False
             self._thread = threading.Thread(target=self._loop, name="gkm-server", daemon=True)
@@ -180,6 +320,9 @@ class GkmServer:
             self._thread.join()
             self._harvester.join()
             self._thread = self._harvester = None
+        if self._froze:
+            _gc_unfreeze()
+            self._froze = False
 
     def __enter__(self):
         return self.start()
@@ -241,9 +384,7 @@ class GkmServer:
                 with self._lock:
                     self._free.append(stream)
                     self._lock.notify_all()
-                for r in batch:
-                    if not r.future.done():
-                        r.future.set_exception(exc)
+                self._finish([(r, None, exc) for r in batch])
                 continue
             with self._lock:
                 self._inflight.append(st)
@@ -265,9 +406,7 @@ class GkmServer:
                 st.done.synchronize()
                 self._deliver(st)
             except Exception as exc:  # a device error or a delivery bug fails the
-                for r in st.reqs:     # step's requests, not the server
-                    if not r.future.done():
-                        r.future.set_exception(exc)
+                self._finish([(r, None, exc) for r in st.reqs])  # step's requests, not the server
             with self._lock:
                 self._inflight.pop(0)
                 self._free.append(st.stream)
@@ -285,8 +424,14 @@ class GkmServer:
             pool[key] = buf
         return buf[:n].view(shape)
 
+    def _staging(self, stream, strings) -> torch.Tensor:
+        """The stream's pinned staging buffer, large enough for the strings'
+        bytes, their int64 starts and int32 lengths."""
+        n = len(strings)
+        need = -(-sum(map(len, strings)) // 8) * 8 + 12 * n
+        return self._host_buffer(stream, "stage", (need,), torch.uint8)
+
     def _deliver(self, st):
-        now = time.perf_counter()
         # rows with a letter other than ACGTN (flagged on the device) or empty
         bad = None
         if st.bad_host is not None:
@@ -313,7 +458,7 @@ class GkmServer:
             else:
                 results[id(req)] = vals
         for req, host, r0, n, L, short, g0 in st.explain_out:
-            if req.future.done():  # rejected at launch
+            if req.done():  # rejected at launch
                 continue
             if g0 is not None and bad is not None and bad[g0:g0 + n].any():
                 results[id(req)] = ValueError(_BAD_SEQ)
@@ -322,19 +467,13 @@ class GkmServer:
             if short is not None and short[r0:r0 + n].any():
                 res[short[r0:r0 + n]] = np.nan
             results[id(req)] = res
-        # results set newest first: a client that waits on its requests in
-        # submission order wakes once, after the step's last result, rather
-        # than contending for the interpreter while the rest are handed out
-        for req in reversed(st.reqs):
+        items = []
+        for req in st.reqs:
             res = results.get(id(req))
             if res is None:
                 continue  # failed at launch
-            req.t_done = now
-            if isinstance(res, Exception):
-                req.future.set_exception(res)
-            else:
-                req.future.set_result(res)
-                self.completed.append(req)
+            items.append((req, None, res) if isinstance(res, Exception) else (req, res, None))
+        self._finish(items)
 
     def _rows(self, batch, t):
         """The step's scoring/variant rows: (spans, strings), and its explain
@@ -367,7 +506,7 @@ class GkmServer:
             good, r0 = [], 0
             for req in group:
                 if bad[r0:r0 + len(req.seqs)].any():
-                    req.future.set_exception(ValueError(_BAD_SEQ))
+                    self._finish([(req, None, ValueError(_BAD_SEQ))])
                 else:
                     good.append(req)
                 r0 += len(req.seqs)
@@ -415,7 +554,8 @@ class GkmServer:
         with torch.cuda.stream(stream):
             if strings:
                 xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
-                                                     min_width=kp.l)
+                                                     min_width=kp.l,
+                                                     staging=self._staging(stream, strings))
                 lens = xs.lengths
                 empty = lens == 0
                 short = lens < kp.l  # NaN rows, set when delivered
@@ -480,7 +620,8 @@ class GkmServer:
         with torch.cuda.stream(stream):
             if strings:
                 xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
-                                                     min_width=kp.l)
+                                                     min_width=kp.l,
+                                                     staging=self._staging(stream, strings))
                 empty = xs.lengths == 0
                 out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
                 E.score_rows(sv, xs, model.bias, out, stream=stream)
@@ -498,9 +639,12 @@ class GkmServer:
                      keep)
 
 
-def latency_stats(reqs) -> dict:
-    """p50 / p99 / mean request latency (submit to result) in milliseconds."""
-    lat = np.array([(r.t_done - r.t_submit) * 1e3 for r in reqs])
+def latency_stats(records) -> dict:
+    """p50 / p99 / mean request latency (submit to result) in milliseconds of
+    GkmServer.completed records (kind, t_submit, t_admit, t_done) or of
+    Request objects."""
+    lat = np.array([(r[3] - r[1]) * 1e3 if isinstance(r, tuple) else (r.t_done - r.t_submit) * 1e3
+                    for r in records])
     if lat.size == 0:
         return {"p50_ms": None, "p99_ms": None, "mean_ms": None, "n": 0}
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

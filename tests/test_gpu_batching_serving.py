@@ -195,3 +195,33 @@ def test_server_survives_a_failed_step(gpu):
         got = np.array([srv.submit("score", [q]).result(timeout=60)[0] for q in qs])
     np.testing.assert_array_equal(got, want)
     assert len(srv.completed) == 2
+
+
+@pytest.mark.parametrize("tables", [True, False])
+def test_server_first_requests_see_finished_tables(gpu, tables):
+    """Requests submitted right after GkmServer(...).start(), with no sync of
+    the caller's stream in between, read the tables and caches the
+    constructor built on that stream (l-mer and explain tables, or the SVs'
+    l-mer index and explain neighbourhoods) whole: perturbation explain and
+    scoring equal the same calls made afterwards."""
+    from gkmsvm_amd import gkmexplain
+    from gkmsvm_amd.serving import GkmServer
+
+    m = _random_model(S=2500, seed=8)
+    if not tables:
+        m.use_lmer_table = m.use_explain_table = False
+        m.use_lmer_index = True
+    qs = _queries(8, 200, 900, 21)
+    ex = _queries(4, 300, 600, 22)
+    srv = GkmServer(m, hbm_budget=2 << 30, tables=tables).start()
+    f_ex = [srv.submit("explain", [q], mode="perturbation") for q in ex]
+    f_sc = [srv.submit("score", [q]) for q in qs]
+    got_ex = [f.result(timeout=120) for f in f_ex]
+    got_sc = np.array([f.result(timeout=120)[0] for f in f_sc])
+    srv.stop()
+    torch.cuda.synchronize()
+    want_sc = m(qs).cpu().numpy().ravel()
+    np.testing.assert_array_equal(got_sc, want_sc)
+    for q, g in zip(ex, got_ex):
+        H.assert_close(g, gkmexplain(m, [q], mode="perturbation").cpu().numpy(), 1e-10,
+                       "explain right after start")
