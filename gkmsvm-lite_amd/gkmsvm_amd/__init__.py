@@ -6,11 +6,26 @@ work runs in hand-written HIP kernels
 (libgkmhip.so, C-ABI in include/gkmhip.h) called through ctypes.
 """
 
-from .codec import codes_to_one_hot, encode_codes, one_hot_encode, to_codes
+from .codec import (
+    codes_to_one_hot,
+    encode_batch,
+    encode_codes,
+    one_hot_decode,
+    one_hot_encode,
+    reverse_complement,
+    to_codes,
+    validate,
+)
 from .deltasvm import DeltaSVM
 from .explain import gkmexplain, mutation_impact
 from .gram import compute_gram
-from .importers import load_classic_model, load_lsgkm_model, load_r_gkmsvm_model, read_fasta
+from .importers import (
+    load_classic_model,
+    load_deltasvm_model,
+    load_lsgkm_model,
+    load_r_gkmsvm_model,
+    read_fasta,
+)
 from .ism import ism
 from .kernels import (
     KERNEL_ALIASES,
@@ -24,7 +39,6 @@ from .kernels import (
     resolve_kernel_type,
 )
 from .serialization import (
-    load_deltasvm_model,
     load_deltasvm_npz,
     load_model,
     load_npz,
@@ -43,5 +57,6 @@ __all__ = [
     "DirectGkmKernel", "EstTruncGkmKernel", "CenterWeightedGkmKernel", "RbfKernel",
     "RbfGkmKernel", "CenterWeightedRbfGkmKernel", "DeltaSVM", "load_classic_model",
     "load_r_gkmsvm_model", "load_model", "load_npz", "save_npz", "save_lsgkm",
-    "load_deltasvm_npz", "save_deltasvm_npz", "load_deltasvm_model",
+    "load_deltasvm_npz", "save_deltasvm_npz", "load_deltasvm_model", "encode_batch",
+    "one_hot_decode", "reverse_complement", "validate",
 ]

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from . import engine as E
-from .inputs import as_batch, full_lengths, nan_rows, short_rows
+from .inputs import nan_rows, read_batch
 
 
 class _Plan:
@@ -174,10 +174,10 @@ class BatchScorer:
 
     def __call__(self, x, *, lengths=None) -> torch.Tensor:
         """[B, 1] float64 decision values (NaN for rows shorter than l)."""
-        codes, lengths = as_batch(x, lengths)
+        batch = read_batch(x, lengths)
         l = self.kp.l
-        short = short_rows(lengths, l)
-        lens = full_lengths(codes, lengths)
+        short = batch.short(l)
+        codes, lens = batch.codes, batch.lengths
         # under "auto" the whole call decides whether the l-mer table pays
         self.model._plan_lmer_table(float(np.clip(lens - l + 1, 0, None).sum()))
         if self.model._lmer_table_for() is not None:

@@ -41,7 +41,7 @@ def encode_codes(sequence: str, *, allow_n: bool = False, allow_other: bool = Fa
     return np.where(raw > 3, 0, raw).astype(np.uint8)
 
 
-def encode_batch(sequences, min_width: int = 1):
+def encode_strings(sequences, min_width: int = 1):
     """Codes of many DNA strings in one pass: ([n, max(Lmax, min_width)] uint8
     padded with A, lengths [n] int64, bad [n] bool).  N is stored as A, as by
     encode_codes(allow_n=True); a string with any other letter, or empty, is
@@ -105,3 +105,69 @@ def codes_to_one_hot(codes, dtype=np.float64) -> np.ndarray:
 
 def decode_codes(codes) -> str:
     return "".join(BASES[int(c)] for c in np.asarray(codes).ravel())
+
+
+# -- one-hot utilities of the public API (codec.py:40-220 of gkmsvm-lite) -------------
+
+
+def _is_torch(x) -> bool:
+    return hasattr(x, "detach") and hasattr(x, "flip")
+
+
+def one_hot_decode(arr) -> str:
+    """The DNA string of a [4, L] one-hot array (an all-zero column, N, reads
+    as A).  ValueError for another shape."""
+    a = arr.detach().cpu().numpy() if _is_torch(arr) else np.asarray(arr)
+    if a.ndim != 2 or a.shape[0] != 4:
+        raise ValueError(f"one_hot_decode takes a [4, L] array, got shape {list(a.shape)}")
+    return _LETTERS[np.argmax(a, axis=0)].tobytes().decode()
+
+
+_LETTERS = np.frombuffer(BASES.encode(), dtype=np.uint8)
+
+
+def reverse_complement(arr):
+    """Reverse complement of a one-hot [4, L] or [B, 4, L] array (NumPy or
+    torch): channels A<->T, C<->G swap (the channel order A C G T reversed)
+    and positions run backwards.  ValueError for another shape."""
+    if arr.ndim not in (2, 3) or arr.shape[-2] != 4:
+        raise ValueError(f"reverse_complement takes [4, L] or [B, 4, L], got shape "
+                         f"{list(arr.shape)}")
+    if _is_torch(arr):
+        return arr.flip((-2, -1)).contiguous()
+    return np.ascontiguousarray(np.asarray(arr)[..., ::-1, ::-1])
+
+
+def validate(arr) -> None:
+    """ValueError unless arr [..., 4, L] is one-hot DNA: at every position
+    exactly one non-zero channel and channels summing to 1."""
+    a = arr.detach().cpu().numpy() if _is_torch(arr) else np.asarray(arr)
+    if a.ndim < 2 or a.shape[-2] != 4:
+        raise ValueError(f"a one-hot array has 4 channels on its second-to-last axis, got shape "
+                         f"{list(a.shape)}")
+    if not np.allclose(a.sum(axis=-2), 1.0):
+        raise ValueError("not one-hot: the channels of some position do not sum to 1")
+    if not ((a != 0).sum(axis=-2) == 1).all():
+        raise ValueError("not one-hot: some position has more than one non-zero channel")
+
+
+def encode_batch(sequences, *, dtype=np.float32, pad_value: float = 0.0,
+                 allow_n: bool = False, allow_other: bool = False):
+    """(one-hot [B, 4, L_max] of DNA strings, padded with pad_value past each
+    string; mask [B, L_max] bool, True inside each string).  N is an all-zero
+    column when allow_n (and any other letter too when allow_other); otherwise
+    they raise ValueError, as does an empty list or an empty string."""
+    if not len(sequences):
+        raise ValueError("encode_batch needs at least one sequence")
+    raw = [_raw_codes(s, allow_n, allow_other) for s in sequences]
+    width = max(len(r) for r in raw)
+    out = np.full((len(raw), 4, width), pad_value, dtype=dtype)
+    mask = np.zeros((len(raw), width), dtype=bool)
+    for i, r in enumerate(raw):
+        n = len(r)
+        mask[i, :n] = True
+        col = np.zeros((4, n), dtype=dtype)
+        keep = r < 4
+        col[r[keep], np.flatnonzero(keep)] = 1
+        out[i, :, :n] = col
+    return out, mask

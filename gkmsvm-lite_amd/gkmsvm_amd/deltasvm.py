@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from . import engine as E
-from .inputs import as_batch, full_lengths
+from .inputs import read_batch
 
 _BASE_MAP = {"A": 0, "C": 1, "G": 2, "T": 3}
 
@@ -61,7 +61,7 @@ class DeltaSVM:
         self._device = None
         self._dev_weights = None
         self._dev_combos = None
-        if device is not None and str(device) != "cpu":
+        if E.placement(device) is not None:
             self.rocm(device)
 
     @property
@@ -96,8 +96,8 @@ class DeltaSVM:
                 "this model is on the CPU: DeltaSVM scoring runs on ROCm GPUs only; "
                 "call model.rocm()"
             )
-        codes, lengths = as_batch(x, lengths)
-        lens = full_lengths(codes, lengths)
+        batch = read_batch(x, lengths)
+        codes, lens = batch.codes, batch.lengths
         B, L = codes.shape
         out = torch.empty(B, dtype=torch.float64, device=self._device)
         c = torch.from_numpy(np.ascontiguousarray(codes)).to(self._device)

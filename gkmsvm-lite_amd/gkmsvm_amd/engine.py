@@ -27,8 +27,27 @@ WAVE = 64
 MAX_HIST_BYTES = 1 << 30
 
 
+GPU_NAMES = ("rocm", "cuda", "gpu", "auto")
+
+
+def placement(device) -> torch.device | None:
+    """Where a model created or loaded with `device` is held: the visible
+    ROCm GPU for "rocm", "cuda", "gpu" or "cuda:N" (or a torch.device);
+    None -- held on the host, where it cannot compute until rocm() -- for
+    "cpu", and for "auto" on a machine without a ROCm GPU ("auto" takes the
+    GPU when one is visible, as the reference's backend.py:150-164 takes
+    CUDA).  ValueError for any other name."""
+    name = "rocm" if device is None else str(device)
+    if name == "cpu" or (name == "auto" and not torch.cuda.is_available()):
+        return None
+    if name in GPU_NAMES or name.startswith("cuda:"):
+        return device_of(None if name in GPU_NAMES else device)
+    raise ValueError(f"unknown device {device!r}: use 'rocm' (or 'cuda', 'gpu', 'cuda:N'), "
+                     "'auto' or 'cpu'")
+
+
 def device_of(device=None) -> torch.device:
-    if device is None or str(device) in ("rocm", "cuda", "gpu"):
+    if device is None or str(device) in GPU_NAMES:
         if not torch.cuda.is_available():
             raise RuntimeError(
                 "no ROCm GPU visible: the gkm engine runs on MI355X (gfx950) only"

@@ -10,15 +10,7 @@ import torch
 from . import _lib
 from . import engine as E
 from .codec import codes_to_one_hot
-from .inputs import (
-    allele_arrays,
-    as_batch,
-    full_lengths,
-    joint_lengths,
-    nan_rows,
-    per_position,
-    short_rows,
-)
+from .inputs import mask_positions, nan_rows, read_batch, read_pair, stacked
 
 _MODE_ALIASES = {
     0: "importance",
@@ -96,10 +88,10 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
             f"(or 0/1/2), got {mode!r}"
         )
     dev = model._require_device()
-    codes, lengths = as_batch(x, lengths)
+    batch = read_batch(x, lengths)
     base = model.kernel.base
-    short = short_rows(lengths, base.l, "attributions")
-    lens = full_lengths(codes, lengths)
+    short = batch.short(base.l, "attributions")
+    codes, lens = batch.codes, batch.lengths
     B, L = codes.shape
     A = model._explain_table_for(B, lens, resolved == "perturbation") if L >= base.l else None
     # the table kernel writes every entry (0 past each row's length)
@@ -126,7 +118,7 @@ def gkmexplain(model, x, *, mode="importance", lengths=None, batch_size: int = 5
         out = out * torch.from_numpy(codes_to_one_hot(codes)).to(dev)
     if A is not None:
         return nan_rows(out, short)
-    return per_position(out, lengths, short)
+    return mask_positions(out, batch, short)
 
 
 def _explain_rows(model, codes, lens, resolved, out):
@@ -274,16 +266,13 @@ def mutation_impact(model, ref, alt, *, ref_lengths=None, alt_lengths=None,
     dev = model._require_device()
     kp = model._params()
     sv = model._svs()
-    ref, alt, ref_lengths, alt_lengths = allele_arrays(ref, alt, ref_lengths, alt_lengths)
-    B, L = ref.shape
-    short = None
-    if ref_lengths is not None or alt_lengths is not None:
-        both = joint_lengths(L, (B, ref_lengths), (B, alt_lengths))
-        short = short_rows(both, kp.l, "mutation impacts")
-        if short is not None:
-            short = short.reshape(2, -1).any(axis=0)
-    rl = full_lengths(ref, ref_lengths)
-    al = full_lengths(alt, alt_lengths)
+    rb, ab = read_pair(ref, alt, ref_lengths, alt_lengths)
+    B = rb.n
+    # a variant whose either allele has no window scores NaN
+    short = stacked(rb, ab).short(kp.l, "mutation impacts")
+    if short is not None:
+        short = short.reshape(2, -1).any(axis=0)
+    ref, alt, rl, al = rb.codes, ab.codes, rb.lengths, ab.lengths
     out = torch.zeros(B, dtype=torch.float64, device=dev)
     bs = B if batch_size is None else max(1, int(batch_size))
     st = E.stream_handle()

@@ -7,7 +7,7 @@ import torch
 
 from . import _lib
 from . import engine as E
-from .inputs import as_batch, full_lengths
+from .inputs import read_batch
 
 
 GRAM_DEVICE_BYTES = 32 << 30  # a Gram up to this size is assembled in HBM
@@ -55,14 +55,14 @@ def compute_gram(kernel, X, Y=None, *, chunk_size: int = 1000, verbose: bool = F
     """
     dev = E.device_of(device)
     kp = E.KernelParams(kernel, dev)
-    xc, xl = as_batch(X, lengths)
-    xl = full_lengths(xc, xl)
+    xb = read_batch(X, lengths)
+    xc, xl = xb.codes, xb.lengths
     symmetric = Y is None
     if symmetric:
         yc, yl = xc, xl
     else:
-        yc, yl = as_batch(Y, y_lengths)
-        yl = full_lengths(yc, yl)
+        yb = read_batch(Y, y_lengths)
+        yc, yl = yb.codes, yb.lengths
     N, M = xc.shape[0], yc.shape[0]
     gram = np.zeros((N, M), dtype=np.float64) if out is None else out
     st = E.stream_handle()

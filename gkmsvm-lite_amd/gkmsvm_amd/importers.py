@@ -1,7 +1,18 @@
-"""LS-GKM model files, classic / R gkmSVM model files and FASTA (the formats
-the fixtures and users hold; gkmsvm-lite importers/lsgkm.py:23-97,
-importers/__init__.py:15-113, importers/classic.py, importers/r_gkmsvm.py,
-fasta.py:19-42)."""
+"""Readers for the model and sequence files users already hold.
+
+* LS-GKM models (what lsgkm's gkmtrain writes; plain or gzipped): a
+  ``key value`` header, a line ``SV``, then ``coefficient SEQUENCE`` lines.
+  The decision value is sum_i coef_i K(x, sv_i) - rho, so bias = -rho
+  (the reference reads the same format in importers/lsgkm.py:23-97).
+* Classic C++ / R gkmSVM models: PREFIX_svalpha.out (``id alpha`` lines) and
+  PREFIX_svseq.fa.  They hold neither a bias nor the kernel's parameters,
+  which are arguments here (importers/classic.py, importers/r_gkmsvm.py).
+* deltaSVM weight files: ``kmer weight`` or ``kmer revcomp weight`` lines
+  (importers/deltasvm.py:21-115).
+* FASTA (fasta.py:19-42).
+
+Files are read in one pass; gzip is recognised by its magic bytes.
+"""
 
 from __future__ import annotations
 
@@ -11,172 +22,265 @@ from pathlib import Path
 
 import numpy as np
 
-from .codec import encode_codes
-from .inputs import as_batch
+from .codec import encode_strings
 from .kernels import KERNEL_ALIASES
 
-_INT_KEYS = ("L", "k", "d", "M", "nr_class", "total_sv", "norc")
-_FLOAT_KEYS = ("gamma", "H")
-_LIST_KEYS = {"rho": float, "label": int, "nr_sv": int, "probA": float, "probB": float}
+
+def open_text(path):
+    """A text handle on path, through gzip when the file starts with its
+    magic bytes."""
+    with open(path, "rb") as raw:
+        gz = raw.read(2) == b"\x1f\x8b"
+    return gzip.open(path, "rt") if gz else open(path, "r")
 
 
-def _open(path):
-    path = Path(path)
-    with open(path, "rb") as f:
-        magic = f.read(2)
-    return gzip.open(path, "rt") if magic == b"\x1f\x8b" else open(path, "r")
-
-
-def read_fasta(path):
-    """[(name, sequence)] of a (possibly gzipped) FASTA file."""
-    recs, name, seq = [], None, []
-    with _open(path) as f:
-        for line in f:
-            line = line.strip()
-            if not line:
+def read_fasta(path) -> list[tuple[str, str]]:
+    """[(header without '>', sequence)] of a plain or gzipped FASTA file; a
+    record's sequence lines are joined, blank lines skipped."""
+    records: list[tuple[str, list[str]]] = []
+    with open_text(path) as f:
+        for raw in f:
+            text = raw.strip()
+            if not text:
                 continue
-            if line.startswith(">"):
-                if name is not None:
-                    recs.append((name, "".join(seq)))
-                name, seq = line[1:].strip(), []
-            else:
-                seq.append(line)
-    if name is not None:
-        recs.append((name, "".join(seq)))
-    return recs
+            if text[0] == ">":
+                records.append((text[1:].strip(), []))
+            elif records:
+                records[-1][1].append(text)
+    return [(name, "".join(parts)) for name, parts in records]
 
 
-def parse_lsgkm_header(path) -> dict:
-    header = {"norc": 0}
-    with _open(path) as f:
-        for line in f:
-            line = line.strip()
-            if line.startswith("SV"):
-                break
-            parts = line.split(None, 1)
-            if len(parts) != 2:
+# -- LS-GKM ------------------------------------------------------------------------------
+
+
+def _kernel_name(text: str):
+    """kernel_type as written: a name, or a -t number mapped to its name."""
+    return KERNEL_ALIASES.get(int(text), text) if text.isdigit() else text
+
+
+def _numbers(cast):
+    return lambda text: [cast(v) for v in text.split()]
+
+
+# header fields and how each is typed; unknown keys are skipped
+LSGKM_FIELDS = {
+    "svm_type": str, "kernel_type": _kernel_name,
+    "L": int, "k": int, "d": int, "M": int, "nr_class": int, "total_sv": int, "norc": int,
+    "gamma": float, "H": float,
+    "rho": _numbers(float), "probA": _numbers(float), "probB": _numbers(float),
+    "label": _numbers(int), "nr_sv": _numbers(int),
+}
+
+
+def _lsgkm_sections(path):
+    """(header dict, [(coefficient, sequence)]) of an LS-GKM model file."""
+    header: dict = {"norc": 0}
+    svs: list[tuple[float, str]] = []
+    in_svs = False
+    with open_text(path) as f:
+        for lineno, raw in enumerate(f, 1):
+            fields = raw.split()
+            if not fields:
                 continue
-            key, value = parts
-            if key == "svm_type":
-                header[key] = value
-            elif key == "kernel_type":
-                header[key] = KERNEL_ALIASES.get(int(value), value) if value.isdigit() else value
-            elif key in _INT_KEYS:
-                header[key] = int(value)
-            elif key in _FLOAT_KEYS:
-                header[key] = float(value)
-            elif key in _LIST_KEYS:
-                header[key] = [_LIST_KEYS[key](v) for v in value.split()]
-    return header
+            if in_svs:
+                if len(fields) < 2:
+                    raise ValueError(f"{path}:{lineno}: an SV line needs a coefficient and a "
+                                     f"sequence, got {raw.strip()[:60]!r}")
+                svs.append((float(fields[0]), fields[1]))
+            elif fields[0] == "SV":
+                in_svs = True
+            elif len(fields) >= 2 and fields[0] in LSGKM_FIELDS:
+                key = fields[0]
+                header[key] = LSGKM_FIELDS[key](raw.strip()[len(key):].strip())
+    return header, svs
+
+
+def read_lsgkm_header(path) -> dict:
+    """The typed header of an LS-GKM model file (norc defaults to 0)."""
+    return _lsgkm_sections(path)[0]
+
+
+_ACGTN = frozenset("ACGTN")
+
+
+def _model_from_strings(seqs, coefs, bias, kernel_type, params, *, sv_chunk_size, device):
+    """GkmSVM of SV strings (letters other than ACGT read as A, with the
+    reverse strand reading T, as LS-GKM packs them)."""
+    from .svm import GkmSVM
+
+    codes, lens, _ = encode_strings(seqs)
+    given = None if bool((lens == codes.shape[1]).all()) else lens
+    return GkmSVM(codes, np.asarray(coefs, dtype=np.float64), bias, kernel_type, params,
+                  sv_lengths=given, sv_chunk_size=sv_chunk_size, device=device)
 
 
 def load_lsgkm_model(path, *, sv_chunk_size: int | None = None, device: str = "rocm"):
-    """GkmSVM from an LS-GKM model file (.txt or .txt.gz); bias = -rho."""
-    from .svm import GkmSVM
-
-    header = parse_lsgkm_header(path)
-    for key in ("kernel_type", "total_sv", "rho"):
-        if header.get(key) is None:
-            raise ValueError(f"Model file missing {key} field")
+    """GkmSVM of an LS-GKM model file (.txt or .txt.gz): kernel type and L,
+    k, d, gamma, M, H, norc from the header, bias = -rho.  ValueError for a
+    file without kernel_type, total_sv or rho, without SVs, or whose SV
+    count differs from total_sv; NotImplementedError unless nr_class is 2."""
+    header, svs = _lsgkm_sections(path)
+    absent = [key for key in ("kernel_type", "total_sv", "rho") if key not in header]
+    if absent:
+        raise ValueError(f"{path} is not an LS-GKM model: no {', '.join(absent)} in its header")
     if header.get("nr_class", 2) != 2:
-        raise NotImplementedError("Only binary classification (nr_class=2) is supported")
-    coefs, seqs, in_sv = [], [], False
-    with _open(path) as f:
-        for line in f:
-            line = line.strip()
-            if not line:
-                continue
-            if line.startswith("SV"):
-                in_sv = True
-                continue
-            if not in_sv:
-                continue
-            parts = line.split()
-            if len(parts) < 2:
-                raise ValueError(f"Malformed SV line: {line[:80]}")
-            coefs.append(float(parts[0]))
-            seqs.append(parts[1])
-    if not seqs:
-        raise ValueError("No support vector sequences found")
-    if len(seqs) != header["total_sv"]:
-        raise ValueError(f"Expected {header['total_sv']} support vectors, got {len(seqs)}")
-    other = sum(bool(set(s.upper()) - set("ACGTN")) for s in seqs)
-    if other:
-        warnings.warn(f"{other} support vectors contain letters other than ACGTN; reading "
-                      "them as N (scored as A, as LS-GKM does)", stacklevel=2)
-    codes, lengths = as_batch([encode_codes(s, allow_other=True) for s in seqs])
-    params = {"L": header["L"], "k": header["k"], "include_rc": header.get("norc", 0) == 0}
-    for key in ("d", "gamma", "M", "H"):
-        if key in header:
-            params[key] = header[key]
-    return GkmSVM(codes, np.array(coefs, dtype=np.float64), -header["rho"][0],
-                  header["kernel_type"], params, sv_lengths=lengths,
-                  sv_chunk_size=sv_chunk_size, device=device)
+        raise NotImplementedError(
+            f"{path} has {header['nr_class']} classes; only two-class models (nr_class 2) load")
+    if not svs:
+        raise ValueError(f"{path} lists no support vectors")
+    if len(svs) != header["total_sv"]:
+        raise ValueError(f"{path} declares total_sv {header['total_sv']} but lists {len(svs)} "
+                         "support vectors")
+    seqs = [s for _, s in svs]
+    odd = sum(1 for s in seqs if not set(s.upper()) <= _ACGTN)
+    if odd:
+        warnings.warn(f"{odd} support vectors of {path} hold letters beyond ACGTN; they are "
+                      "read as N (A on the forward strand), as LS-GKM reads them", stacklevel=2)
+    params = {"L": header["L"], "k": header["k"], "include_rc": header["norc"] == 0}
+    params.update({key: header[key] for key in ("d", "gamma", "M", "H") if key in header})
+    return _model_from_strings(seqs, [c for c, _ in svs], -header["rho"][0],
+                               header["kernel_type"], params, sv_chunk_size=sv_chunk_size,
+                               device=device)
 
 
-# gkmsvm_kernel / gkmsvm_classify -t (R useTgkm) as kernel types: full filter,
-# truncated filter (the tools' default) and gkm counts, each cut at d
-# mismatches (importers/classic.py:24-27); 3 and 4 have no equivalent.
+# -- classic C++ and R gkmSVM --------------------------------------------------------------
+
+# gkmsvm_kernel / gkmsvm_classify -t (R: useTgkm) as kernel types, each cut
+# at d mismatches: full filter, truncated filter (the tools' default), gkm
+# counts.  3 (wildcard) and 4 (mismatch kernel) have no counterpart here.
 GKMSVM_FILTERS = {0: "gkm_estfull", 1: "gkm_esttrunc", 2: "gkm_cnt"}
+
+_ACGT = frozenset(b"ACGTacgt")
+_NOT_ACGT = bytes(b for b in range(256) if b not in _ACGT)
+
+
+def _acgt_only(seq: str) -> str:
+    """seq without its non-ACGT letters, upper case (how gkmSVM's FASTA
+    reader sees it)."""
+    return seq.encode("latin-1", "replace").translate(None, _NOT_ACGT).decode().upper()
+
+
+def _svalpha(path) -> list[tuple[str, float]]:
+    pairs = []
+    with open_text(path) as f:
+        for lineno, raw in enumerate(f, 1):
+            fields = raw.split()
+            if not fields:
+                continue
+            if len(fields) != 2:
+                raise ValueError(f"{path}:{lineno}: expected an SV id and its alpha, got "
+                                 f"{raw.strip()[:60]!r}")
+            pairs.append((fields[0], float(fields[1])))
+    return pairs
 
 
 def load_classic_model(alpha_path, svseq_path=None, *, L: int = 10, k: int = 6, d: int = 3,
                        filter_type: int = 1, include_rc: bool = True,
                        sv_chunk_size: int | None = None, device: str = "rocm"):
-    """GkmSVM from the two files classic (C++) gkmsvm_train writes
-    (importers/classic.py:30-127): PREFIX_svalpha.out (``id <TAB> alpha``) and
-    PREFIX_svseq.fa (default: next to it).  The files hold no bias and no
-    kernel parameters, so L, k, d, filter_type (-t) and include_rc are
-    arguments; SV letters other than ACGT are dropped, as the tools read them
-    (with a warning).  Bias 0."""
-    import re
-
-    from .svm import GkmSVM
-
+    """GkmSVM of the two files gkmsvm_train (classic C++, or the R package)
+    writes: PREFIX_svalpha.out and PREFIX_svseq.fa (by default the file
+    beside alpha_path).  L, k, d, filter_type (-t) and include_rc are the
+    kernel's parameters, which the files do not hold; bias 0.  SVs are
+    matched to alphas by FASTA id and lose their non-ACGT letters, as the
+    tools read them (with a warning)."""
     alpha_path = Path(alpha_path)
-    if svseq_path is None:
-        name = alpha_path.name
-        if not name.endswith("svalpha.out"):
-            raise ValueError(f"give svseq_path for {alpha_path}")
-        svseq_path = alpha_path.with_name(name[: -len("svalpha.out")] + "svseq.fa")
-    ids, coefs = [], []
-    with _open(alpha_path) as f:
-        for line in f:
-            line = line.strip()
-            if not line:
-                continue
-            parts = line.split()
-            if len(parts) != 2:
-                raise ValueError(f"Expected 'id <TAB> alpha', got: {line[:80]}")
-            ids.append(parts[0])
-            coefs.append(float(parts[1]))
-    by_id = {name.split()[0]: seq for name, seq in read_fasta(svseq_path)}
-    missing = [i for i in ids if i not in by_id]
-    if missing:
-        raise ValueError(f"{len(missing)} SV ids of {alpha_path} are not in {svseq_path}, "
-                         f"e.g. {missing[0]}")
-    seqs = [re.sub("[^ACGTacgt]", "", by_id[i]).upper() for i in ids]
-    dropped = sum(len(by_id[i]) != len(s) for i, s in zip(ids, seqs))
-    if dropped:
-        warnings.warn(f"{dropped} support vectors hold letters other than ACGT, dropped as "
-                      "gkmSVM reads them (queries with N score as in gkmsvm_classify only "
-                      "with those letters dropped too)", stacklevel=2)
     if filter_type not in GKMSVM_FILTERS:
         raise NotImplementedError(
-            f"gkmSVM filter type {filter_type} (wildcard/mismatch kernels) is not supported; "
-            f"use {sorted(GKMSVM_FILTERS)}")
-    if not seqs:
-        raise ValueError("No support vector sequences found")
-    codes, lengths = as_batch([encode_codes(s) for s in seqs])
+            f"filter type {filter_type} is not a gkm kernel this engine computes "
+            f"(supported: {sorted(GKMSVM_FILTERS)})")
+    if svseq_path is None:
+        stem = alpha_path.name
+        if not stem.endswith("svalpha.out"):
+            raise ValueError(f"{alpha_path} is not named PREFIX_svalpha.out: pass svseq_path")
+        svseq_path = alpha_path.with_name(stem[: -len("svalpha.out")] + "svseq.fa")
+    pairs = _svalpha(alpha_path)
+    if not pairs:
+        raise ValueError(f"{alpha_path} lists no support vectors")
+    fasta = {}
+    for name, seq in read_fasta(svseq_path):
+        fasta[name.split()[0] if name.split() else name] = seq
+    absent = [sv_id for sv_id, _ in pairs if sv_id not in fasta]
+    if absent:
+        raise ValueError(f"{svseq_path} lacks {len(absent)} of the SV ids in {alpha_path} "
+                         f"(first: {absent[0]}); they are not in the FASTA file")
+    raw = [fasta[sv_id] for sv_id, _ in pairs]
+    seqs = [_acgt_only(s) for s in raw]
+    changed = sum(len(a) != len(b) for a, b in zip(raw, seqs))
+    if changed:
+        warnings.warn(f"{changed} support vectors held letters other than ACGT, dropped as "
+                      "gkmSVM drops them (queries match gkmsvm_classify only when prepared the "
+                      "same way)", stacklevel=2)
     params = {"L": L, "k": k, "d": d, "include_rc": bool(include_rc)}
-    return GkmSVM(codes, np.array(coefs, dtype=np.float64), 0.0, GKMSVM_FILTERS[filter_type],
-                  params, sv_lengths=lengths, sv_chunk_size=sv_chunk_size, device=device)
+    return _model_from_strings(seqs, [a for _, a in pairs], 0.0, GKMSVM_FILTERS[filter_type],
+                               params, sv_chunk_size=sv_chunk_size, device=device)
 
 
 def load_r_gkmsvm_model(path, svseq_path=None, *, L: int = 10, K: int = 6, maxnmm: int = 3,
                         useTgkm: int = 1, addRC: bool = True, sv_chunk_size: int | None = None,
                         device: str = "rocm"):
-    """GkmSVM from the files R gkmSVM's gkmsvm_train writes (the classic
-    format, importers/r_gkmsvm.py:17-56) with the R package's argument names."""
+    """load_classic_model under the R gkmSVM package's argument names (L, K,
+    maxnmm, useTgkm, addRC)."""
     return load_classic_model(path, svseq_path, L=L, k=K, d=maxnmm, filter_type=useTgkm,
                               include_rc=addRC, sv_chunk_size=sv_chunk_size, device=device)
+
+
+# -- deltaSVM weight files ---------------------------------------------------------------
+
+
+def _is_kmer(text: str) -> bool:
+    return bool(text) and not text.strip("ACGT")
+
+
+def load_deltasvm_model(path, l: int, *, include_rc: bool | None = None, bias: float = 0.0,
+                        dtype=np.float32, device: str = "rocm"):
+    """DeltaSVM of a deltaSVM weight file.  Lines hold ``kmer weight``, or
+    ``kmer revcomp weight`` when the file is collapsed by reverse
+    complement (both k-mers then get the weight, and include_rc defaults to
+    False; so does a ``#`` comment containing ``include_rc=false``).
+    Columns split on tabs, else on spaces; a first line with no A/C/G/T
+    k-mer is a header.  Weights are kept at `dtype` (deltaSVM prints single
+    precision) and scored in float64.  ValueError for a malformed line, a
+    k-mer of other letters or another length, an empty file, or l < k."""
+    from .deltasvm import DeltaSVM, kmer_to_index
+
+    table: dict[str, float] = {}
+    k = None
+    collapsed = comment_no_rc = False
+    with open_text(path) as f:
+        for lineno, raw in enumerate(f, 1):
+            text = raw.strip()
+            if not text:
+                continue
+            if text[0] == "#":
+                comment_no_rc |= "include_rc=false" in text.lower()
+                continue
+            cols = text.split("\t") if "\t" in text else text.split()
+            if len(cols) not in (2, 3):
+                raise ValueError(f"{path}:{lineno}: expected 2 or 3 columns, found {len(cols)}")
+            kmer = cols[0].upper()
+            if not _is_kmer(kmer):
+                if lineno == 1 and not any(c in "ACGT" for c in kmer):
+                    continue  # a header line
+                raise ValueError(f"{path}:{lineno}: {cols[0]!r} is not a k-mer of A, C, G, T")
+            if k is None:
+                k = len(kmer)
+            if len(kmer) != k:
+                raise ValueError(f"{path}:{lineno}: k-mer of length {len(kmer)} in a file of "
+                                 f"length-{k} k-mers")
+            w = float(cols[-1])
+            table[kmer] = w
+            if len(cols) == 3:
+                collapsed = True
+                partner = cols[1].upper()
+                if len(partner) == k and _is_kmer(partner):
+                    table[partner] = w
+    if k is None:
+        raise ValueError(f"{path} holds no k-mer weights")
+    if l < k:
+        raise ValueError(f"window length l = {l} must be >= the file's k = {k}")
+    if include_rc is None:
+        include_rc = not (collapsed or comment_no_rc)
+    weights = np.zeros(4**k, dtype=dtype)
+    for kmer, w in table.items():
+        weights[kmer_to_index(kmer)] = w
+    return DeltaSVM(weights, l, k, include_rc=include_rc, bias=bias, device=device)

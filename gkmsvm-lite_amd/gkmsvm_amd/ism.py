@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from . import engine as E
-from .inputs import as_batch, full_lengths, per_position, short_rows
+from .inputs import mask_positions, read_batch
 
 
 def _sv_chunk(B: int, S: int) -> int:
@@ -29,10 +29,10 @@ def ism(model, x, *, lengths=None, batch_size: int | None = None,
     the reference base and past each sequence's length, NaN rows for sequences
     shorter than l)."""
     dev = model._require_device()
-    codes, lengths = as_batch(x, lengths)
+    batch = read_batch(x, lengths)
     kp = model._params()
-    short = short_rows(lengths, kp.l, "ISM deltas")
-    lens = full_lengths(codes, lengths)
+    short = batch.short(kp.l, "ISM deltas")
+    codes, lens = batch.codes, batch.lengths
     B, L = codes.shape
     out = torch.zeros((B, 4, L), dtype=torch.float64, device=dev)
     if L >= kp.l and kp.dmax >= 0:
@@ -53,7 +53,7 @@ def ism(model, x, *, lengths=None, batch_size: int | None = None,
                 _ism_rows_table(model, codes[s:e], lens[s:e], table, out[s:e])
             else:
                 _ism_rows(model, codes[s:e], lens[s:e], out[s:e])
-    return per_position(out, lengths, short)
+    return mask_positions(out, batch, short)
 
 
 def _mutant_diag(kp, xs, dx):

@@ -38,7 +38,7 @@ import torch
 
 from . import engine as E
 from . import tables as T
-from .codec import encode_batch
+from .codec import encode_strings
 from . import _lib
 
 _BAD_SEQ = "Invalid sequence (empty, or a letter other than ACGTN)"
@@ -502,7 +502,7 @@ class GkmServer:
         for req in reqs:
             by_mode.setdefault(req.mode, []).append(req)
         for i, (mode, group) in enumerate(by_mode.items()):
-            _, _, bad = encode_batch([q for req in group for q in req.seqs])
+            _, _, bad = encode_strings([q for req in group for q in req.seqs])
             good, r0 = [], 0
             for req in group:
                 if bad[r0:r0 + len(req.seqs)].any():

@@ -15,16 +15,7 @@ from . import _lib
 from . import engine as E
 from . import tables as T
 from .codec import to_codes
-from .inputs import (
-    allele_arrays,
-    as_batch,
-    full_lengths,
-    joint_lengths,
-    nan_rows,
-    rows_of,
-    short_rows,
-    uniform_or_lengths,
-)
+from .inputs import check_lengths, nan_rows, read_batch, read_pair, row_lengths, stacked
 from .kernels import KERNEL_ALIASES, KERNEL_BUILDERS, build_kernel, resolve_kernel_type  # noqa: F401
 
 
@@ -59,7 +50,7 @@ class GkmSVM:
                 f"coefficients ({coef.shape[0]})"
             )
         self.support_sequences = np.ascontiguousarray(sv)
-        self.sv_lengths = uniform_or_lengths(sv_lengths, sv.shape)
+        self.sv_lengths = check_lengths(sv_lengths, sv.shape[0], sv.shape[-1])
         self.coefficients = coef
         self.bias = float(bias)
         self.kernel_type = resolve_kernel_type(kernel_type)
@@ -82,7 +73,7 @@ class GkmSVM:
         self._device = None
         self._kp = None
         self._sv = None
-        if device is not None and str(device) != "cpu":
+        if E.placement(device) is not None:
             self.rocm(device)
 
     # -- device switch -------------------------------------------------------
@@ -133,7 +124,7 @@ class GkmSVM:
         if self._sv is None:
             kp = self._params()
             self._sv = E.SVModel(kp, self.support_sequences,
-                                 full_lengths(self.support_sequences, self.sv_lengths),
+                                 row_lengths(self.support_sequences, self.sv_lengths),
                                  self.coefficients)
         return self._sv
 
@@ -291,16 +282,16 @@ class GkmSVM:
         codes or DNA strings; lengths: [B] lengths of a padded x's rows.
         """
         self._require_device()
-        codes, lengths = as_batch(x, lengths)
+        batch = read_batch(x, lengths)
         l = self.kernel.base.l
-        short = short_rows(lengths, l)
-        lens = full_lengths(codes, lengths)
-        B = codes.shape[0]
+        short = batch.short(l)
+        codes, lens = batch.codes, batch.lengths
+        B = batch.n
         if short is not None and short.all():
             return nan_rows(torch.zeros((B, 1), dtype=torch.float64, device=self._device), short)
-        if codes.shape[1] < l:
-            raise ValueError(f"Sequence length {codes.shape[1]} is shorter than window length {l}")
-        self._plan_lmer_table(float(np.clip(lens - l + 1, 0, None).sum()))
+        if batch.width < l:
+            raise ValueError(f"sequences of {batch.width} bases hold no window of l = {l}")
+        self._plan_lmer_table(float(batch.n_windows(l).sum()))
         bs = B if batch_size is None else max(1, int(batch_size))
         parts = []
         starts = range(0, B, bs)
@@ -319,15 +310,11 @@ class GkmSVM:
         """[B, 1] variant effect scores: score(alt) - score(ref) ("kernel") or
         the GkmExplain mutation impact ("gkmexplain", lsgkm gkmexplain -m 5)."""
         if method == "kernel":
-            ref, alt, ref_lengths, alt_lengths = allele_arrays(ref, alt, ref_lengths, alt_lengths)
-            n = ref.shape[0]
-            scores = self(
-                np.concatenate([alt, ref]),
-                lengths=joint_lengths(ref.shape[-1], (n, alt_lengths), (n, ref_lengths)),
-                batch_size=batch_size,
-                verbose=verbose,
-            )
-            return scores[:n] - scores[n:]
+            r, a = read_pair(ref, alt, ref_lengths, alt_lengths)
+            both = stacked(a, r)
+            scores = self(both.codes, lengths=both.given_lengths(), batch_size=batch_size,
+                          verbose=verbose)
+            return scores[: r.n] - scores[r.n:]
         if method == "gkmexplain":
             from .explain import mutation_impact
 
@@ -339,6 +326,3 @@ class GkmSVM:
     def _sv_diag(self) -> torch.Tensor:
         """K_raw(sv, sv) [S] of the base kernel, in SV order."""
         return self._svs().diag_original_order()
-
-    def _rows(self, lengths, start, n):
-        return rows_of(lengths, start, n)

@@ -28,7 +28,7 @@ import torch
 from . import _lib
 from . import engine as E
 from .gram import _gram_index, compute_gram
-from .inputs import as_batch, full_lengths
+from .inputs import read_batch
 from .kernels import KERNEL_BUILDERS, resolve_kernel_type
 from .svm import GkmSVM
 
@@ -141,9 +141,9 @@ class KernelColumns:
 
 
 def _codes(seqs):
-    codes, lengths = as_batch(seqs if not isinstance(seqs, np.ndarray) or seqs.ndim != 1
-                              else list(seqs))
-    return codes, full_lengths(codes, lengths)
+    batch = read_batch(seqs if not isinstance(seqs, np.ndarray) or seqs.ndim != 1
+                       else list(seqs))
+    return batch.codes, batch.lengths
 
 
 def _concat(parts):

@@ -34,10 +34,10 @@ def tool_scores(path):
 
 
 def oracle_model(m):
-    from gkmsvm_amd.inputs import full_lengths
+    from gkmsvm_amd.inputs import row_lengths
 
     return O.Model(O.Kern.make(m.kernel_type, m.kernel_params), m.support_sequences,
-                   full_lengths(m.support_sequences, m.sv_lengths), m.coefficients, m.bias)
+                   row_lengths(m.support_sequences, m.sv_lengths), m.coefficients, m.bias)
 
 
 @pytest.mark.parametrize("prefix,tool,ft", CLASSIC)

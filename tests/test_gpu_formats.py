@@ -88,20 +88,20 @@ def test_deltasvm_weight_file_scores(gpu, tmp_path):
 
 def test_strings_decoded_on_the_device(gpu):
     """DeviceSeqs.from_strings (the strings' bytes decoded by gkm_decode_ascii)
-    gives the codes, planes and windows of encode_batch + DeviceSeqs, and flags
-    exactly the rows encode_batch flags for a letter other than ACGTN (either
+    gives the codes, planes and windows of encode_strings + DeviceSeqs, and flags
+    exactly the rows encode_strings flags for a letter other than ACGTN (either
     case, non-ASCII included)."""
     import torch
 
     from gkmsvm_amd import engine as E
-    from gkmsvm_amd.codec import encode_batch
+    from gkmsvm_amd.codec import encode_strings
 
     rng = np.random.default_rng(8)
     letters = list("ACGTacgtNn")
     strs = ["".join(rng.choice(letters, int(rng.integers(1, 700)))) for _ in range(40)]
     strs += ["", "A", "ACGTACGTACG", "acgtXacgt", "ACGTéACGT", "ACGT ACGT", "NNNNNNNNNNNN",
              "ACGU", "@ACGT", "acgt[", "ACGTacgt" * 300]
-    codes, lens, bad = encode_batch(strs, 11)
+    codes, lens, bad = encode_strings(strs, 11)
     xs, dbad = E.DeviceSeqs.from_strings(strs, gpu, rc=True, min_width=11)
     ref = E.DeviceSeqs(codes, lens, gpu, rc=True)
     torch.cuda.synchronize()

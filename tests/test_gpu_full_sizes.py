@@ -40,11 +40,11 @@ def _histograms_both_paths(m, qs, gpu, n_cols, rng):
     """Index and pairwise histograms of the rows qs against every SV (equal bit
     for bit), and n_cols sampled SV columns against the oracle."""
     from gkmsvm_amd import engine as E
-    from gkmsvm_amd.codec import encode_batch
-    from gkmsvm_amd.inputs import full_lengths
+    from gkmsvm_amd.codec import encode_strings
+    from gkmsvm_amd.inputs import row_lengths
 
     s = m._svs()
-    qc, ql, bad = encode_batch(qs)
+    qc, ql, bad = encode_strings(qs)
     assert not bad.any()
     xs = E.DeviceSeqs(qc, ql, gpu, rc=s.kp.rc)
     brute = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups).cpu().numpy()
@@ -53,7 +53,7 @@ def _histograms_both_paths(m, qs, gpu, n_cols, rng):
     assert brute[..., 0].sum() > 0  # exact matches among the pairs
     cols = np.sort(rng.choice(s.S, n_cols, replace=False))
     column = np.argsort(s.order)  # SV (input order) -> histogram column
-    svl = full_lengths(m.support_sequences, m.sv_lengths)
+    svl = row_lengths(m.support_sequences, m.sv_lengths)
     kern = O.Kern.make(m.kernel_type, m.kernel_params)
     oc, ol = O.batch(qs)
     want = O.pair_hist(kern, O.pack_both(kern, oc, ol),

@@ -55,10 +55,10 @@ def _model(kt, p, codes, lens, coef, bias=0.0):
 
 
 def _oracle_model(m):
-    from gkmsvm_amd.inputs import full_lengths
+    from gkmsvm_amd.inputs import row_lengths
 
     kern = O.Kern.make(m.kernel_type, m.kernel_params)
-    return O.Model(kern, m.support_sequences, full_lengths(m.support_sequences, m.sv_lengths),
+    return O.Model(kern, m.support_sequences, row_lengths(m.support_sequences, m.sv_lengths),
                    m.coefficients, m.bias)
 
 

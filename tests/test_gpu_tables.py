@@ -25,10 +25,10 @@ def _load(name, table=True, explain_table=True):
 
 
 def _oracle_model(m):
-    from gkmsvm_amd.inputs import full_lengths
+    from gkmsvm_amd.inputs import row_lengths
 
     kern = O.Kern.make(m.kernel_type, m.kernel_params)
-    return O.Model(kern, m.support_sequences, full_lengths(m.support_sequences, m.sv_lengths),
+    return O.Model(kern, m.support_sequences, row_lengths(m.support_sequences, m.sv_lengths),
                    m.coefficients, m.bias)
 
 

@@ -35,18 +35,44 @@ def test_codec_round_trip_and_n():
     np.testing.assert_array_equal(codes_to_one_hot(np.array([[3, 0]]))[0].argmax(0), [3, 0])
 
 
-def test_as_batch_lists_and_lengths():
-    from gkmsvm_amd.inputs import allele_arrays, as_batch, joint_lengths
+def test_read_batch_lists_lengths_and_masks():
+    """Batches of strings, one-hot and padded codes; validation errors;
+    allele pairs widened to one width; NaN rows and zeroed positions."""
+    import torch
 
-    codes, lens = as_batch(["ACG", "ACGTT", "A"])
-    assert codes.shape == (3, 5) and lens.tolist() == [3, 5, 1]
-    codes, lens = as_batch(np.zeros((2, 4, 7)))
-    assert codes.shape == (2, 7) and lens is None
+    from gkmsvm_amd.inputs import mask_positions, nan_rows, read_batch, read_pair, stacked
+
+    b = read_batch(["ACG", "ACGTN", "a"])
+    assert b.codes.shape == (3, 5) and b.lengths.tolist() == [3, 5, 1] and b.padded
+    assert b.codes[1].tolist() == [0, 1, 2, 3, 0]  # N read as A
+    b = read_batch(np.zeros((2, 4, 7)))
+    assert b.codes.shape == (2, 7) and not b.padded and b.given_lengths() is None
+    assert read_batch(np.zeros((2, 7), np.uint8), lengths=[7, 7]).given_lengths() is None
+    for bad in (dict(x=np.zeros((2, 7), np.uint8), lengths=[8, 1]),
+                dict(x=np.zeros((2, 7), np.uint8), lengths=[3]),
+                dict(x=["ACGT"], lengths=[4]), dict(x=[]), dict(x=["ACGT", ""]),
+                dict(x=["ACXT"])):
+        with pytest.raises(ValueError):
+            read_batch(**bad)
+    with pytest.warns(UserWarning, match="2047"):
+        read_batch(["A" * 2048])
+    r, a = read_pair(["ACGTA"], ["ACGT"])
+    assert r.codes.shape == a.codes.shape == (1, 5) and a.lengths.tolist() == [4] and a.padded
     with pytest.raises(ValueError):
-        as_batch(np.zeros((2, 7)), lengths=[8, 1])
-    r, a, rl, al = allele_arrays(["ACGTA"], ["ACGT"])
-    assert r.shape == a.shape == (1, 5) and al.tolist() == [4]
-    assert joint_lengths(5, (1, None), (1, np.array([3]))).tolist() == [5, 3]
+        read_pair(["ACGT", "ACGT"], ["ACGT"])
+    both = stacked(a, r)
+    assert both.given_lengths().tolist() == [4, 5]
+    with pytest.warns(UserWarning, match="fewer than l = 4"):
+        short = read_batch(["ACG", "ACGTA"]).short(4)
+    assert short.tolist() == [True, False]
+    out = torch.ones((2, 4, 5), dtype=torch.float64)
+    m = mask_positions(out, read_batch(["ACG", "ACGTA"]), short).numpy()
+    assert np.isnan(m[0]).all() and (m[1] == 1).all()
+    m = mask_positions(out, read_batch(["ACGTAC", "ACGTA"]).rows(1, 2), None).numpy()
+    assert (m[0, :, :5] == 1).all()
+    m = mask_positions(out[:1], read_batch(["ACGTAC", "AC"]).rows(1, 2), None).numpy()
+    assert (m[0, :, :2] == 1).all() and (m[0, :, 2:] == 0).all()
+    assert torch.isnan(nan_rows(torch.zeros((3, 1)), np.array([False, True, False]))[1, 0])
 
 
 # -- kernel tables -------------------------------------------------------------
@@ -131,6 +157,53 @@ def test_load_lsgkm_header_and_svs():
     np.testing.assert_array_equal(m.support_sequences, om.sv_codes)
     r = load_lsgkm_model(H.path("lsgkm_ragged_t4.model.txt.gz"), device="cpu")
     assert r.sv_lengths is not None and r.sv_lengths.min() < r.sv_lengths.max()
+
+
+def test_device_auto_without_gpu_holds_on_the_host():
+    """device="auto" (and the loaders' device argument) picks the ROCm GPU
+    when one is visible and otherwise holds the model on the host, as the
+    reference's "auto" falls back to the CPU; unknown names raise."""
+    import torch
+
+    from gkmsvm_amd import GkmSVM, load_lsgkm_model, load_model
+    from gkmsvm_amd import engine as E
+
+    m = load_lsgkm_model(H.path("lsgkm_wgkm_t2.model.txt.gz"), device="auto")
+    assert m.device == ("rocm" if torch.cuda.is_available() else "cpu")
+    assert m.num_support_vectors > 0
+    m2 = load_model(H.path("lsgkm_wgkm_t2.model.txt.gz"), device="auto")
+    assert m2.device == m.device
+    assert E.placement("cpu") is None
+    with pytest.raises(ValueError, match="unknown device"):
+        GkmSVM(np.zeros((2, 20), np.uint8), np.ones(2), 0.0, 2, {"L": 5, "k": 3}, device="tpu")
+
+
+def test_one_hot_api_helpers():
+    """encode_batch / one_hot_decode / reverse_complement / validate behave
+    as the reference's codec functions (codec.py:155-220)."""
+    import torch
+
+    from gkmsvm_amd import encode_batch, one_hot_decode, reverse_complement, validate
+
+    x, mask = encode_batch(["ACGTN", "GA"], allow_n=True, pad_value=-1.0)
+    assert x.shape == (2, 4, 5) and x.dtype == np.float32
+    assert mask.tolist() == [[True] * 5, [True, True, False, False, False]]
+    assert (x[0, :, 4] == 0).all() and (x[1, :, 2:] == -1).all()
+    with pytest.raises(ValueError):
+        encode_batch(["ACGN"])
+    with pytest.raises(ValueError):
+        encode_batch([])
+    assert one_hot_decode(x[0, :, :4]) == "ACGT"
+    rc = reverse_complement(x[:1, :, :4])
+    assert one_hot_decode(rc[0]) == "ACGT"  # ACGT is its own reverse complement
+    y, _ = encode_batch(["AACG"])
+    assert one_hot_decode(reverse_complement(y[0])) == "CGTT"
+    assert one_hot_decode(reverse_complement(torch.from_numpy(y))[0]) == "CGTT"
+    validate(y)
+    with pytest.raises(ValueError):
+        validate(x)  # the N column sums to 0
+    with pytest.raises(ValueError):
+        one_hot_decode(y)
 
 
 def test_cpu_model_refuses_to_score():

@@ -72,10 +72,10 @@ def test_table_scores_equal_per_sv_scores_fixtures(kind):
     from gkmsvm_amd import load_lsgkm_model
 
     m = load_lsgkm_model(H.path(f"lsgkm_wgkm_{kind}.model.txt.gz"), device="cpu")
-    from gkmsvm_amd.inputs import full_lengths
+    from gkmsvm_amd.inputs import row_lengths
 
     om = O.Model(O.Kern.make(m.kernel_type, m.kernel_params), m.support_sequences,
-                 full_lengths(m.support_sequences, m.sv_lengths), m.coefficients, m.bias)
+                 row_lengths(m.support_sequences, m.sv_lengths), m.coefficients, m.bias)
     recs = O.read_fasta(H.path("lsgkm_wgkm_test100.fa"))
     codes, lens = O.batch([s for _, s in recs])
     w = O.lmer_table(om)
