@@ -33,6 +33,8 @@
 #include <string.h>
 #include <math.h>
 
+#include <mutex>
+
 #include "gkm_core.h"
 #include "gkm_internal.h"
 #include "gkmhip.h"
@@ -1960,6 +1962,89 @@ GKM_API int gkm_index_reduce(const void *partial, int count_bytes, const int32_t
                            chunk_windows, nchunks, (unsigned long long *)hist);
     GKM_CHECK_LAUNCH("gkm_index_reduce");
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// gkm_index_hist without a workspace argument: the partial counts live in a
+// file-scope device array, one copy per GPU, which the host addresses only
+// through the pointer hipGetSymbolAddress gives on that GPU (the array's name
+// in host code is a host-side stand-in, not a device address).
+
+static __device__ __align__(256) uint8_t scratch[GKM_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];  // the array's address on each device, once looked up
+}  // namespace
+
+// The scratch array's device address on the current device (looked up on
+// that device's first call); 0 or a negative status.
+static int scratch_address(void **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return fail(-3, "gkm_index_hist_noalloc: no current device: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(-1, "gkm_index_hist_noalloc: device %d beyond the %d supported", dev,
+                    kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(-3, "gkm_index_hist_noalloc: hipGetSymbolAddress(scratch) failed on "
+                        "device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+GKM_API int64_t gkm_scratch_bytes(void) { return (int64_t)GKM_SCRATCH_BYTES; }
+
+GKM_API int64_t gkm_index_partial_bytes(int B, int Wx, int S, int nm, int n_tiles,
+                                        int chunk_windows, int weighted, int pack_counts)
+{
+    if (B <= 0 || Wx <= 0 || S <= 0 || nm <= 0 || chunk_windows < 1)
+        return 0;
+    const int64_t counts = (int64_t)((Wx + chunk_windows - 1) / chunk_windows) * B * S * nm;
+    if (weighted)
+        return counts * 4;  // uint32 partials (16-bit LDS counts or not)
+    if (pack_counts == 2)   // low bytes, high bytes, one flag per block
+        return 2 * counts + (int64_t)((Wx + chunk_windows - 1) / chunk_windows) * B * n_tiles;
+    return counts * (pack_counts == 1 ? 2 : 4);
+}
+
+GKM_API int gkm_index_hist_noalloc(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
+                                   int B, int Wx, const uint32_t *offsets, const uint16_t *ids,
+                                   const uint8_t *vals, const uint32_t *records,
+                                   int record_tables, const uint32_t *masks,
+                                   const uint8_t *levels, int n_masks, int l, int nm,
+                                   const uint8_t *wtab, int S, int tile, int n_tiles,
+                                   int chunk_windows, int pack_counts, int64_t col_min,
+                                   int64_t row_base, void *hist, void *stream)
+{
+    if (hist == nullptr)
+        return fail(-1, "gkm_index_hist_noalloc: hist is required (the partial counts live in "
+                        "the library's scratch and are summed within the call)");
+    if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
+        return 0;
+    const int64_t need = gkm_index_partial_bytes(B, Wx, S, nm, n_tiles, chunk_windows,
+                                                 vals != nullptr, pack_counts);
+    if (need > (int64_t)GKM_SCRATCH_BYTES)
+        return fail(-2, "gkm_index_hist_noalloc: the partial counts of this call take %lld "
+                        "bytes, over the %lld-byte scratch: use fewer rows or SVs, or larger "
+                        "chunk_windows (or gkm_index_hist with a workspace)",
+                    (long long)need, (long long)GKM_SCRATCH_BYTES);
+    void *partial = nullptr;
+    int rc = scratch_address(&partial);
+    if (rc)
+        return rc;
+    return gkm_index_hist(xwin, xpos, xlen, B, Wx, offsets, ids, vals, records, record_tables,
+                          masks, levels, n_masks, l, nm, wtab, S, tile, n_tiles, chunk_windows,
+                          pack_counts, col_min, row_base, partial, hist, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -607,49 +607,61 @@ def index_blocks(x_lengths, l: int, n_tiles: int = 1, chunk: int | None = None) 
     return per * max(1, -(-w // chunk))
 
 
+class IndexLaunch:
+    """The arguments of one gkm_index_hist call over rows x0..x1 (window
+    chunking, sorted query windows, partial-count format); `args` are its
+    C-ABI arguments up to (not including) partial."""
+
+    def __init__(self, kp, idx, xs, x0, x1, S, col_min, stream, target_blocks):
+        B = x1 - x0
+        if target_blocks is None:
+            target_blocks = index_blocks(xs.lengths[x0:x1], kp.l, idx.n_tiles)
+        fwd, _ = xs.windows(kp.l, stream)
+        Wx = fwd.shape[1]
+        wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
+        nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)),
+                             -(-wx_max // INDEX_CHUNK_MIN)))
+        cw = -(-wx_max // nchunks)
+        if idx.vals is not None:  # per-block uint32 counts must not overflow
+            cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
+            cw = max(1, min(cw, cap))
+        elif idx.pack16:  # nor the 16-bit ones: a window meets an SV window at most once
+            cw = max(1, min(cw, 65535 // max(1, idx.max_sv_windows)))
+        self.B, self.Wx, self.cw = B, Wx, cw
+        self.nchunks = -(-Wx // cw)
+        self.pack_code = {1: 0, 2: 1, 4: 2}[idx.pack]
+        # bytes, uint16 or uint32 partials (gkm_index_reduce's count_bytes)
+        self.count_bytes = 1 if idx.pack == 4 else 2 if idx.pack16 else 4
+        wt, _ = kp.wtab(xs.lmax)
+        # each row's windows in l-mer order (PAD last): a block's windows then
+        # share high digits, and so do their neighbours' offsets -- cache lines
+        # reused (32-bit keys: PAD, 0xFFFFFFFF, sorts last as INT32_MAX, >= 4^l)
+        keys = fwd[x0:x1]
+        keys = torch.where(keys < 0, torch.full_like(keys, 2**31 - 1), keys)
+        keys, pos = torch.sort(keys, dim=1)
+        self.swin = keys.contiguous()
+        self.spos = pos.to(torch.int32).contiguous() if wt is not None else None
+        self.args = (_p(self.swin), _p(self.spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
+                     _p(idx.ids), _p(idx.vals), _p(idx.records), idx.record_tables,
+                     _p(idx.masks), _p(idx.levels), idx.n_masks, kp.l, kp.nm, _p(wt), S,
+                     idx.tile, idx.n_tiles, cw, self.pack_code, col_min, x0)
+
+    def partial_bytes(self, kp, idx, S) -> int:
+        n_all = self.nchunks * self.B * S * kp.nm
+        if self.count_bytes == 1:
+            return 2 * n_all + self.nchunks * self.B * idx.n_tiles
+        return n_all * self.count_bytes
+
+
 def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
-    B = x1 - x0
-    if target_blocks is None:
-        target_blocks = index_blocks(xs.lengths[x0:x1], kp.l, idx.n_tiles)
-    fwd, _ = xs.windows(kp.l, stream)
-    Wx = fwd.shape[1]
-    wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
-    nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)), -(-wx_max // INDEX_CHUNK_MIN)))
-    cw = -(-wx_max // nchunks)
-    if idx.vals is not None:  # per-block uint32 counts must not overflow
-        cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
-        cw = max(1, min(cw, cap))
-    elif idx.pack16:  # nor the 16-bit ones: a window meets an SV window at most once
-        cw = max(1, min(cw, 65535 // max(1, idx.max_sv_windows)))
-    nchunks = -(-Wx // cw)
-    if idx.pack == 4:  # bytes: low, high (fallback blocks), block flags
-        n_all = nchunks * B * S * kp.nm
-        partial = torch.empty(2 * n_all + nchunks * B * idx.n_tiles, device=kp.device,
-                              dtype=torch.uint8)
-    else:
-        partial = torch.empty((nchunks, B, S, kp.nm), device=kp.device,
-                              dtype=torch.int16 if idx.pack16 else torch.int32)
-    wt, _ = kp.wtab(xs.lmax)
-    # each row's windows in l-mer order (PAD last): a block's windows then share
-    # high digits, and so do their neighbours' offsets -- cache lines reused
-    # (32-bit keys: PAD, 0xFFFFFFFF, sorts last as INT32_MAX, still >= 4^l)
-    keys = fwd[x0:x1]
-    keys = torch.where(keys < 0, torch.full_like(keys, 2**31 - 1), keys)
-    keys, pos = torch.sort(keys, dim=1)
-    swin = keys.contiguous()
-    spos = pos.to(torch.int32).contiguous() if wt is not None else None
+    L = IndexLaunch(kp, idx, xs, x0, x1, S, col_min, stream, target_blocks)
+    partial = torch.empty(L.partial_bytes(kp, idx, S), device=kp.device, dtype=torch.uint8)
     # the query (timed on its own) and the sum of its per-chunk partials
     _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
-           lambda: _lib.call(
-               "gkm_index_hist", _p(swin), _p(spos), _p(xs.len[x0:x1]), B, Wx, _p(idx.offsets),
-               _p(idx.ids), _p(idx.vals), _p(idx.records), idx.record_tables, _p(idx.masks),
-               _p(idx.levels),
-               idx.n_masks, kp.l,
-               kp.nm, _p(wt), S, idx.tile, idx.n_tiles, cw, {1: 0, 2: 1, 4: 2}[idx.pack], col_min, x0,
-               _p(partial), None, stream_handle(stream)))
-    fmt = 1 if idx.pack == 4 else 2 if idx.pack16 else 4  # bytes, uint16, uint32 partials
-    _lib.call("gkm_index_reduce", _p(partial), fmt, _p(xs.len[x0:x1]),
-              B, Wx, kp.l, kp.nm, S, idx.tile, cw, _p(out), stream_handle(stream))
+           lambda: _lib.call("gkm_index_hist", *L.args, _p(partial), None,
+                             stream_handle(stream)))
+    _lib.call("gkm_index_reduce", _p(partial), L.count_bytes, _p(xs.len[x0:x1]),
+              L.B, L.Wx, kp.l, kp.nm, S, idx.tile, L.cw, _p(out), stream_handle(stream))
     return out
 
 

@@ -219,6 +219,36 @@ int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xle
 int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B, int Wx,
                      int l, int nm, int S, int tile, int chunk_windows, void *hist, void *stream);
 
+/* ---- No-allocation variant --------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory or pass a workspace (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  gkm_index_hist_noalloc is
+ * gkm_index_hist with hist required and the partial counts kept in a
+ * GKM_SCRATCH_BYTES static device array that the library holds on each GPU
+ * it runs on until the process ends; it calls no allocator.  A call whose
+ * partials (gkm_index_partial_bytes) exceed the array fails with status -2
+ * before launching anything.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race, so
+ * serialise them.  The array needs no clearing: every partial a call reads
+ * was written by that call. */
+#define GKM_SCRATCH_BYTES (64u << 20)
+
+/* GKM_SCRATCH_BYTES. */
+int64_t gkm_scratch_bytes(void);
+
+/* Bytes of gkm_index_hist's partial counts for these arguments (weighted:
+ * vals != NULL). */
+int64_t gkm_index_partial_bytes(int B, int Wx, int S, int nm, int n_tiles, int chunk_windows,
+                                int weighted, int pack_counts);
+
+int gkm_index_hist_noalloc(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B,
+                           int Wx, const uint32_t *offsets, const uint16_t *ids,
+                           const uint8_t *vals, const uint32_t *records, int record_tables,
+                           const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                           int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                           int chunk_windows, int pack_counts, int64_t col_min, int64_t row_base,
+                           void *hist, void *stream);
+
 /* ---- Exact l-mer tables of non-RBF models (gkm_tables.hip) ----------------
  * V, table  [4^l] float64 indexed by the packed l-mer (base k in bits 2k).
  * A         [4^l][l][4] float64 explain table: window u, digit k, delta d.
