@@ -25,14 +25,19 @@ rates), served and plain.
 roofline: every launch of the hot kernels is timed with HIP events on the
 stream it runs on during the timed steps; the object describes the kernel
 with the most GPU time in the value run (the others under "others"; the
-table run's under config.served_lmer_tables.roofline).  Pairwise kernels
-(k_cross_hist, k_self_hist, k_sweep): window-pair comparisons x 6 integer
-operations (XOR, shift, OR, AND, popcount, accumulate) / launch time,
-against the int32 VALU peak.  HBM-bound kernels (k_index_tile and
-k_index_explain: 16 B per (window, mask) probe + 12 B per hit, a 4 B id and
-an 8 B FP64 read-modify-write; k_table_score: 4 B window + 8 B table entry per
-window; k_table_explain: 4 B window + 32 l B table row per window + 32 B
-output per position) / launch time, against 8 TB/s.
+table run's under config.served_lmer_tables.roofline).  achieved and frac
+are measured work per launch over the live mean launch time: HBM bytes
+(FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a device copy's counts,
+from rocprofv3 --pmc runs of this command's headline run, read from
+profiles/serving_72kSV_headline_pmc.txt) against 8 TB/s, or executed VALU
+lane operations (SQ_INSTS_VALU x 64) against the measured int32 VALU peak.
+The algorithmic figure of each kernel's unit is kept beside it under
+"algorithmic": index kernels 16 B per (window, mask) probe + 12 B per hit
+(a 4 B id and an 8 B FP64 read-modify-write, which these kernels keep in
+LDS, so it can exceed the peak), pairwise kernels 6 integer operations per
+window-pair comparison, k_table_score 4 B window + 8 B entry per window,
+k_table_explain 4 B window + 32 l B row per window + 32 B per output
+position.
 cpu_baseline: the reference's CPU algorithm on the host cores (the oracle, C
 with OpenMP + numpy) on a bounded sample of the stream's scoring queries: the
 l-mer index + fused reduce (kernels/lmer_index_cpu.py:70-120, what the
@@ -139,17 +144,37 @@ def run_served(server, reqs, rate, rng):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(sv, coef, reqs, seconds):
-    """The reference's CPU algorithm (oracle: C + OpenMP, numpy) on a sample of
-    the stream's scoring queries: query x SV evaluations per second through the
-    exact l-mer table, and through the l-mer index + fused reduce."""
+def host_cpu() -> dict:
+    """The host's CPU model and logical CPU counts (lscpu's model name)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "cpus_usable": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(sv, coef, step_reqs, seconds):
+    """The reference's CPU algorithm (the oracle's C + OpenMP restatement,
+    compiled here with -O3 -march=native) on the scoring queries of the
+    given steps: query x SV evaluations per second through the l-mer index +
+    fused reduce (kernels/lmer_index_cpu.py:70-120, what the reference picks
+    for these calls), a steady-state pass after a warm-up call; the exact
+    l-mer table rate beside it."""
     from oracle import oracle as O
 
+    O.use_native_build()
     om = O.Model(O.Kern.make(2, PARAMS), sv, np.full(len(coef), SV_LEN), coef, 0.0)
-    queries = [p[0] for k, p in reqs if k == "score"]
+    queries = [p[0] for reqs in step_reqs for k, p in reqs if k == "score"]
     nt = O.threads()
 
     def run(score_fn, budget):
+        score_fn(*O.batch(queries[:nt]))  # warm-up (first-touch of the index, page cache)
         done, lens, t0 = 0, [], time.perf_counter()
         for i in range(0, len(queries), nt):
             chunk = queries[i:i + nt]
@@ -165,22 +190,26 @@ def cpu_baseline(sv, coef, reqs, seconds):
     w = O.lmer_table(om)
     table_build_s = time.perf_counter() - t
     v_table, n_table, mean_table = run(lambda c, ln: O.score_with_table(om, w, c, ln),
-                                       seconds / 2)
+                                       seconds * 0.25)
     t = time.perf_counter()
     index = O.LmerIndex(om)
     index_build_s = time.perf_counter() - t
-    v_index, n_index, mean_index = run(index.score, seconds / 2)
+    v_index, n_index, mean_index = run(index.score, seconds * 0.75)
+    cpu = host_cpu()
     return {
         "value": v_index,
         "unit": "query x SV kernel evals/s",
         "cores": nt,
         "kind": "port",
-        "sample": (f"{n_index} scoring queries of the request stream (mean {mean_index:.0f} bp) "
-                   f"through the l-mer index + fused reduce (C/OpenMP; index build "
-                   f"{index_build_s:.1f} s, not counted); through the exact l-mer table "
-                   f"(numpy lookups, C/OpenMP self-kernel; table build {table_build_s:.1f} s by "
-                   f"masked marginals, not counted) {n_table} of them (mean {mean_table:.0f} bp) "
-                   f"score {v_table:.3g} evals/s"),
+        "cpu": cpu,
+        "sample": (f"{n_index} of the {len(queries)} scoring queries of {len(step_reqs)} timed "
+                   f"steps (mean {mean_index:.0f} bp) against the {len(coef):,} SVs, through the l-mer "
+                   f"index + fused reduce (oracle C/OpenMP, -O3 -march=native, {nt} threads on "
+                   f"{cpu['model']}, {cpu['logical_cpus']} logical CPUs; index build "
+                   f"{index_build_s:.1f} s, not counted); through the exact l-mer table (numpy "
+                   f"lookups, C/OpenMP self-kernel; table build {table_build_s:.1f} s by masked "
+                   f"marginals, not counted) {n_table} of them (mean {mean_table:.0f} bp) score "
+                   f"{v_table:.3g} evals/s"),
         "lmer_table_value": v_table,
     }
 
@@ -195,40 +224,66 @@ HBM_KINDS = {
     "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",
 }
 
-
-# HBM traffic per launch (bytes) of the serving run's per-SV kernels: mean per
-# timed dispatch of FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from device
-# copies of known size) in rocprofv3 --pmc runs of this benchmark's headline
-# serve run, as recorded in profiles/serving_72kSV_mix_trace.txt.
-PROFILED_TRAFFIC = {
-    "k_index_tile": (15_538_564 * 2 + 1_369_626) * 1024.0,
-    "k_index_explain": (2_409_884 * 2 + 701) * 1024.0,
-}
-TRAFFIC_SOURCE = "profiles/serving_72kSV_mix_trace.txt (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
+# Per-dispatch HBM bytes (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
+# device copy's counts) and VALU wave instructions (SQ_INSTS_VALU) of the
+# hot kernels, from rocprofv3 --pmc runs of this benchmark's headline run
+# (the command is written in the file).
+PROFILE = os.path.join(ROOT, "profiles", "serving_72kSV_headline_pmc.txt")
 
 
-def roofline_of(summary: dict) -> dict:
-    """The roofline object of the kernel with the most GPU time; the others as
-    "others"."""
+def _profile() -> dict:
+    from gkmsvm_amd.rocpd_summary import read_profile
+
+    if not os.path.exists(PROFILE):
+        return {"traffic": {}, "valu": {}}
+    return read_profile(PROFILE)
+
+
+def roofline_of(summary: dict, counters: dict | None = None) -> dict:
+    """The roofline object of the kernel with the most GPU time (the others as
+    "others").  achieved / frac come from the kernel's measured work per
+    launch over its live mean launch time: HBM bytes (the profile's
+    TRAFFIC line) against 8 TB/s, or executed VALU lane operations
+    (SQ_INSTS_VALU x 64) against the measured int32 VALU peak; the
+    algorithmic figure of the kernel's unit (16 B per probe + 12 B per hit;
+    6 int ops per window-pair comparison) is kept under "algorithmic".  A
+    kernel the profile does not cover is priced by its algorithmic figure
+    (basis says which)."""
     if not summary:
         return {}
+    counters = _profile() if counters is None else counters
+    src = os.path.relpath(PROFILE, ROOT)
     rows = {}
     for kind, d in summary.items():
-        per_s = d["units_per_launch"] / (d["mean_ms"] * 1e-3)
+        t = d["mean_ms"] * 1e-3
+        per_s = d["units_per_launch"] / t
+        r = {"kernel": kind, "launches": d["launches"], "mean_launch_ms": d["mean_ms"],
+             "total_ms": d["total_ms"]}
         if kind in HBM_KINDS:
-            r = {"bound": "hbm", "achieved": per_s / 1e9, "peak": HBM_PEAK / 1e9,
-                 "unit": "GB/s", "frac": per_s / HBM_PEAK, "work": HBM_KINDS[kind],
-                 "bytes_per_launch": d["units_per_launch"]}
+            alg = {"bytes_per_launch": d["units_per_launch"], "GBps": per_s / 1e9,
+                   "x_peak": per_s / HBM_PEAK, "work": HBM_KINDS[kind]}
+            traffic = counters["traffic"].get(kind)
+            achieved = traffic / t if traffic else per_s
+            r.update({"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                      "basis": (f"measured HBM bytes per launch ({src}) / live launch time"
+                                if traffic else "algorithmic bytes (no profile)"),
+                      "algorithmic": alg})
         else:
             ops = per_s * OPS_PER_COMPARISON
-            r = {"bound": "valu-int32", "achieved": ops / 1e12, "peak": INT_VALU_PEAK / 1e12,
-                 "unit": "TOP/s", "frac": ops / INT_VALU_PEAK, "comparisons_per_s": per_s,
-                 "comparisons_per_launch": d["units_per_launch"],
-                 "work": "6 int32 ops per window-pair comparison (XOR, SHR, OR, AND, POPC, add)"}
-        r.update({"kernel": kind, "launches": d["launches"], "mean_launch_ms": d["mean_ms"],
-                  "total_ms": d["total_ms"], "traffic": PROFILED_TRAFFIC.get(kind)})
-        if kind in PROFILED_TRAFFIC:
-            r["traffic_source"] = TRAFFIC_SOURCE
+            alg = {"comparisons_per_launch": d["units_per_launch"], "comparisons_per_s": per_s,
+                   "TOPs": ops / 1e12, "x_peak": ops / INT_VALU_PEAK,
+                   "work": "6 int32 ops per window-pair comparison (XOR, SHR, OR, AND, POPC, "
+                           "add); the bit-sliced kernels execute ~1.5"}
+            valu = counters["valu"].get(kind)
+            lane_ops = valu * 64 / t if valu else ops
+            r.update({"bound": "valu-int32", "achieved": lane_ops / 1e12,
+                      "peak": INT_VALU_PEAK / 1e12, "unit": "TOP/s",
+                      "frac": lane_ops / INT_VALU_PEAK, "traffic": counters["traffic"].get(kind),
+                      "basis": (f"SQ_INSTS_VALU x 64 lanes per launch ({src}) / live launch "
+                                "time, against the measured int32 VALU peak" if valu
+                                else "algorithmic ops (no profile)"),
+                      "algorithmic": alg})
         rows[kind] = r
     top = max(rows.values(), key=lambda r: r["total_ms"])
     out = dict(top)
@@ -481,7 +536,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.5,
                     help="arrival rate of the latency steps, as a fraction of the measured capacity")
     ap.add_argument("--lat-steps", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="budget of the CPU-baseline timing (index and table passes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tables", action="store_true",
@@ -654,7 +710,8 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sv, coef, steps[args.warmup], args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(sv, coef, steps[args.warmup:args.warmup + 3],
+                                            args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

@@ -34,6 +34,7 @@
 #include <math.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "gkm_core.h"
 #include "gkm_internal.h"
@@ -867,6 +868,146 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep(SweepArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Exact scoring of kernels without RBF.  A decision value is linear in the
+// mismatch histograms: f(x) = sum_m h[m] S_m / sqrt(K_raw(x, x)) + b with
+// S_m = sum_s w_s N_s[m] and w_s = coef_s / sqrt(K_raw(s, s)) (svm.py:43-61
+// normalises each pair by sqrt(dx ds), the same product).  The SV weights are
+// fixed point, wfix_s = w_s 2^shift rounded (|wfix_s| <= 2^61), and S_m is an
+// exact 128-bit integer sum, so a row's value does not depend on the order in
+// which its pairs, window chunks and SV tiles are added: the l-mer index with
+// any chunking, the pairwise kernel, a row alone or in any batch give the same
+// bits.  fixed_score is the one place the sums become a float64.
+// ---------------------------------------------------------------------------
+
+typedef __int128 i128;
+
+__device__ __forceinline__ double i128_to_double(i128 v)
+{
+    const long long hi = (long long)(v >> 64);
+    const unsigned long long lo = (unsigned long long)v;
+    return fma((double)hi, 18446744073709551616.0, (double)lo);
+}
+
+__device__ __forceinline__ i128 wave_sum_i128(i128 v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const long long hi = __shfl_xor((long long)(v >> 64), o, WAVE);
+        const unsigned long long lo = __shfl_xor((unsigned long long)v, o, WAVE);
+        v += ((i128)hi << 64) + (i128)lo;
+    }
+    return v;
+}
+
+__device__ __noinline__ double fixed_score(const i128 *S, int nm, const double *tab, int shift,
+                                           double dx, int normalize, double bias)
+{
+    double acc = 0.0;
+    for (int m = 0; m < nm; ++m)
+        acc = fma(tab[m], i128_to_double(S[m]), acc);
+    acc = ldexp(acc, -shift);
+    if (normalize)
+        acc = dx > 0.0 ? __ddiv_rn(acc, __dsqrt_rn(dx)) : 0.0;
+    return acc + bias;
+}
+
+// bsum[lv] += sum_{r < n} wfix[r] * count(r, lv) for lv < nm over a block (all
+// its threads call it), counts laid out [lv / PER][n] in 32-bit words of PER
+// counts (4: 8-bit, 2: 16-bit, 1: 32-bit).  One pass over the SVs per 4 levels.
+// Counts below 2^16 (PER 4, 2) go through float64 FMAs, exactly: wfix =
+// whi 2^31 + wlo with |whi| <= 2^30, wlo < 2^31, so each product with a count is an
+// integer below 2^47 and a thread's at most 64 of them sum below 2^53 without
+// rounding; the two sums become int64 and then the 128-bit level sum.  32-bit
+// counts take 32 x 32 -> 64-bit integer products (|wfix >> 32| <= 2^29).  It runs once per block after
+// the walk, whose registers it reuses.  U: SV weights loaded per round.
+template <int U, int PER>
+__device__ __forceinline__ void fold_level_sums(const uint32_t *cnt, int n,
+                                                const long long *wfix, int nm, i128 *bsum,
+                                                i128 *red)
+{
+    constexpr int bits = 32 / PER;
+    constexpr uint32_t mask = PER == 1 ? 0xFFFFFFFFu : (1u << bits) - 1u;
+    constexpr bool fp = PER > 1;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    for (int l0 = 0; l0 < nm; l0 += 4) {
+        i128 acc[4] = {0, 0, 0, 0};
+        double fhi[4] = {0.0, 0.0, 0.0, 0.0}, flo[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int r0 = threadIdx.x; r0 < n; r0 += U * (int)blockDim.x) {
+            long long wv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = r0 + u * (int)blockDim.x;
+                wv[u] = wfix[r < n ? r : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = r0 + u * (int)blockDim.x;
+                if (r >= n)
+                    break;
+                const long long w = wv[u];
+                if constexpr (fp) {
+                    const double dhi = (double)(int)(w >> 31);
+                    const double dlo = (double)(uint32_t)(w & 0x7FFFFFFFll);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int lv = l0 + k;
+                        if (lv < nm) {
+                            const double c = (double)((cnt[(lv / PER) * n + r] >>
+                                                       (bits * (lv % PER))) & mask);
+                            fhi[k] = fma(dhi, c, fhi[k]);
+                            flo[k] = fma(dlo, c, flo[k]);
+                        }
+                    }
+                } else {
+                    const uint32_t wlo = (uint32_t)w;
+                    const long long whi = w >> 32;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int lv = l0 + k;
+                        if (lv < nm) {
+                            const uint32_t c = cnt[lv * n + r];
+                            const unsigned long long plo = (unsigned long long)wlo * c;
+                            const long long phi = whi * (long long)c;
+                            acc[k] += ((i128)phi << 32) + (i128)plo;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (l0 + k < nm) {
+                i128 v = acc[k];
+                if constexpr (fp)
+                    v = ((i128)(long long)fhi[k] << 31) + (i128)(long long)flo[k];
+                v = wave_sum_i128(v);
+                if (lane == 0)
+                    red[wave] = v;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    i128 t = bsum[l0 + k];
+                    for (int q = 0; q < (int)(blockDim.x / WAVE); ++q)
+                        t += red[q];
+                    bsum[l0 + k] = t;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// The same, out of line: for the kernel variants where two inlined copies
+// would add to the walk's register pressure (weighted 16-bit counts).
+template <int PER>
+__device__ __noinline__ void fold_level_sums_call(const uint32_t *cnt, int n,
+                                                  const long long *wfix, int nm, i128 *bsum,
+                                                  i128 *red)
+{
+    fold_level_sums<2, PER>(cnt, n, wfix, nm, bsum, red);
+}
+
+// ---------------------------------------------------------------------------
 // k_index_tile: the same histograms through an l-mer index of the SV windows
 // ---------------------------------------------------------------------------
 
@@ -927,7 +1068,9 @@ struct IndexArgs {
     const uint8_t *levels;   // [n_masks] mismatch count of each mask | its record table << 4
     int record_tables;       // tables in records (table q: digits 0 and q of the key swapped)
     const uint8_t *wtab;     // weight by distance, or NULL
-    void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16)
+    void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16),
+                             // or with wfix [nchunks][B][n_tiles][nm] int128 level sums
+    const long long *wfix;   // [S] fixed-point SV weights (exact scoring), or NULL
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
     int sub_tile;            // PACK 4: SVs per pass of the 16-bit walk
     int64_t col_min, row_base;
@@ -990,7 +1133,28 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     const int ncnt = ns * wps;
     for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS)
         cnt[i] = 0u;
+    // exact scoring (wfix): the block's level sums S_m over its tile's SVs
+    __shared__ i128 bsum[16], red128[INDEX_THREADS / WAVE];
+    if (threadIdx.x < 16)
+        bsum[threadIdx.x] = 0;
     __syncthreads();
+    // bsum[lv] += sum over SVs s0 .. s0 + n - 1 of the tile of wfix times the
+    // count of level lv, counts laid out [lv / per][n] with per counts per word
+    auto fold = [&](int n, int s0, auto per) {  // per: std::integral_constant
+        constexpr int PER = decltype(per)::value;
+        if constexpr (W16)
+            fold_level_sums_call<PER>(cnt, n, a.wfix + s_base + s0, a.nm, bsum, red128);
+        else
+            fold_level_sums<12, PER>(cnt, n, a.wfix + s_base + s0, a.nm, bsum, red128);
+    };
+    using per4 = std::integral_constant<int, 4>;
+    using per2 = std::integral_constant<int, 2>;
+    using per1 = std::integral_constant<int, 1>;
+    auto emit = [&]() {  // the level sums out: [nchunks][B][n_tiles][nm]
+        if (threadIdx.x < a.nm)
+            ((i128 *)a.partial)[(((int64_t)chunk * a.B + b) * a.n_tiles + tile_id) * a.nm +
+                                threadIdx.x] = bsum[threadIdx.x];
+    };
     const uint32_t nl = 1u << (2 * a.l);
     const int i1 = min(wx, i0 + a.chunk_windows);
     const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
@@ -1281,6 +1445,11 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
             carried += red64[k];
         uint32_t *dst = (uint32_t *)a.partial + off;
         if (carried == 0) {
+            if (a.wfix) {
+                fold(ns, 0, per2{});
+                emit();
+                return;
+            }
             for (int i = threadIdx.x; i < ns * a.nm; i += INDEX_THREADS) {
                 const int sid = i / a.nm, lv = i - sid * a.nm;
                 dst[i] = (cnt[(lv >> 1) * ns + sid] >> (16 * (lv & 1))) & 0xFFFFu;
@@ -1301,11 +1470,17 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 atomicAdd(&cnt[lv * n + r], add);
             });
             __syncthreads();
+            if (a.wfix) {
+                fold(n, s0, per1{});
+                continue;
+            }
             for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS) {
                 const int sid = i / a.nm, lv = i - sid * a.nm;
                 dst[(int64_t)s0 * a.nm + i] = cnt[lv * n + sid];
             }
         }
+        if (a.wfix)
+            emit();
         return;
     } else if constexpr (PACK == 4) {
         uint32_t adds = 0;
@@ -1341,6 +1516,11 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
         uint8_t *hi = lo + n_all;
         uint8_t *flag = (uint8_t *)a.partial + 2 * n_all +
                         ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id;
+        if (carried == 0 && a.wfix) {
+            fold(ns, 0, per4{});
+            emit();
+            return;
+        }
         if (carried == 0) {
             if (a.nm == 4) {
                 // the word of an SV is its 4 counts in level order: a plain copy
@@ -1372,6 +1552,10 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 atomicAdd(&cnt[(lv >> 1) * n + r], 1u << (16 * (lv & 1)));
             });
             __syncthreads();
+            if (a.wfix) {
+                fold(n, s0, per2{});
+                continue;
+            }
             for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS) {
                 const int sid = i / a.nm, lv = i - sid * a.nm;
                 const uint32_t c = (cnt[(lv >> 1) * n + sid] >> (16 * (lv & 1))) & 0xFFFFu;
@@ -1379,6 +1563,10 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 lo[e] = (uint8_t)c;
                 hi[e] = (uint8_t)(c >> 8);
             }
+        }
+        if (a.wfix) {
+            emit();
+            return;
         }
         if (threadIdx.x == 0)
             *flag = 1;
@@ -1394,7 +1582,10 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 atomicAdd(&cnt[lv * ns + sid], add);
         });
         __syncthreads();
-        if (PACK == 2) {
+        if (a.wfix) {
+            fold(ns, 0, std::integral_constant<int, PACK == 2 ? 2 : 1>{});
+            emit();
+        } else if (PACK == 2) {
             write16(0, ns);
         } else {
             uint32_t *dst = (uint32_t *)a.partial + off;
@@ -1837,40 +2028,46 @@ __global__ __launch_bounds__(256) void k_index_reduce8(const uint8_t *__restrict
             hist[i + k] += v[k];
 }
 
-GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
-                           const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
-                           const uint32_t *records, int record_tables, const uint32_t *masks,
-                           const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
-                           int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
-                           int64_t col_min, int64_t row_base, void *partial, void *hist,
-                           void *stream)
+// Launch k_index_tile (shared by gkm_index_hist and gkm_index_score): wfix NULL
+// writes the per-chunk counts to `partial`, else the blocks' exact level sums.
+// *count_bytes: the format of those counts (1, 2 or 4) for gkm_index_reduce.
+static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_t *xpos,
+                             const int32_t *xlen, int B, int Wx, const uint32_t *offsets,
+                             const uint16_t *ids, const uint8_t *vals, const uint32_t *records,
+                             int record_tables, const uint32_t *masks, const uint8_t *levels,
+                             int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile,
+                             int n_tiles, int chunk_windows, int pack_counts, int64_t col_min,
+                             int64_t row_base, void *partial, const long long *wfix,
+                             int *count_bytes, void *stream)
 {
     int rc = check_l_nm(l, nm);
     if (rc)
         return rc;
     if (l > 13)
-        return fail(-1, "gkm_index_hist: the l-mer index needs l <= 13, got %d", l);
+        return fail(-1, "%s: the l-mer index needs l <= 13, got %d", who, l);
     if (tile < 1 || tile > 65536 || n_tiles != (S + tile - 1) / tile || chunk_windows < 1)
-        return fail(-1, "gkm_index_hist: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", tile,
+        return fail(-1, "%s: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", who, tile,
                     n_tiles, S, chunk_windows);
     if (B > 65535)
-        return fail(-1, "gkm_index_hist: at most 65535 rows per call, got %d", B);
-    if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
-        return 0;
+        return fail(-1, "%s: at most 65535 rows per call, got %d", who, B);
     if (pack_counts < 0 || pack_counts > 2)
-        return fail(-1, "gkm_index_hist: pack_counts must be 0, 1 or 2, got %d", pack_counts);
+        return fail(-1, "%s: pack_counts must be 0, 1 or 2, got %d", who, pack_counts);
     if (records != nullptr && (record_tables < 1 || record_tables > l))
-        return fail(-1, "gkm_index_hist: record_tables must be 1..l (%d), got %d", l,
-                    record_tables);
+        return fail(-1, "%s: record_tables must be 1..l (%d), got %d", who, l, record_tables);
+    if (wfix != nullptr && nm > 16)
+        return fail(-1, "%s: exact scoring takes at most 16 levels, got %d", who, nm);
     // counts per LDS word: 1 (uint32), 2 (16-bit), 4 (8-bit); weighted
     // indexes take 16-bit counts (with the carry check) when pack_counts is 1
     const bool w16 = vals != nullptr && pack_counts == 1;
     const int pack = vals != nullptr ? (w16 ? 2 : 1) : pack_counts == 2 ? 4 : pack_counts == 1 ? 2 : 1;
     const bool pack16 = pack > 1 && !w16;  // uint16 partials
+    *count_bytes = pack == 4 ? 1 : pack16 ? 2 : 4;
+    if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
+        return 0;
     const size_t words = (size_t)tile * ((nm + pack - 1) / pack);
     size_t lds = words * sizeof(uint32_t);
-    if (lds > 160 * 1024)
-        return fail(-2, "gkm_index_hist: a tile of %d SVs x %d levels exceeds LDS", tile, nm);
+    if (lds > 159 * 1024)
+        return fail(-2, "%s: a tile of %d SVs x %d levels exceeds LDS", who, tile, nm);
     IndexArgs a;
     a.xwin = xwin;
     a.xpos = xpos;
@@ -1883,6 +2080,7 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
     a.record_tables = records != nullptr ? record_tables : 1;
     a.wtab = wtab;
     a.partial = partial;
+    a.wfix = wfix;
     a.B = B;
     a.Wx = Wx;
     a.n_masks = n_masks;
@@ -1910,11 +2108,143 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
     dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
     hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
-    GKM_CHECK_LAUNCH("gkm_index_hist");
-    if (hist == nullptr)
-        return 0;  // the caller sums the partials with gkm_index_reduce
-    return gkm_index_reduce(partial, pack == 4 ? 1 : pack16 ? 2 : 4, xlen, B, Wx, l, nm, S, tile,
-                            chunk_windows, hist, stream);
+    GKM_CHECK_LAUNCH(who);
+    return 0;
+}
+
+GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
+                           const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                           const uint32_t *records, int record_tables, const uint32_t *masks,
+                           const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                           int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
+                           int64_t col_min, int64_t row_base, void *partial, void *hist,
+                           void *stream)
+{
+    int count_bytes = 4;
+    int rc = launch_index_tile("gkm_index_hist", xwin, xpos, xlen, B, Wx, offsets, ids, vals,
+                               records, record_tables, masks, levels, n_masks, l, nm, wtab, S,
+                               tile, n_tiles, chunk_windows, pack_counts, col_min, row_base,
+                               partial, nullptr, &count_bytes, stream);
+    if (rc || hist == nullptr || B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
+        return rc;  // hist NULL: the caller sums the partials with gkm_index_reduce
+    return gkm_index_reduce(partial, count_bytes, xlen, B, Wx, l, nm, S, tile, chunk_windows,
+                            hist, stream);
+}
+
+// out[b] = the row's decision value from the level sums of its blocks (the
+// chunks that hold its windows, every SV tile).
+__global__ __launch_bounds__(WAVE) void k_index_score_reduce(
+    const i128 *__restrict__ psum, int B, int n_tiles, int nm, int nchunks,
+    const int32_t *__restrict__ xlen, int l, int chunk_windows, const double *__restrict__ tab,
+    const double *__restrict__ dx, int shift, int normalize, double bias,
+    double *__restrict__ out)
+{
+    __shared__ i128 sums[16];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int wx = xlen[b] - l + 1;
+    const int u = wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
+    for (int lv = 0; lv < nm; ++lv) {
+        i128 acc = 0;
+        for (int k = lane; k < u * n_tiles; k += WAVE) {
+            const int c = k / n_tiles, t = k - c * n_tiles;
+            acc += psum[(((int64_t)c * B + b) * n_tiles + t) * nm + lv];
+        }
+        acc = wave_sum_i128(acc);
+        if (lane == 0)
+            sums[lv] = acc;
+    }
+    if (lane == 0)
+        out[b] = fixed_score(sums, nm, tab, shift, dx ? dx[b] : 1.0, normalize, bias);
+}
+
+// out[b] = the row's decision value from full histograms [B][S][nm] (the
+// pairwise kernel's): the same exact level sums, then fixed_score.
+__global__ __launch_bounds__(256) void k_combine_fixed(
+    const unsigned long long *__restrict__ hist, int B, int S, int nm,
+    const double *__restrict__ tab, const double *__restrict__ dx,
+    const long long *__restrict__ wfix, int shift, int normalize, double bias,
+    double *__restrict__ out)
+{
+    __shared__ i128 red[256 / WAVE], sums[16];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    const unsigned long long *h = hist + (int64_t)b * S * nm;
+    for (int lv = 0; lv < nm; ++lv) {
+        i128 acc = 0;
+        for (int sv = threadIdx.x; sv < S; sv += 256)
+            acc += (i128)wfix[sv] * (i128)h[(int64_t)sv * nm + lv];
+        acc = wave_sum_i128(acc);
+        if (lane == 0)
+            red[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            i128 t = 0;
+#pragma unroll
+            for (int k = 0; k < 256 / WAVE; ++k)
+                t += red[k];
+            sums[lv] = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[b] = fixed_score(sums, nm, tab, shift, dx ? dx[b] : 1.0, normalize, bias);
+}
+
+GKM_API int gkm_combine_fixed(const void *hist, int B, int S, int nm, const double *tab,
+                              const double *dx, const long long *wfix, int shift, int normalize,
+                              double bias, double *out, void *stream)
+{
+    if (nm < 1 || nm > 16)
+        return fail(-1, "gkm_combine_fixed: 1..16 levels, got %d", nm);
+    if (normalize && dx == nullptr)
+        return fail(-1, "gkm_combine_fixed: normalize needs dx");
+    if (B <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_combine_fixed, dim3((unsigned)B), dim3(256), 0, S_(stream),
+                       (const unsigned long long *)hist, B, S, nm, tab, dx, wfix, shift,
+                       normalize, bias, out);
+    GKM_CHECK_LAUNCH("gkm_combine_fixed");
+    return 0;
+}
+
+GKM_API int64_t gkm_index_psum_bytes(int B, int Wx, int nm, int n_tiles, int chunk_windows)
+{
+    if (B <= 0 || Wx <= 0 || nm <= 0 || chunk_windows < 1)
+        return 0;
+    return (int64_t)((Wx + chunk_windows - 1) / chunk_windows) * B * n_tiles * nm * 16;
+}
+
+GKM_API int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B,
+                            int Wx, const uint32_t *offsets, const uint16_t *ids,
+                            const uint8_t *vals, const uint32_t *records, int record_tables,
+                            const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                            int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                            int chunk_windows, int pack_counts, const long long *wfix,
+                            void *psum, const double *tab, const double *dx, int shift,
+                            int normalize, double bias, double *out, void *stream)
+{
+    if (wfix == nullptr || psum == nullptr || out == nullptr || tab == nullptr)
+        return fail(-1, "gkm_index_score: wfix, psum, tab and out are required");
+    if (normalize && dx == nullptr)
+        return fail(-1, "gkm_index_score: normalize needs dx");
+    if ((uintptr_t)psum % 16)
+        return fail(-1, "gkm_index_score: psum must be 16-byte aligned");
+    int count_bytes = 4;
+    int rc = launch_index_tile("gkm_index_score", xwin, xpos, xlen, B, Wx, offsets, ids, vals,
+                               records, record_tables, masks, levels, n_masks, l, nm, wtab, S,
+                               tile, n_tiles, chunk_windows, pack_counts, -1, 0, psum, wfix,
+                               &count_bytes, stream);
+    if (rc || B <= 0)
+        return rc;
+    // rows with no probe (Wx, S or n_masks 0) get the value of empty sums
+    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
+    hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
+                       (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
+                       dx, shift, normalize, bias, out);
+    GKM_CHECK_LAUNCH("gkm_index_score");
+    return 0;
 }
 
 GKM_API int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B,

@@ -41,6 +41,10 @@ SIGNATURES = {
     "gkm_index_hist": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, I64, I64,
                        P, P, P],
     "gkm_index_reduce": [P, I, P, I, I, I, I, I, I, I, P, P],
+    "gkm_combine_fixed": [P, I, I, I, P, P, P, I, I, D, P, P],
+    "gkm_index_psum_bytes": [I, I, I, I, I],
+    "gkm_index_score": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P, P, P,
+                        P, I, I, D, P, P],
     "gkm_scratch_bytes": [],
     "gkm_index_partial_bytes": [I, I, I, I, I, I, I, I],
     "gkm_index_hist_noalloc": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I,
@@ -60,7 +64,7 @@ SIGNATURES = {
     "gkm_peak_valu": [I, P, I, P],
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p, "gkm_scratch_bytes": ctypes.c_int64,
-            "gkm_index_partial_bytes": ctypes.c_int64}
+            "gkm_index_partial_bytes": ctypes.c_int64, "gkm_index_psum_bytes": ctypes.c_int64}
 
 _lib = None
 

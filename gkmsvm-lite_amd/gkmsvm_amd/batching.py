@@ -61,15 +61,22 @@ def _score_step(sv, kp, codes_t, len_t, lengths, out, plan: _Plan):
     if plan.table is not None:
         return E.table_rows(kp, xs, plan.table, 0.0, out)
     dx = E.self_kraw(kp, xs) if kp.normalize else None
+    if plan.use_index and sv.wfix is not None:  # exact level sums from the index itself
+        return E.index_score(sv, xs, 0, xs.n, dx, 0.0, out, target_blocks=plan.target_blocks)
     hist = torch.empty((xs.n, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
     if plan.use_index:
         E.index_hist(kp, sv.index(), xs, 0, xs.n, sv.S, out=hist,
                      target_blocks=plan.target_blocks)
     else:
         E.cross_hist(kp, xs, 0, xs.n, sv.seqs, sv.groups, out=hist, shape=plan.shape)
-    _lib.call("gkm_combine_score", E._p(hist), xs.n, sv.S, kp.nm, E._p(kp.tab),
-              E._p(dx) if dx is not None else None, E._p(sv.ds), E._p(sv.coef),
-              int(kp.normalize), int(kp.rbf), kp.gamma, 0.0, E._p(out), E.stream_handle())
+    if sv.wfix is not None:
+        _lib.call("gkm_combine_fixed", E._p(hist), xs.n, sv.S, kp.nm, E._p(kp.tab),
+                  E._p(dx) if dx is not None else None, E._p(sv.wfix), sv.fix_shift,
+                  int(kp.normalize), 0.0, E._p(out), E.stream_handle())
+    else:
+        _lib.call("gkm_combine_score", E._p(hist), xs.n, sv.S, kp.nm, E._p(kp.tab),
+                  E._p(dx) if dx is not None else None, E._p(sv.ds), E._p(sv.coef),
+                  int(kp.normalize), int(kp.rbf), kp.gamma, 0.0, E._p(out), E.stream_handle())
     return out
 
 

@@ -696,6 +696,9 @@ class SVModel:
             self.ds = self_kraw(kp, self.seqs, stream)
         else:
             self.ds = torch.ones(self.S, dtype=torch.float64, device=kp.device)
+        # exact scoring of kernels without RBF: fixed-point SV weights
+        self.wfix, self.fix_shift = (None, 0) if kp.rbf else fixed_weights(
+            coef[order], self.ds.cpu().numpy() if kp.normalize else None, kp.device)
 
         self._index = None
         # exact l-mer tables of non-RBF models (tables.py), built on demand:
@@ -757,6 +760,43 @@ class SVModel:
         return sum(x.numel() * x.element_size() for x in t)
 
 
+FIX_BITS = 61  # the largest |fixed-point SV weight| is 2^61
+
+
+def fixed_weights(coef: np.ndarray, ds: np.ndarray | None, device):
+    """(wfix int64 [S] on the device, shift): the SV weights of exact scoring,
+    w_s = coef_s / sqrt(K_raw(s, s)) (coef_s without normalisation; 0 for an
+    SV without windows), as round(w_s 2^shift) with the largest at 2^61."""
+    w = np.asarray(coef, dtype=np.float64).copy()
+    if ds is not None:
+        ds = np.asarray(ds, dtype=np.float64)
+        w = np.where(ds > 0, w / np.sqrt(np.where(ds > 0, ds, 1.0)), 0.0)
+    top = float(np.abs(w).max()) if w.size else 0.0
+    shift = 0 if top == 0.0 else FIX_BITS - int(np.ceil(np.log2(top)))
+    wfix = np.rint(np.ldexp(w, shift)).astype(np.int64)
+    return torch.from_numpy(wfix).to(device), int(shift)
+
+
+def index_score(sv: SVModel, xs: DeviceSeqs, r0: int, r1: int, dx, bias: float, out: torch.Tensor,
+                stream=None, target_blocks: int | None = None) -> torch.Tensor:
+    """out[r0:r1] = the rows' decision values straight from the l-mer index
+    (gkm_index_score: each block reduces its counts to exact level sums; no
+    histogram leaves the chip).  Kernels without RBF only."""
+    kp, idx = sv.kp, sv.index()
+    step = 65535
+    for a in range(r0, r1, step):
+        b = min(r1, a + step)
+        L = IndexLaunch(kp, idx, xs, a, b, sv.S, -1, stream, target_blocks)
+        psum = torch.empty(L.nchunks * L.B * idx.n_tiles * kp.nm * 16, dtype=torch.uint8,
+                           device=kp.device)
+        _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths[a:b]), stream,
+               lambda: _lib.call(
+                   "gkm_index_score", *L.args[:-2], _p(sv.wfix), _p(psum), _p(kp.tab),
+                   _p(dx[a:b]) if dx is not None else None, sv.fix_shift, int(kp.normalize),
+                   float(bias), _p(out[a:b]), stream_handle(stream)))
+    return out
+
+
 def rows_per_chunk(S: int, nm: int, cap: int = MAX_HIST_BYTES) -> int:
     return max(1, cap // max(1, S * nm * 8))
 
@@ -794,6 +834,8 @@ def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stre
     if table is not None:
         return table_rows(kp, xs, table, bias, out, stream=stream)
     dx = self_kraw(kp, xs, stream) if kp.normalize else None
+    if sv.wfix is not None and sv.use_index_for(xs.lengths, use_index):
+        return index_score(sv, xs, 0, xs.n, dx, bias, out, stream=stream)
     step = rows_per_chunk(sv.S, kp.nm)
     hist = None
     for r0 in range(0, xs.n, step):
@@ -801,8 +843,13 @@ def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stre
         if hist is None or hist.shape[0] != r1 - r0:
             hist = torch.empty((r1 - r0, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
         sv_hist(sv, xs, r0, r1, out=hist, stream=stream, shape=shape, use_index=use_index)
-        _lib.call("gkm_combine_score", _p(hist), r1 - r0, sv.S, kp.nm, _p(kp.tab),
-                  _p(dx[r0:r1]) if dx is not None else None, _p(sv.ds), _p(sv.coef),
-                  int(kp.normalize), int(kp.rbf), kp.gamma, float(bias), _p(out[r0:r1]),
-                  stream_handle(stream))
+        if sv.wfix is not None:  # exact level sums, the same bits as index_score
+            _lib.call("gkm_combine_fixed", _p(hist), r1 - r0, sv.S, kp.nm, _p(kp.tab),
+                      _p(dx[r0:r1]) if dx is not None else None, _p(sv.wfix), sv.fix_shift,
+                      int(kp.normalize), float(bias), _p(out[r0:r1]), stream_handle(stream))
+        else:
+            _lib.call("gkm_combine_score", _p(hist), r1 - r0, sv.S, kp.nm, _p(kp.tab),
+                      _p(dx[r0:r1]) if dx is not None else None, _p(sv.ds), _p(sv.coef),
+                      int(kp.normalize), int(kp.rbf), kp.gamma, float(bias), _p(out[r0:r1]),
+                      stream_handle(stream))
     return out

@@ -219,6 +219,40 @@ int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int32_t *xle
 int gkm_index_reduce(const void *partial, int count_bytes, const int32_t *xlen, int B, int Wx,
                      int l, int nm, int S, int tile, int chunk_windows, void *hist, void *stream);
 
+/* ---- Exact scoring of kernels without RBF -----------------------------------
+ * The decision value is linear in the histograms: out[b] = 2^-shift
+ * sum_m tab[m] S_m / sqrt(dx[b]) + bias (no division without normalize; 0
+ * where dx[b] <= 0), S_m = sum_s wfix[s] N_bs[m] summed exactly in 128-bit
+ * integers, wfix[s] = round(coef_s / sqrt(K_raw(s, s)) * 2^shift) (coef_s
+ * without normalize; 0 for an SV without windows), |wfix[s]| <= 2^61.  The
+ * sums do not depend on the order of the pairs, window chunks or SV tiles
+ * they are made of, so every path and launch shape gives the same bits.
+ * svm.py:43-61 (_normalized_dot), svm.py:368-405 (_sv_scores). */
+
+/* From full histograms hist [B][S][nm] uint64 (gkm_cross_hist or
+ * gkm_index_reduce), nm <= 16. */
+int gkm_combine_fixed(const void *hist, int B, int S, int nm, const double *tab, const double *dx,
+                      const long long *wfix, int shift, int normalize, double bias, double *out,
+                      void *stream);
+
+/* Bytes of gkm_index_score's psum workspace: one int128 per (window chunk,
+ * row, SV tile, level). */
+int64_t gkm_index_psum_bytes(int B, int Wx, int nm, int n_tiles, int chunk_windows);
+
+/* Decision values straight from the l-mer index (gkm_index_hist's arguments,
+ * without col_min): each block reduces its tile's counts to the exact level
+ * sums of its window chunk in psum (caller scratch of gkm_index_psum_bytes,
+ * 16-byte aligned), and a second kernel adds them per row -- no histogram
+ * or partial counts leave the chip.  LmerIndex.reduce
+ * (kernels/lmer_index.py:275-298; kernels/lmer_index_cpu.py:70-120). */
+int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B, int Wx,
+                    const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                    const uint32_t *records, int record_tables, const uint32_t *masks,
+                    const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                    int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
+                    const long long *wfix, void *psum, const double *tab, const double *dx,
+                    int shift, int normalize, double bias, double *out, void *stream);
+
 /* ---- No-allocation variant --------------------------------------------------
  * No counterpart in the reference: for callers that cannot allocate device
  * memory or pass a workspace (code under HIP graph capture, tight serving

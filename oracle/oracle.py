@@ -43,33 +43,51 @@ def build() -> str:
     return _SO
 
 
+def use_native_build() -> str:
+    """Compile gkm_oracle.c for this host's CPU (-O3 -march=native, OpenMP)
+    into a temporary directory and route every oracle call through it; for
+    timing the CPU baseline on the machine it runs on.  Returns the .so path."""
+    global _lib
+    import tempfile
+
+    out = os.path.join(tempfile.mkdtemp(prefix="gkm_oracle_native_"), "libgkm_oracle.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fPIC", "-fopenmp", "-std=c11", "-shared",
+                    "-o", out, os.path.join(_HERE, "gkm_oracle.c"), "-lm"], check=True)
+    _lib = _configure(ctypes.CDLL(out))
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_SO):
             build()
-        L = ctypes.CDLL(_SO)
-        P = ctypes.c_void_p
-        I64 = ctypes.c_int64
-        I = ctypes.c_int
-        D = ctypes.c_double
-        L.orc_pack.argtypes = [P, P, I64, I64, I, P]
-        L.orc_rc_windows.argtypes = [P, I64, I64, I, P]
-        L.orc_pair_hist.argtypes = [P, P, I64, I64, P, P, P, P, I64, I64, I, P]
-        L.orc_self_hist.argtypes = [P, P, P, P, I64, I64, I, P]
-        L.orc_explain.argtypes = [P, I64, I64, P, I64, I64, P, P, P, I, I, I64, I,
-                                  P, P, P]
-        L.orc_neighbor_count.argtypes = [I, I]
-        L.orc_neighbor_count.restype = I64
-        L.orc_neighbor_masks.argtypes = [I, I, P, P, P]
-        L.orc_neighbor_masks.restype = I64
-        L.orc_index_build.argtypes = [P, P, P, P, I64, I64, I, P, P, P]
-        L.orc_index_build.restype = I64
-        L.orc_index_reduce.argtypes = [P, P, I64, I64, P, P, P, P, P, I64, I, P, P,
-                                       P, I64, I, D, P]
-        L.orc_threads.restype = I
-        _lib = L
+        _lib = _configure(ctypes.CDLL(_SO))
     return _lib
+
+
+def _configure(L):
+    """Declare the C entry points' argument types on a loaded oracle library."""
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    I = ctypes.c_int
+    D = ctypes.c_double
+    L.orc_pack.argtypes = [P, P, I64, I64, I, P]
+    L.orc_rc_windows.argtypes = [P, I64, I64, I, P]
+    L.orc_pair_hist.argtypes = [P, P, I64, I64, P, P, P, P, I64, I64, I, P]
+    L.orc_self_hist.argtypes = [P, P, P, P, I64, I64, I, P]
+    L.orc_explain.argtypes = [P, I64, I64, P, I64, I64, P, P, P, I, I, I64, I,
+                              P, P, P]
+    L.orc_neighbor_count.argtypes = [I, I]
+    L.orc_neighbor_count.restype = I64
+    L.orc_neighbor_masks.argtypes = [I, I, P, P, P]
+    L.orc_neighbor_masks.restype = I64
+    L.orc_index_build.argtypes = [P, P, P, P, I64, I64, I, P, P, P]
+    L.orc_index_build.restype = I64
+    L.orc_index_reduce.argtypes = [P, P, I64, I64, P, P, P, P, P, I64, I, P, P,
+                                   P, I64, I, D, P]
+    L.orc_threads.restype = I
+    return L
 
 
 def _p(a):

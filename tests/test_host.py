@@ -464,3 +464,28 @@ def test_emulated_sweep_explain_and_ism_deltas(mf):
                 got = dk[b, :, 0, pos] + dk[b, :, 1 + v, pos]
                 np.testing.assert_allclose(got, km[0] - kr[b], rtol=0,
                                            atol=1e-12 * np.abs(kr[b]).max())
+
+
+def test_fixed_point_sv_weights():
+    """wfix = round(coef / sqrt(K(s, s)) 2^shift): the largest at 2^61 or just
+    under, 0 for SVs without windows, each within half a unit of its exact
+    value (checked with Python's exact rationals)."""
+    from fractions import Fraction
+
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(3)
+    coef = rng.normal(size=200) * 10.0 ** rng.integers(-6, 3, 200)
+    ds = rng.uniform(1, 1e4, 200)
+    ds[7] = 0.0
+    wfix, shift = E.fixed_weights(coef, ds, "cpu")
+    wfix = wfix.numpy()
+    w = np.where(ds > 0, coef / np.sqrt(np.where(ds > 0, ds, 1)), 0.0)
+    assert wfix[7] == 0 and np.abs(wfix).max() <= 2**61
+    assert np.abs(wfix).max() > 2**60
+    for i in range(200):
+        assert abs(Fraction(int(wfix[i])) - Fraction(w[i]) * 2**shift) <= Fraction(1, 2)
+    wf2, s2 = E.fixed_weights(coef, None, "cpu")  # no normalisation: the coefficients
+    assert abs(int(wf2.numpy()[0]) - coef[0] * 2.0**s2) <= 0.5
+    z, s0 = E.fixed_weights(np.zeros(3), None, "cpu")
+    assert s0 == 0 and not z.numpy().any()

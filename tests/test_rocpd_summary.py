@@ -49,3 +49,33 @@ def test_pmc_skips_in_dispatch_order(tmp_path):
     assert float(_rows(R.pmc(d, "k_"))["k_a"][2]) == 6.0
     rows = _rows(R.pmc(d, "k_", "k_a=1"))
     assert float(rows["k_a"][2]) == 10.0 and float(rows["k_b"][2]) == 7.0
+
+
+def test_traffic_scales_counters_by_the_copy_calibration(tmp_path):
+    """HBM bytes = FETCH_SIZE x (copy bytes / copy FETCH_SIZE) + WRITE_SIZE x
+    (copy bytes / copy WRITE_SIZE), template instances pooled by base name;
+    bench.py reads the TRAFFIC / VALU lines back."""
+    def run(name, rows):
+        d = tmp_path / name / "h"
+        d.mkdir(parents=True)
+        body = '"Correlation_Id","Dispatch_Id","Kernel_Name","Counter_Name","Counter_Value"\n'
+        body += "".join(f'{i},{i},"{k}","{c}",{v}\n' for i, (k, c, v) in enumerate(rows, 1))
+        (d / "run_counter_collection.csv").write_text(body)
+        return str(tmp_path / name)
+
+    cb = R.COPY_BYTES // 1024
+    cf = run("cf", [("__amd_rocclr_copyBuffer", "FETCH_SIZE", cb / 2)])
+    cw = run("cw", [("__amd_rocclr_copyBuffer", "WRITE_SIZE", cb)])
+    f = run("f", [("void k_x<4, 1>(A)", "FETCH_SIZE", 100), ("void k_x<2, 0>(A)", "FETCH_SIZE", 300),
+                  ("k_y", "FETCH_SIZE", 10)])
+    w = run("w", [("void k_x<4, 1>(A)", "WRITE_SIZE", 8), ("void k_x<2, 0>(A)", "WRITE_SIZE", 8),
+                  ("k_y", "WRITE_SIZE", 0)])
+    v = run("v", [("void k_x<4, 1>(A)", "SQ_INSTS_VALU", 640)])
+    text = R.traffic(f, w, cf, cw, 0, v)
+    assert "FETCH_SIZE x2.000 WRITE_SIZE x1.000" in text
+    p = tmp_path / "summary.txt"
+    p.write_text("header\n" + text + "\n")
+    prof = R.read_profile(str(p))
+    assert prof["traffic"]["k_x"] == (200 * 2 + 8) * 1024.0
+    assert prof["traffic"]["k_y"] == 10 * 2 * 1024.0
+    assert prof["valu"] == {"k_x": 640.0}
