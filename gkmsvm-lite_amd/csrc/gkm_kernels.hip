@@ -1071,6 +1071,9 @@ struct IndexArgs {
     void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16),
                              // or with wfix [nchunks][B][n_tiles][nm] int128 level sums
     const long long *wfix;   // [S] fixed-point SV weights (exact scoring), or NULL
+    const int32_t *items;    // persistent form: [n_pairs][2] (row, window chunk) pairs
+    int *queue;              // persistent form: 8 item counters (zero at launch)
+    int n_pairs;
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
     int sub_tile;            // PACK 4: SVs per pass of the 16-bit walk
     int64_t col_min, row_base;
@@ -1110,17 +1113,14 @@ struct IndexArgs {
 // congruent mod 8 and so share an XCD's L2: p -> q = p / 8, tile = q % n_tiles,
 // chunk = (q / n_tiles) * 8 + p % 8.  Ids past the last chunk, and chunks past
 // the row's last window, exit at once.
-template <int PACK, int RK, bool W16 = false>
-__global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
+// One work item of the index query: window chunk `chunk` of row b against SV
+// tile tile_id; with wfix its level sums go to slot `slot` of the psum output.
+template <int PACK, int RK, bool W16>
+__device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk, int tile_id,
+                                            int64_t slot)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     uint32_t *cnt = (uint32_t *)smem;
-    const int q = blockIdx.x >> 3;
-    const int tile_id = q % a.n_tiles;
-    const int chunk = (q / a.n_tiles) * 8 + (int)(blockIdx.x & 7);
-    if (chunk >= a.nchunks)
-        return;
-    const int b = blockIdx.y;
     const int wx = a.xlen[b] - a.l + 1;
     const int i0 = chunk * a.chunk_windows;
     if (i0 >= wx)
@@ -1150,10 +1150,9 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     using per4 = std::integral_constant<int, 4>;
     using per2 = std::integral_constant<int, 2>;
     using per1 = std::integral_constant<int, 1>;
-    auto emit = [&]() {  // the level sums out: [nchunks][B][n_tiles][nm]
+    auto emit = [&]() {  // the level sums out: slot * nm + level
         if (threadIdx.x < a.nm)
-            ((i128 *)a.partial)[(((int64_t)chunk * a.B + b) * a.n_tiles + tile_id) * a.nm +
-                                threadIdx.x] = bsum[threadIdx.x];
+            ((i128 *)a.partial)[slot * a.nm + threadIdx.x] = bsum[threadIdx.x];
     };
     const uint32_t nl = 1u << (2 * a.l);
     const int i1 = min(wx, i0 + a.chunk_windows);
@@ -1594,6 +1593,59 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
                 dst[i] = cnt[lv * ns + sid];
             }
         }
+    }
+}
+
+template <int PACK, int RK, bool W16 = false>
+__global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
+{
+    const int q = blockIdx.x >> 3;
+    const int tile_id = q % a.n_tiles;
+    const int chunk = (q / a.n_tiles) * 8 + (int)(blockIdx.x & 7);
+    if (chunk >= a.nchunks)
+        return;
+    const int b = blockIdx.y;
+    index_block<PACK, RK, W16>(a, b, chunk, tile_id,
+                               ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id);
+}
+
+// The persistent form: as many blocks as the chip holds at once (one per CU),
+// each popping work items -- (row, window chunk) pairs a.items[rc], times the
+// SV tiles -- until none is left.  The pairs are dealt round-robin to 8
+// queues, one per XCD (blocks are dispatched to the XCDs round-robin by
+// linear id, so block p runs on XCD p % 8), and a pair's tile items follow each
+// other in its queue, so the blocks probing one pair's record lines share an
+// L2; a block whose queue is empty takes items from the others.  Item k of
+// queue x is pair x + 8 (k / n_tiles), tile k % n_tiles; its level sums go to
+// slot pair * n_tiles + tile.
+template <int PACK, int RK, bool W16 = false>
+__global__ __launch_bounds__(INDEX_THREADS) void k_index_tile_queue(IndexArgs a)
+{
+    __shared__ int pick[2];
+    const int home = (int)(blockIdx.x & 7);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int rc = -1, tile = 0;
+            for (int t = 0; t < 8 && rc < 0; ++t) {
+                const int qx = (home + t) & 7;
+                const int k = atomicAdd(&a.queue[qx], 1);
+                const int r = qx + 8 * (k / a.n_tiles);
+                if (r < a.n_pairs) {
+                    rc = r;
+                    tile = k - (k / a.n_tiles) * a.n_tiles;
+                }
+            }
+            pick[0] = rc;
+            pick[1] = tile;
+        }
+        __syncthreads();
+        const int rc = pick[0], tile = pick[1];
+        __syncthreads();  // every thread has its item before thread 0 picks again
+        if (rc < 0)
+            return;
+        index_block<PACK, RK, W16>(a, a.items[2 * rc], a.items[2 * rc + 1], tile,
+                                   (int64_t)rc * a.n_tiles + tile);
+        __syncthreads();
     }
 }
 
@@ -2038,7 +2090,8 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
                              int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile,
                              int n_tiles, int chunk_windows, int pack_counts, int64_t col_min,
                              int64_t row_base, void *partial, const long long *wfix,
-                             int *count_bytes, void *stream)
+                             int *count_bytes, void *stream, const int32_t *items = nullptr,
+                             int n_pairs = 0, int *queue = nullptr, int n_blocks = 0)
 {
     int rc = check_l_nm(l, nm);
     if (rc)
@@ -2048,7 +2101,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     if (tile < 1 || tile > 65536 || n_tiles != (S + tile - 1) / tile || chunk_windows < 1)
         return fail(-1, "%s: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", who, tile,
                     n_tiles, S, chunk_windows);
-    if (B > 65535)
+    if (B > 65535 && items == nullptr)
         return fail(-1, "%s: at most 65535 rows per call, got %d", who, B);
     if (pack_counts < 0 || pack_counts > 2)
         return fail(-1, "%s: pack_counts must be 0, 1 or 2, got %d", who, pack_counts);
@@ -2081,6 +2134,9 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     a.wtab = wtab;
     a.partial = partial;
     a.wfix = wfix;
+    a.items = items;
+    a.queue = queue;
+    a.n_pairs = n_pairs;
     a.B = B;
     a.Wx = Wx;
     a.n_masks = n_masks;
@@ -2097,17 +2153,30 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     const int rk = records == nullptr ? 0 : vals ? 2 : 1;  // 32-byte records per key
     a.records = records;
     typedef void (*tile_fn)(IndexArgs);
-    const tile_fn kfn = w16       ? (rk ? k_index_tile<2, 2, true> : k_index_tile<2, 0, true>)
-                      : pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
-                      : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
-                      : rk == 2   ? k_index_tile<1, 2>
-                      : rk == 1   ? k_index_tile<1, 1> : k_index_tile<1, 0>;
+    tile_fn kfn;
+    if (items == nullptr)
+        kfn = w16       ? (rk ? k_index_tile<2, 2, true> : k_index_tile<2, 0, true>)
+            : pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
+            : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
+            : rk == 2   ? k_index_tile<1, 2>
+            : rk == 1   ? k_index_tile<1, 1> : k_index_tile<1, 0>;
+    else
+        kfn = w16       ? (rk ? k_index_tile_queue<2, 2, true> : k_index_tile_queue<2, 0, true>)
+            : pack == 4 ? (rk ? k_index_tile_queue<4, 1> : k_index_tile_queue<4, 0>)
+            : pack == 2 ? (rk ? k_index_tile_queue<2, 1> : k_index_tile_queue<2, 0>)
+            : rk == 2   ? k_index_tile_queue<1, 2>
+            : rk == 1   ? k_index_tile_queue<1, 1> : k_index_tile_queue<1, 0>;
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
-    dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
-    hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    if (items == nullptr) {
+        dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
+        hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
+    } else if (n_pairs > 0) {
+        hipLaunchKernelGGL(kfn, dim3((unsigned)n_blocks), dim3(INDEX_THREADS), lds, S_(stream),
+                           a);
+    }
     GKM_CHECK_LAUNCH(who);
     return 0;
 }
@@ -2133,11 +2202,13 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
 
 // out[b] = the row's decision value from the level sums of its blocks (the
 // chunks that hold its windows, every SV tile).
+// pair_first NULL: slots [chunk][B][n_tiles] of gkm_index_score; else the
+// persistent form's slots (pair_first[b] + chunk) * n_tiles + tile.
 __global__ __launch_bounds__(WAVE) void k_index_score_reduce(
     const i128 *__restrict__ psum, int B, int n_tiles, int nm, int nchunks,
     const int32_t *__restrict__ xlen, int l, int chunk_windows, const double *__restrict__ tab,
     const double *__restrict__ dx, int shift, int normalize, double bias,
-    double *__restrict__ out)
+    double *__restrict__ out, const int32_t *__restrict__ pair_first)
 {
     __shared__ i128 sums[16];
     const int b = blockIdx.x;
@@ -2148,7 +2219,9 @@ __global__ __launch_bounds__(WAVE) void k_index_score_reduce(
         i128 acc = 0;
         for (int k = lane; k < u * n_tiles; k += WAVE) {
             const int c = k / n_tiles, t = k - c * n_tiles;
-            acc += psum[(((int64_t)c * B + b) * n_tiles + t) * nm + lv];
+            const int64_t slot = pair_first ? ((int64_t)pair_first[b] + c) * n_tiles + t
+                                            : ((int64_t)c * B + b) * n_tiles + t;
+            acc += psum[slot * nm + lv];
         }
         acc = wave_sum_i128(acc);
         if (lane == 0)
@@ -2242,8 +2315,60 @@ GKM_API int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int
     const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
     hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
                        (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
-                       dx, shift, normalize, bias, out);
+                       dx, shift, normalize, bias, out, nullptr);
     GKM_CHECK_LAUNCH("gkm_index_score");
+    return 0;
+}
+
+__global__ void k_zero_i32(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+GKM_API int64_t gkm_index_queue_bytes(int n_pairs, int n_tiles, int nm)
+{
+    return 256 + (int64_t)(n_pairs > 0 ? n_pairs : 0) * n_tiles * nm * 16;
+}
+
+GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
+                                  int B, int Wx, const uint32_t *offsets, const uint16_t *ids,
+                                  const uint8_t *vals, const uint32_t *records, int record_tables,
+                                  const uint32_t *masks, const uint8_t *levels, int n_masks,
+                                  int l, int nm, const uint8_t *wtab, int S, int tile,
+                                  int n_tiles, int chunk_windows, int pack_counts,
+                                  const long long *wfix, const int32_t *items, int n_pairs,
+                                  const int32_t *pair_first, int n_blocks, void *workspace,
+                                  const double *tab, const double *dx, int shift, int normalize,
+                                  double bias, double *out, void *stream)
+{
+    if (wfix == nullptr || workspace == nullptr || out == nullptr || tab == nullptr ||
+        pair_first == nullptr || (n_pairs > 0 && items == nullptr))
+        return fail(-1, "gkm_index_score_queue: wfix, items, pair_first, workspace, tab and "
+                        "out are required");
+    if (normalize && dx == nullptr)
+        return fail(-1, "gkm_index_score_queue: normalize needs dx");
+    if ((uintptr_t)workspace % 256)
+        return fail(-1, "gkm_index_score_queue: workspace must be 256-byte aligned");
+    if (n_blocks < 1 || n_blocks > 65536)
+        return fail(-1, "gkm_index_score_queue: n_blocks must be 1..65536, got %d", n_blocks);
+    int *queue = (int *)workspace;
+    void *psum = (uint8_t *)workspace + 256;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, S_(stream), queue, 64);
+    GKM_CHECK_LAUNCH("gkm_index_score_queue");
+    int count_bytes = 4;
+    int rc = launch_index_tile("gkm_index_score_queue", xwin, xpos, xlen, B, Wx, offsets, ids,
+                               vals, records, record_tables, masks, levels, n_masks, l, nm, wtab,
+                               S, tile, n_tiles, chunk_windows, pack_counts, -1, 0, psum, wfix,
+                               &count_bytes, stream, items, n_pairs, queue, n_blocks);
+    if (rc || B <= 0)
+        return rc;
+    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
+    hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
+                       (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
+                       dx, shift, normalize, bias, out, pair_first);
+    GKM_CHECK_LAUNCH("gkm_index_score_queue");
     return 0;
 }
 

@@ -45,6 +45,9 @@ SIGNATURES = {
     "gkm_index_psum_bytes": [I, I, I, I, I],
     "gkm_index_score": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P, P, P,
                         P, I, I, D, P, P],
+    "gkm_index_queue_bytes": [I, I, I],
+    "gkm_index_score_queue": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P,
+                              P, I, P, I, P, P, P, I, I, D, P, P],
     "gkm_scratch_bytes": [],
     "gkm_index_partial_bytes": [I, I, I, I, I, I, I, I],
     "gkm_index_hist_noalloc": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I,
@@ -64,7 +67,8 @@ SIGNATURES = {
     "gkm_peak_valu": [I, P, I, P],
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p, "gkm_scratch_bytes": ctypes.c_int64,
-            "gkm_index_partial_bytes": ctypes.c_int64, "gkm_index_psum_bytes": ctypes.c_int64}
+            "gkm_index_partial_bytes": ctypes.c_int64, "gkm_index_psum_bytes": ctypes.c_int64,
+            "gkm_index_queue_bytes": ctypes.c_int64}
 
 _lib = None
 

@@ -84,7 +84,13 @@ class BatchScorer:
     """Score many sequences of uneven lengths against one resident model."""
 
     def __init__(self, model, *, rows: int = 256, bucket: int = 128, graphs: bool = True,
-                 autotune: bool = True):
+                 autotune: bool = True, persistent: bool = True):
+        """rows, bucket: rows per step and length granularity of the bucketed
+        path; graphs: capture each bucket shape as a HIP graph; autotune: time
+        the candidate plans (chunk sizes of the persistent launch) once;
+        persistent: kernels without RBF whose batch goes through the l-mer
+        index score every row in one persistent launch (gkm_index_score_queue)
+        instead of bucket by bucket."""
         model.rocm()
         self.model = model
         self.sv = model._svs()
@@ -95,6 +101,9 @@ class BatchScorer:
         self.autotune = autotune
         self.plans: dict = {}
         self._graphs: dict = {}
+        self.persistent = persistent
+        self.queue_chunk: int | None = None  # autotuned windows per persistent work item
+        self.queue_ms: dict = {}
         # the bucket histogram must fit the hist budget of one step
         self.rows = max(1, min(self.rows, E.rows_per_chunk(self.sv.S, self.kp.nm)))
 
@@ -191,6 +200,8 @@ class BatchScorer:
             # plans and graphs made before the table existed are stale
             self.plans = {k: v for k, v in self.plans.items() if v.table is not None}
             self._graphs = {k: g for k, g in self._graphs.items() if k in self.plans}
+        if self._use_queue(lens):
+            return nan_rows(self._score_queue(codes, lens)[:, None], short)
         order = np.argsort(lens, kind="stable")
         out = torch.empty(len(lens), dtype=torch.float64, device=self.kp.device)
         i = 0
@@ -205,3 +216,58 @@ class BatchScorer:
             out[torch.from_numpy(idx).to(out.device)] = res + self.model.bias
             i = j
         return nan_rows(out[:, None], short)
+
+    # -- the persistent launch -------------------------------------------------------
+
+    QUEUE_ROWS = 1 << 17   # rows per persistent launch (device memory of one launch)
+    QUEUE_CHUNKS = (16, 32, 64)
+    QUEUE_TUNE_ROWS = 1024
+
+    def _use_queue(self, lens) -> bool:
+        if not self.persistent or self.sv.wfix is None or self.model._lmer_table_for() is not None:
+            return False
+        if self.kp.l > E.INDEX_MAX_L or self.kp.dmax < 0:
+            return False
+        return self.sv.use_index_for(lens, self.model.use_lmer_index)
+
+    def _queue_rows(self, codes, lens, chunk) -> torch.Tensor:
+        kp = self.kp
+        xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+        dx = E.self_kraw(kp, xs) if kp.normalize else None
+        out = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
+        return E.index_score_queue(self.sv, xs, dx, self.model.bias, out, chunk=chunk)
+
+    def _tune_queue(self, codes, lens):
+        """Windows per work item: each candidate timed once on a sample of the
+        rows spread over their lengths (every candidate gives the same bits)."""
+        if not self.autotune:
+            self.queue_chunk = E.QUEUE_CHUNK
+            return
+        order = np.argsort(lens, kind="stable")
+        pick = order[np.linspace(0, len(order) - 1, min(len(order), self.QUEUE_TUNE_ROWS))
+                     .astype(np.int64)]
+        c, ln = codes[pick], lens[pick]
+        best = None
+        for chunk in self.QUEUE_CHUNKS:
+            self._queue_rows(c, ln, chunk)  # warm-up
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            self._queue_rows(c, ln, chunk)
+            torch.cuda.synchronize()
+            self.queue_ms[chunk] = (time.perf_counter() - t) * 1e3
+            if best is None or self.queue_ms[chunk] < self.queue_ms[best]:
+                best = chunk
+        self.queue_chunk = best
+
+    def _score_queue(self, codes, lens) -> torch.Tensor:
+        """[B] decision values of every row through persistent launches of up
+        to QUEUE_ROWS rows each."""
+        if self.queue_chunk is None:
+            self._tune_queue(codes, lens)
+        parts = []
+        for s in range(0, len(lens), self.QUEUE_ROWS):
+            e = min(len(lens), s + self.QUEUE_ROWS)
+            w = max(self.kp.l, int(lens[s:e].max()))
+            parts.append(self._queue_rows(np.ascontiguousarray(codes[s:e, :w]), lens[s:e],
+                                          self.queue_chunk))
+        return parts[0] if len(parts) == 1 else torch.cat(parts)

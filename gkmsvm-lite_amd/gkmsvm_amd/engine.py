@@ -612,16 +612,19 @@ class IndexLaunch:
     chunking, sorted query windows, partial-count format); `args` are its
     C-ABI arguments up to (not including) partial."""
 
-    def __init__(self, kp, idx, xs, x0, x1, S, col_min, stream, target_blocks):
+    def __init__(self, kp, idx, xs, x0, x1, S, col_min, stream, target_blocks, chunk=None):
         B = x1 - x0
-        if target_blocks is None:
+        if target_blocks is None and chunk is None:
             target_blocks = index_blocks(xs.lengths[x0:x1], kp.l, idx.n_tiles)
         fwd, _ = xs.windows(kp.l, stream)
         Wx = fwd.shape[1]
         wx_max = max(1, int(np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None).max()))
-        nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)),
-                             -(-wx_max // INDEX_CHUNK_MIN)))
-        cw = -(-wx_max // nchunks)
+        if chunk is not None:  # windows per block given (the persistent form)
+            cw = max(1, int(chunk))
+        else:
+            nchunks = max(1, min(-(-target_blocks // (idx.n_tiles * B)),
+                                 -(-wx_max // INDEX_CHUNK_MIN)))
+            cw = -(-wx_max // nchunks)
         if idx.vals is not None:  # per-block uint32 counts must not overflow
             cap = (2**32 - 1) // (idx.max_weight**2 * max(1, idx.max_sv_windows))
             cw = max(1, min(cw, cap))
@@ -794,6 +797,43 @@ def index_score(sv: SVModel, xs: DeviceSeqs, r0: int, r1: int, dx, bias: float, 
                    "gkm_index_score", *L.args[:-2], _p(sv.wfix), _p(psum), _p(kp.tab),
                    _p(dx[a:b]) if dx is not None else None, sv.fix_shift, int(kp.normalize),
                    float(bias), _p(out[a:b]), stream_handle(stream)))
+    return out
+
+
+QUEUE_CHUNK = 16  # query windows per work item of the persistent index launch
+
+
+def n_cus(device) -> int:
+    return int(torch.cuda.get_device_properties(device).multi_processor_count)
+
+
+def index_score_queue(sv: SVModel, xs: DeviceSeqs, dx, bias: float, out: torch.Tensor,
+                      stream=None, chunk: int = QUEUE_CHUNK) -> torch.Tensor:
+    """out[:] = every row's decision value from the l-mer index in one
+    persistent launch (gkm_index_score_queue): the rows, longest first, cut
+    into work items of `chunk` windows, popped by one block per CU from
+    per-XCD queues.  The same bits as index_score.  Kernels without RBF."""
+    kp, idx = sv.kp, sv.index()
+    L = IndexLaunch(kp, idx, xs, 0, xs.n, sv.S, -1, stream, None, chunk=chunk)
+    wx = np.clip(xs.lengths - kp.l + 1, 0, None)
+    nck = -(-wx // L.cw)
+    order = np.argsort(-wx, kind="stable")
+    first = np.zeros(xs.n, dtype=np.int64)
+    first[order] = np.concatenate([[0], np.cumsum(nck[order])[:-1]])
+    n_pairs = int(nck.sum())
+    rows = np.repeat(order, nck[order])
+    chunks = np.arange(n_pairs) - np.repeat(first[order], nck[order])
+    items = torch.from_numpy(np.stack([rows, chunks], axis=1).astype(np.int32)).to(kp.device)
+    pair_first = torch.from_numpy(first.astype(np.int32)).to(kp.device)
+    ws = torch.empty(256 + n_pairs * idx.n_tiles * kp.nm * 16, dtype=torch.uint8,
+                     device=kp.device)
+    args = L.args[:-2]
+    _timed("k_index_tile", lambda: index_bytes(kp, idx, xs.lengths), stream,
+           lambda: _lib.call(
+               "gkm_index_score_queue", *args, _p(sv.wfix), _p(items), n_pairs,
+               _p(pair_first), n_cus(kp.device), _p(ws), _p(kp.tab),
+               _p(dx) if dx is not None else None, sv.fix_shift, int(kp.normalize), float(bias),
+               _p(out), stream_handle(stream)))
     return out
 
 

@@ -50,9 +50,13 @@ def test_batch_scorer_matches_plain_calls(gpu, kt, extra, table):
         np.testing.assert_array_equal(got, want, err_msg=f"replay {rep}")
         m(qs[:5])
         torch.cuda.synchronize()
-    assert sc.plans and sc._graphs
+    assert (sc.plans and sc._graphs) or sc.queue_chunk  # buckets, or one persistent launch
     got = BatchScorer(m, rows=32, graphs=False, autotune=False)(qs).cpu().numpy()
     np.testing.assert_array_equal(got, want)
+    sb = BatchScorer(m, rows=64, bucket=128, graphs=True, persistent=False)
+    for rep in range(2):  # the bucketed path with its graphs, always
+        np.testing.assert_array_equal(sb(qs).cpu().numpy(), want, err_msg=f"bucket {rep}")
+    assert sb.plans and sb._graphs
 
 
 @pytest.mark.parametrize("tables", [False, True])

@@ -99,3 +99,55 @@ def test_model_scores_through_either_path_are_identical(gpu):
     m.use_lmer_index = "auto"
     c = BatchScorer(m, rows=16, bucket=128, graphs=True)(codes, lengths=lens).cpu().numpy()
     np.testing.assert_array_equal(c, a)
+
+
+@pytest.mark.parametrize("kt,extra,bits", [(2, {}, 8), (2, {}, 16),
+                                           (4, {"M": 50, "H": 50.0}, 16),
+                                           (4, {"M": 50, "H": 50.0}, 32)])
+def test_persistent_launch_has_the_same_bits(gpu, kt, extra, bits):
+    """gkm_index_score_queue (every row in one persistent launch, work items
+    popped from per-XCD queues) equals the grid launch and the pairwise
+    kernel bit for bit, for several window chunks; rows without windows get
+    the empty sums."""
+    from gkmsvm_amd import _lib
+    from gkmsvm_amd import engine as E
+
+    m = _model(kt, extra, 5)
+    kp, sv = m._params(), m._svs()
+    sv._index = E.DeviceIndex(kp, sv.seqs, count_bits=bits)
+    codes, lens = _queries(60, 6, lo=5, hi=1000)
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    dx = E.self_kraw(kp, xs)
+    want = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
+    E.index_score(sv, xs, 0, xs.n, dx, m.bias, want)
+    hist = E.cross_hist(kp, xs, 0, xs.n, sv.seqs, sv.groups)
+    brute = torch.empty_like(want)
+    _lib.call("gkm_combine_fixed", hist.data_ptr(), xs.n, sv.S, kp.nm, kp.tab.data_ptr(),
+              dx.data_ptr(), sv.wfix.data_ptr(), sv.fix_shift, 1, m.bias, brute.data_ptr(),
+              E.stream_handle())
+    np.testing.assert_array_equal(brute.cpu().numpy(), want.cpu().numpy())
+    for chunk in (7, 16, 64):
+        got = torch.empty_like(want)
+        E.index_score_queue(sv, xs, dx, m.bias, got, chunk=chunk)
+        np.testing.assert_array_equal(got.cpu().numpy(), want.cpu().numpy(),
+                                      err_msg=f"chunk {chunk}")
+
+
+def test_persistent_launch_takes_more_rows_than_a_grid(gpu):
+    """70,000 rows (past the grid's 65,535 rows per launch) in one persistent
+    launch through the batch scorer: equal to model(x) row for row."""
+    from gkmsvm_amd.batching import BatchScorer
+
+    m = _model(2, {}, 7, S=300)
+    m.use_lmer_table = False
+    m.use_lmer_index = True
+    rng = np.random.default_rng(8)
+    n = 70_000
+    lens = rng.integers(12, 41, n)
+    codes = rng.integers(0, 4, (n, 40)).astype(np.uint8)
+    sc = BatchScorer(m, autotune=False)
+    got = sc(codes, lengths=lens).cpu().numpy().ravel()
+    assert sc.queue_chunk is not None
+    pick = np.arange(0, n, 997)
+    want = m(codes[pick], lengths=lens[pick]).cpu().numpy().ravel()
+    np.testing.assert_array_equal(got[pick], want)

@@ -117,9 +117,11 @@ def test_gram_workload_10k(gpu):
 
 def test_batched_uneven_workload_20k_svs(gpu):
     """-t 4 (weighted, 16-bit weighted index counts), 20,000 SVs of 100-1,000
-    bp, 1,024 of the uneven 100-1,000 bp queries with 1% N: the batched scorer
-    equals each row scored alone, the l-mer table path agrees within 1e-10,
-    and 4 rows' histograms agree between the index and the pairwise kernel and
+    bp, 20,000 of the uneven 100-1,000 bp queries with 1% N (stored as A, as
+    the kernels read N): the batch scorer's one persistent launch equals each
+    row scored alone bit for bit (12 rows) and the oracle's l-mer index
+    scores within 1e-10 (48 rows); the l-mer table path agrees within 1e-10;
+    4 rows' histograms agree between the index and the pairwise kernel and
     with the oracle on 48 sampled SVs."""
     from gkmsvm_amd.batching import BatchScorer
 
@@ -129,16 +131,28 @@ def test_batched_uneven_workload_20k_svs(gpu):
     svc = np.zeros((S, 1000), np.uint8)
     for i, n in enumerate(svl):
         svc[i, :n] = rng.integers(0, 4, n)
-    m = _model(4, {**P, "M": 50, "H": 50.0}, svc, svl, rng.normal(size=S))
-    qs = _strings(rng, rng.integers(100, 1001, 1024), 0.01)
-    got = BatchScorer(m, rows=256, bucket=128)(qs).cpu().numpy().ravel()
-    sample = rng.choice(len(qs), 12, replace=False)
-    alone = np.array([m([qs[i]]).cpu().numpy().ravel()[0] for i in sample])
+    coef = rng.normal(size=S)
+    m = _model(4, {**P, "M": 50, "H": 50.0}, svc, svl, coef)
+    nq = 20_000
+    ql = rng.integers(100, 1001, nq)
+    qc = rng.integers(0, 4, (nq, 1000)).astype(np.uint8)
+    qc[rng.random(qc.shape) < 0.01] = 0  # N
+    sc = BatchScorer(m, rows=256, bucket=128)
+    got = sc(qc, lengths=ql).cpu().numpy().ravel()
+    assert sc.queue_chunk is not None  # the persistent launch, chunk autotuned
+    sample = rng.choice(nq, 12, replace=False)
+    alone = np.array([m(qc[i:i + 1, :ql[i]]).cpu().numpy().ravel()[0] for i in sample])
     np.testing.assert_array_equal(got[sample], alone)
+    rows48 = np.sort(rng.choice(nq, 48, replace=False))
+    om = O.Model(O.Kern.make(4, {**P, "M": 50, "H": 50.0}), svc, svl, coef, 0.0)
+    want = O.LmerIndex(om).score(qc[rows48], ql[rows48])
+    H.assert_close(got[rows48], want, 1e-10, "persistent launch vs oracle index")
     m.use_lmer_table = True
-    H.assert_close(m(qs).cpu().numpy().ravel(), got, 1e-10, "table vs per-SV")
+    H.assert_close(m(qc[:1024], lengths=ql[:1024]).cpu().numpy().ravel(), got[:1024], 1e-10,
+                   "table vs per-SV")
     m.use_lmer_table = False
-    s = _histograms_both_paths(m, [qs[i] for i in sample[:4]], gpu, 48, rng)
+    qs = ["".join("ACGT"[c] for c in qc[i, :ql[i]]) for i in sample[:4]]
+    s = _histograms_both_paths(m, qs, gpu, 48, rng)
     idx = s.index()
     assert idx.vals is not None and idx.pack == 2 and idx.n_tiles == 2
 
