@@ -4,7 +4,9 @@ sample of columns is checked against the oracle's brute-force counts), the two
 GPU histogram paths (pairwise kernel, l-mer index) agree bit for bit over every
 column, a row scores the same alone and in a batch, the Gram is exactly
 symmetric with a unit diagonal, ISM equals the rescored mutants, and
-GkmExplain through the index equals the per-pair sweep."""
+GkmExplain through the index equals the per-pair sweep; the serving mix gives
+each request's result alone, and GkmExplain on a slice of the 50,000-SV
+model equals the oracle's."""
 
 import numpy as np
 import pytest
@@ -191,3 +193,88 @@ def test_explain_ism_workload_50k_svs(gpu):
     tol = 1e-12 * np.abs(coef).sum()
     np.testing.assert_allclose(d[alt, pos], f[1:] - f[0], rtol=0, atol=tol)
     assert np.abs(f[1:] - f[0]).max() > 1e3 * tol
+
+
+def _serving_requests(rng, n):
+    """The serving mix: 70% scoring (50-2,000 bp), 20% SNV pairs (300 bp),
+    10% GkmExplain (1 kb)."""
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        if u < 0.7:
+            out.append(("score", _strings(rng, [int(rng.integers(50, 2001))])))
+        elif u < 0.9:
+            ref = _strings(rng, [300])[0]
+            alt = ref[:150] + "ACGT"[("ACGT".index(ref[150]) + int(rng.integers(1, 4))) % 4] + \
+                ref[151:]
+            out.append(("variant", [(ref, alt)]))
+        else:
+            out.append(("explain", _strings(rng, [1000])))
+    return out
+
+
+@pytest.mark.parametrize("tables", [False, True])
+def test_serving_mix_72k_svs_equal_alone(gpu, tables):
+    """The serving configuration: 72,000 SVs x 500 bp (-t 2) resident, two
+    bursts of 96 requests of the serving mix served through GkmServer on the
+    per-SV kernels (index query and index explain) or through the l-mer
+    tables: every result equals the same call made alone -- scores and
+    variant effects bit for bit, attributions within 1e-10."""
+    from gkmsvm_amd import gkmexplain
+    from gkmsvm_amd.serving import GkmServer
+
+    rng = np.random.default_rng(5)
+    S = 72_000
+    m = _model(2, P, rng.integers(0, 4, (S, 500)).astype(np.uint8), np.full(S, 500),
+               rng.normal(size=S))
+    if tables:
+        m.use_lmer_table = m.use_explain_table = "auto"
+    reqs = _serving_requests(rng, 192)
+    with GkmServer(m, hbm_budget=16 << 30, tables=tables) as srv:
+        assert (srv.table is not None) == tables
+        futs = [srv.submit(kind, payload) for kind, payload in reqs[:96]]
+        got = [f.result(timeout=120) for f in futs]
+        futs = [srv.submit(kind, payload) for kind, payload in reqs[96:]]
+        got += [f.result(timeout=120) for f in futs]
+    n_ex = 0
+    for (kind, payload), g in zip(reqs, got):
+        if kind == "score":
+            np.testing.assert_array_equal(g, m(payload).cpu().numpy().ravel())
+        elif kind == "variant":
+            want = m.score_variants([payload[0][0]], [payload[0][1]]).cpu().numpy().ravel()
+            np.testing.assert_array_equal(g, want)
+        else:
+            n_ex += 1
+            H.assert_close(g, gkmexplain(m, payload, mode="hypothetical").cpu().numpy(), 1e-10,
+                           "served explain")
+    assert n_ex > 0
+
+
+def test_explain_50k_model_slice_vs_oracle(gpu):
+    """GkmExplain of the -t 5 explain configuration (gamma 2, 500 bp SVs with
+    Laplace(0, 0.5) coefficients, 1 kb queries) against the oracle's per-SV
+    loop on a 2,000-SV slice of the 50,000-SV model -- attributions are sums
+    over the SVs, so the slice checks the same arithmetic: through the l-mer
+    index and through the per-pair sweep, within 1e-10."""
+    from gkmsvm_amd import explain as X
+    from gkmsvm_amd import gkmexplain
+
+    rng = np.random.default_rng(8)
+    S = 50_000
+    codes = _gc_codes(rng, S, 500, 0.5)
+    coef = rng.laplace(0, 0.5, S)
+    sl = np.sort(rng.choice(S, 2000, replace=False))
+    p5 = {**P, "gamma": 2.0, "M": 50, "H": 50.0}
+    m = _model(5, p5, codes[sl], np.full(2000, 500), coef[sl])
+    qs = _strings(rng, [1000, 1000])
+    qc, ql = O.batch(qs)
+    om = O.Model(O.Kern.make(5, p5), codes[sl], np.full(2000, 500), coef[sl], 0.0)
+    want = O.explain(om, qc, ql, "hypothetical")
+    assert np.abs(want).max() > 0
+    for kernel in ("index", "sweep"):
+        X.EXPLAIN_KERNEL = kernel
+        try:
+            got = gkmexplain(m, qs, mode="hypothetical").cpu().numpy()
+        finally:
+            X.EXPLAIN_KERNEL = "auto"
+        H.assert_close(got, want, 1e-10, f"{kernel} explain vs oracle")
