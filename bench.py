@@ -219,6 +219,8 @@ HBM_PEAK = 8.0e12  # bytes/s
 
 HBM_KINDS = {
     "k_index_tile": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
+    "k_index_tile_queue": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 "
+                          "RMW); the persistent launch",
     "k_index_explain": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
     "k_table_score": "4 B packed window + 8 B table entry per query window",
     "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",

@@ -1071,8 +1071,9 @@ struct IndexArgs {
     void *partial;           // [nchunks][B][S][nm] per-chunk counts (PACK 2, 4: uint16),
                              // or with wfix [nchunks][B][n_tiles][nm] int128 level sums
     const long long *wfix;   // [S] fixed-point SV weights (exact scoring), or NULL
-    const int32_t *items;    // persistent form: [n_pairs][2] (row, window chunk) pairs
-    int *queue;              // persistent form: 8 item counters (zero at launch)
+    const int32_t *pair_first;  // persistent form: [B] each row's first pair (exclusive
+                                // prefix sums of the rows' chunk counts)
+    int *queue;                       // 8 item counters (zero at launch)
     int n_pairs;
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
     int sub_tile;            // PACK 4: SVs per pass of the 16-bit walk
@@ -1610,8 +1611,10 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 }
 
 // The persistent form: as many blocks as the chip holds at once (one per CU),
-// each popping work items -- (row, window chunk) pairs a.items[rc], times the
-// SV tiles -- until none is left.  The pairs are dealt round-robin to 8
+// each popping work items -- (row, window chunk) pairs, times the SV tiles --
+// until none is left; pair rc is chunk rc - first[b] of the row b with the
+// last first[b] <= rc (a binary search: rows without windows share the next
+// row's first, or n_pairs at the end).  The pairs are dealt round-robin to 8
 // queues, one per XCD (blocks are dispatched to the XCDs round-robin by
 // linear id, so block p runs on XCD p % 8), and a pair's tile items follow each
 // other in its queue, so the blocks probing one pair's record lines share an
@@ -1621,7 +1624,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 template <int PACK, int RK, bool W16 = false>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile_queue(IndexArgs a)
 {
-    __shared__ int pick[2];
+    __shared__ int pick[4];
     const int home = (int)(blockIdx.x & 7);
     for (;;) {
         if (threadIdx.x == 0) {
@@ -1635,16 +1638,28 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile_queue(IndexArgs a)
                     tile = k - (k / a.n_tiles) * a.n_tiles;
                 }
             }
+            int lo = 0;
+            if (rc >= 0) {
+                int hi = a.B - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (a.pair_first[mid] <= rc)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+            }
             pick[0] = rc;
             pick[1] = tile;
+            pick[2] = lo;
+            pick[3] = rc >= 0 ? rc - a.pair_first[lo] : 0;
         }
         __syncthreads();
-        const int rc = pick[0], tile = pick[1];
+        const int rc = pick[0], tile = pick[1], row = pick[2], chunk = pick[3];
         __syncthreads();  // every thread has its item before thread 0 picks again
         if (rc < 0)
             return;
-        index_block<PACK, RK, W16>(a, a.items[2 * rc], a.items[2 * rc + 1], tile,
-                                   (int64_t)rc * a.n_tiles + tile);
+        index_block<PACK, RK, W16>(a, row, chunk, tile, (int64_t)rc * a.n_tiles + tile);
         __syncthreads();
     }
 }
@@ -2090,7 +2105,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
                              int n_masks, int l, int nm, const uint8_t *wtab, int S, int tile,
                              int n_tiles, int chunk_windows, int pack_counts, int64_t col_min,
                              int64_t row_base, void *partial, const long long *wfix,
-                             int *count_bytes, void *stream, const int32_t *items = nullptr,
+                             int *count_bytes, void *stream, const int32_t *pair_first = nullptr,
                              int n_pairs = 0, int *queue = nullptr, int n_blocks = 0)
 {
     int rc = check_l_nm(l, nm);
@@ -2101,7 +2116,8 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     if (tile < 1 || tile > 65536 || n_tiles != (S + tile - 1) / tile || chunk_windows < 1)
         return fail(-1, "%s: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", who, tile,
                     n_tiles, S, chunk_windows);
-    if (B > 65535 && items == nullptr)
+    const bool persistent = pair_first != nullptr;
+    if (B > 65535 && !persistent)
         return fail(-1, "%s: at most 65535 rows per call, got %d", who, B);
     if (pack_counts < 0 || pack_counts > 2)
         return fail(-1, "%s: pack_counts must be 0, 1 or 2, got %d", who, pack_counts);
@@ -2134,7 +2150,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     a.wtab = wtab;
     a.partial = partial;
     a.wfix = wfix;
-    a.items = items;
+    a.pair_first = pair_first;
     a.queue = queue;
     a.n_pairs = n_pairs;
     a.B = B;
@@ -2154,7 +2170,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     a.records = records;
     typedef void (*tile_fn)(IndexArgs);
     tile_fn kfn;
-    if (items == nullptr)
+    if (!persistent)
         kfn = w16       ? (rk ? k_index_tile<2, 2, true> : k_index_tile<2, 0, true>)
             : pack == 4 ? (rk ? k_index_tile<4, 1> : k_index_tile<4, 0>)
             : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
@@ -2170,7 +2186,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
         hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
         return fail(-2, "could not raise the LDS limit to %zu bytes", lds);
-    if (items == nullptr) {
+    if (!persistent) {
         dim3 grid((unsigned)(n_tiles * ((a.nchunks + 7) / 8) * 8), (unsigned)B);
         hipLaunchKernelGGL(kfn, grid, dim3(INDEX_THREADS), lds, S_(stream), a);
     } else if (n_pairs > 0) {
@@ -2320,16 +2336,52 @@ GKM_API int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int
     return 0;
 }
 
-__global__ void k_zero_i32(int *p, int n)
+// The persistent launch's plan, in one block: the 8 queue counters zeroed
+// and first[b] = the exclusive prefix sum of the rows' window-chunk counts
+// (min(nchunks, ceil(windows / chunk_windows)), 0 without windows), each
+// thread summing a strip of consecutive rows.
+constexpr int PLAN_THREADS = 1024;
+__global__ __launch_bounds__(PLAN_THREADS) void k_queue_plan(
+    const int32_t *__restrict__ xlen, int B, int l, int chunk_windows, int nchunks,
+    int *__restrict__ queue, int32_t *__restrict__ first)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        p[i] = 0;
+    __shared__ int part[PLAN_THREADS];
+    const int t = threadIdx.x;
+    if (t < 64)
+        queue[t] = 0;
+    const int per = (B + PLAN_THREADS - 1) / PLAN_THREADS;
+    const int b0 = min(B, t * per), b1 = min(B, b0 + per);
+    auto chunks = [&](int b) {
+        const int wx = xlen[b] - l + 1;
+        return wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
+    };
+    int sum = 0;
+    for (int b = b0; b < b1; ++b)
+        sum += chunks(b);
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < PLAN_THREADS; d <<= 1) {  // inclusive scan of the strip sums
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int b = b0; b < b1; ++b) {
+        first[b] = run;
+        run += chunks(b);
+    }
 }
 
-GKM_API int64_t gkm_index_queue_bytes(int n_pairs, int n_tiles, int nm)
+// bytes of the persistent launch's plan: 8 queue counters (256 B) and first[B]
+static int64_t queue_plan_bytes(int B)
 {
-    return 256 + (int64_t)(n_pairs > 0 ? n_pairs : 0) * n_tiles * nm * 16;
+    return 256 + ((int64_t)(B > 0 ? B : 0) * 4 + 255) / 256 * 256;
+}
+
+GKM_API int64_t gkm_index_queue_bytes(int B, int n_pairs, int n_tiles, int nm)
+{
+    return queue_plan_bytes(B) + (int64_t)(n_pairs > 0 ? n_pairs : 0) * n_tiles * nm * 16;
 }
 
 GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
@@ -2338,33 +2390,38 @@ GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, con
                                   const uint32_t *masks, const uint8_t *levels, int n_masks,
                                   int l, int nm, const uint8_t *wtab, int S, int tile,
                                   int n_tiles, int chunk_windows, int pack_counts,
-                                  const long long *wfix, const int32_t *items, int n_pairs,
-                                  const int32_t *pair_first, int n_blocks, void *workspace,
-                                  const double *tab, const double *dx, int shift, int normalize,
-                                  double bias, double *out, void *stream)
+                                  const long long *wfix, int n_pairs, int n_blocks,
+                                  void *workspace, const double *tab, const double *dx,
+                                  int shift, int normalize, double bias, double *out,
+                                  void *stream)
 {
-    if (wfix == nullptr || workspace == nullptr || out == nullptr || tab == nullptr ||
-        pair_first == nullptr || (n_pairs > 0 && items == nullptr))
-        return fail(-1, "gkm_index_score_queue: wfix, items, pair_first, workspace, tab and "
-                        "out are required");
+    if (wfix == nullptr || workspace == nullptr || out == nullptr || tab == nullptr)
+        return fail(-1, "gkm_index_score_queue: wfix, workspace, tab and out are required");
     if (normalize && dx == nullptr)
         return fail(-1, "gkm_index_score_queue: normalize needs dx");
     if ((uintptr_t)workspace % 256)
         return fail(-1, "gkm_index_score_queue: workspace must be 256-byte aligned");
     if (n_blocks < 1 || n_blocks > 65536)
         return fail(-1, "gkm_index_score_queue: n_blocks must be 1..65536, got %d", n_blocks);
+    if (B <= 0)
+        return 0;
+    if (chunk_windows < 1)
+        return fail(-1, "gkm_index_score_queue: chunk_windows must be >= 1");
+    // workspace: 8 queue counters | first[B] | level sums [n_pairs][n_tiles][nm]
     int *queue = (int *)workspace;
-    void *psum = (uint8_t *)workspace + 256;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, S_(stream), queue, 64);
+    int32_t *pair_first = (int32_t *)((uint8_t *)workspace + 256);
+    void *psum = (uint8_t *)workspace + queue_plan_bytes(B);
+    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
+    hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), xlen, B, l,
+                       chunk_windows, nchunks, queue, pair_first);
     GKM_CHECK_LAUNCH("gkm_index_score_queue");
     int count_bytes = 4;
     int rc = launch_index_tile("gkm_index_score_queue", xwin, xpos, xlen, B, Wx, offsets, ids,
                                vals, records, record_tables, masks, levels, n_masks, l, nm, wtab,
                                S, tile, n_tiles, chunk_windows, pack_counts, -1, 0, psum, wfix,
-                               &count_bytes, stream, items, n_pairs, queue, n_blocks);
-    if (rc || B <= 0)
+                               &count_bytes, stream, pair_first, n_pairs, queue, n_blocks);
+    if (rc)
         return rc;
-    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
     hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
                        (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
                        dx, shift, normalize, bias, out, pair_first);

@@ -254,24 +254,25 @@ int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int32_t *xl
                     int shift, int normalize, double bias, double *out, void *stream);
 
 /* gkm_index_score as one persistent launch over any number of rows: the
- * work items are (row, window chunk) pairs items [n_pairs][2] (each row's
- * ceil((xlen - l + 1) / chunk_windows) chunks, rows in any order, a row's
- * pairs consecutive from pair_first[row]) times the SV tiles; n_blocks blocks
- * (one per CU: the tile fills its LDS) pop them from 8 per-XCD queues and
- * take from the other queues when theirs is empty.  Equal-sized items make
- * the list its own length bucketing: rows of every length share one launch.
- * workspace: gkm_index_queue_bytes bytes, 256-byte aligned (8 queue counters,
- * then one int128 per (pair, tile, level)).  The values are those of
- * gkm_index_score, bit for bit.  Replaces the row-block loop of
- * GkmSVM._score_unbatched (svm.py:584-598). */
-int64_t gkm_index_queue_bytes(int n_pairs, int n_tiles, int nm);
+ * work items are (row, window chunk) pairs -- each row's ceil((xlen - l + 1) /
+ * chunk_windows) chunks (none without windows), n_pairs in all, which the
+ * caller counts from the lengths it uploaded -- times the SV tiles.  A
+ * one-block plan kernel turns xlen into each row's first pair on the device;
+ * then n_blocks blocks (one per CU: the tile fills its LDS) pop items from 8
+ * per-XCD queues and take from the other queues when theirs is empty.
+ * Equal-sized items make the list its own length bucketing: rows of every
+ * length share one launch.  workspace: gkm_index_queue_bytes(B, n_pairs, ...)
+ * bytes, 256-byte aligned (8 queue counters, first[B], then one int128 per
+ * (pair, tile, level)).  The values are those of gkm_index_score, bit for
+ * bit.  Replaces the row-block loop of GkmSVM._score_unbatched
+ * (svm.py:584-598). */
+int64_t gkm_index_queue_bytes(int B, int n_pairs, int n_tiles, int nm);
 int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B,
                           int Wx, const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
                           const uint32_t *records, int record_tables, const uint32_t *masks,
                           const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
                           int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
-                          const long long *wfix, const int32_t *items, int n_pairs,
-                          const int32_t *pair_first, int n_blocks, void *workspace,
+                          const long long *wfix, int n_pairs, int n_blocks, void *workspace,
                           const double *tab, const double *dx, int shift, int normalize,
                           double bias, double *out, void *stream);
 

@@ -302,6 +302,10 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == names
     assert lib.gkm_version() == 1
     assert lib.gkm_planes_words(300) == 300 // 32 + 4
+    # persistent launch workspace: 256 B of queue counters, first[B] rounded
+    # to 256 B, one int128 per (pair, tile, level)
+    assert lib.gkm_index_queue_bytes(1000, 10, 3, 5) == 256 + 4096 + 10 * 3 * 5 * 16
+    assert lib.gkm_index_queue_bytes(1, 0, 3, 5) == 512
 
 
 def test_smo_library_exports_its_declared_symbol():
@@ -324,6 +328,12 @@ def test_library_rejects_bad_arguments_without_gpu_work():
     with pytest.raises(_lib.GkmError, match="l must be"):
         _lib.call("gkm_cross_hist", None, None, 1, 4, None, None, None, None, 1, 4, 16, 1, None,
                   0, 1, 1, -1, 0, None, None)
+    ws = ctypes.create_string_buffer(1024)
+    aligned = (ctypes.addressof(ws) + 255) // 256 * 256
+    with pytest.raises(_lib.GkmError, match="n_blocks"):
+        _lib.call("gkm_index_score_queue", None, None, None, 4, 100, None, None, None, None, 1,
+                  None, None, 1, 11, 4, None, 1000, 1000, 1, 16, 2, 1, 4, 0, aligned, 1,
+                  None, 0, 0, 0.0, 1, None)
 
 
 # -- lane code emulated on the CPU -----------------------------------------------------
