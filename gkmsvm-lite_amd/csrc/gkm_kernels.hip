@@ -1056,6 +1056,23 @@ struct __attribute__((aligned(4))) bounds2 {
     uint32_t x, y;
 };
 
+// In silico mutagenesis through the index (gkm_index_ism): the rows are the
+// windows covering one position -- as they are (old rows, one per position) or
+// with the position's base substituted (new rows 3 p + alt) -- and each block's
+// counts finish per SV instead of going out.
+struct IsmArgs {
+    const double *tab;    // h[m], m < nm
+    const double *svp;    // [S][4] per SV: K_raw(x, s), its finished value, coefficient,
+                          // K_raw(s, s)
+    double *R;            // [positions][S]: each SV's K_raw part from the windows covering p
+    const double *dmut;   // K_raw(mutant, mutant) at ((v lmax + p) stride), or NULL
+    const uint8_t *codes; // the query's base codes
+    double *part;         // new rows: [pairs][n_tiles] sums over the tile's SVs
+    int lmax, dmut_stride, wx;  // wx: the query's windows (window weights)
+    int old_rows, normalize, rbf;
+    double gamma;
+};
+
 struct IndexArgs {
     const uint32_t *xwin;    // [B][Wx] packed query windows (PAD past each row's)
     const int32_t *xpos;     // [B][Wx] position of each window in its row, or NULL
@@ -1078,7 +1095,47 @@ struct IndexArgs {
     int B, Wx, n_masks, l, nm, S, tile, n_tiles, nchunks, chunk_windows;
     int sub_tile;            // PACK 4: SVs per pass of the 16-bit walk
     int64_t col_min, row_base;
+    IsmArgs ism;             // the ISM form's epilogue (ISM instances only)
 };
+
+// ISM epilogue over n SVs of a block's tile from SV s_first (counts laid out
+// [lv / PER][n], PER counts per word): K_raw's part from the block's windows,
+// k = sum_m h[m] N[m] in level order.  Old rows store it as R[p][s]; new rows
+// (p, v) finish K_raw(x, s) - R[p][s] + k like a score (normalised by the
+// mutant's self-kernel zmut and the SV's, RBF) and return this thread's sum
+// of coef_s (f(mutant, s) - f(x, s)) (ism.py:49-52 rescoring, restated per SV).
+// Arguments by value: no copy of the kernel's argument block to the stack.
+template <int PER>
+__device__ __noinline__ double ism_fold(const double *tab, const double *svp, double *R,
+                                        int old_rows, int normalize, int rbf, double gamma,
+                                        int p, double zmut, const uint32_t *cnt, int n,
+                                        int s_first, int nm, int S)
+{
+    constexpr int BITS = 32 / PER;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += INDEX_THREADS) {
+        double k = 0.0;
+        for (int lv = 0; lv < nm; ++lv) {
+            uint32_t c = cnt[(lv / PER) * n + i];
+            if (PER > 1)
+                c = (c >> (BITS * (lv % PER))) & ((1u << BITS) - 1u);
+            k += tab[lv] * (double)c;
+        }
+        const int s = s_first + i;
+        if (old_rows) {
+            R[(int64_t)p * S + s] = k;
+            continue;
+        }
+        const double *q = svp + (int64_t)s * 4;
+        double kn = q[0] - R[(int64_t)p * S + s] + k;
+        if (normalize)
+            kn = kn / fmax(sqrt(zmut * q[3]), 1e-10);
+        if (rbf)
+            kn = exp(gamma * (kn - 1.0));
+        acc += q[2] * (kn - q[1]);
+    }
+    return acc;
+}
 
 // Block = (SV tile, chunk of query windows, query row).  Its LDS holds the
 // tile's counts [tile][nm]; each wave takes a query window and its lanes
@@ -1116,13 +1173,14 @@ struct IndexArgs {
 // the row's last window, exit at once.
 // One work item of the index query: window chunk `chunk` of row b against SV
 // tile tile_id; with wfix its level sums go to slot `slot` of the psum output.
-template <int PACK, int RK, bool W16>
+template <int PACK, int RK, bool W16, bool ISM = false>
 __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk, int tile_id,
                                             int64_t slot)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     uint32_t *cnt = (uint32_t *)smem;
     const int wx = a.xlen[b] - a.l + 1;
+    const int wxw = ISM ? a.ism.wx : wx;  // the window count the window weights take
     const int i0 = chunk * a.chunk_windows;
     if (i0 >= wx)
         return;  // no windows of this row: the reduce skips this chunk
@@ -1141,19 +1199,54 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     __syncthreads();
     // bsum[lv] += sum over SVs s0 .. s0 + n - 1 of the tile of wfix times the
     // count of level lv, counts laid out [lv / per][n] with per counts per word
+    double iacc = 0.0;  // ISM: this thread's sum over its SVs
+    int ism_p = b;       // ISM: the row's position and mutant self-kernel
+    double ism_z = 1.0;
+    if constexpr (ISM) {
+        if (!a.ism.old_rows) {
+            ism_p = b / 3;
+            const int alt = b - 3 * ism_p, ref = a.ism.codes[ism_p] & 3;
+            const int v = alt + (alt >= ref ? 1 : 0);
+            if (a.ism.normalize)
+                ism_z = a.ism.dmut[((int64_t)v * a.ism.lmax + ism_p) * a.ism.dmut_stride];
+        }
+    }
     auto fold = [&](int n, int s0, auto per) {  // per: std::integral_constant
         constexpr int PER = decltype(per)::value;
-        if constexpr (W16)
+        if constexpr (ISM)
+            iacc += ism_fold<PER>(a.ism.tab, a.ism.svp, a.ism.R, a.ism.old_rows,
+                                  a.ism.normalize, a.ism.rbf, a.ism.gamma, ism_p, ism_z, cnt, n,
+                                  s_base + s0, a.nm, a.S);
+        else if constexpr (W16)
             fold_level_sums_call<PER>(cnt, n, a.wfix + s_base + s0, a.nm, bsum, red128);
         else
             fold_level_sums<12, PER>(cnt, n, a.wfix + s_base + s0, a.nm, bsum, red128);
     };
+    // exact scoring and ISM finish the counts in the block
+    const bool fused = ISM || a.wfix != nullptr;
     using per4 = std::integral_constant<int, 4>;
     using per2 = std::integral_constant<int, 2>;
     using per1 = std::integral_constant<int, 1>;
     auto emit = [&]() {  // the level sums out: slot * nm + level
-        if (threadIdx.x < a.nm)
+        if constexpr (ISM) {  // new rows: the tile's sum to part[slot] (fixed order)
+            if (a.ism.old_rows)
+                return;
+            __shared__ double redd[INDEX_THREADS / WAVE];
+            double v = iacc;
+            for (int o = WAVE / 2; o > 0; o >>= 1)
+                v += __shfl_xor(v, o, WAVE);
+            if (lane == 0)
+                redd[wave] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0.0;
+                for (int k = 0; k < INDEX_THREADS / WAVE; ++k)
+                    t += redd[k];
+                a.ism.part[slot] = t;
+            }
+        } else if (threadIdx.x < a.nm) {
             ((i128 *)a.partial)[slot * a.nm + threadIdx.x] = bsum[threadIdx.x];
+        }
     };
     const uint32_t nl = 1u << (2 * a.l);
     const int i1 = min(wx, i0 + a.chunk_windows);
@@ -1300,7 +1393,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                     cpos_next = load_pos(ci + INDEX_THREADS / WAVE);
                 }
             }
-            const uint32_t aw = a.wtab ? window_weight(a.wtab, wx, pos) : 1u;
+            const uint32_t aw = a.wtab ? window_weight(a.wtab, wxw, pos) : 1u;
             if (__ballot(!small) == 0) {
                 // lane walk: every probe's next INDEX_WALK entries per round
                 int cmax = c[0];
@@ -1445,7 +1538,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             carried += red64[k];
         uint32_t *dst = (uint32_t *)a.partial + off;
         if (carried == 0) {
-            if (a.wfix) {
+            if (fused) {
                 fold(ns, 0, per2{});
                 emit();
                 return;
@@ -1470,7 +1563,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 atomicAdd(&cnt[lv * n + r], add);
             });
             __syncthreads();
-            if (a.wfix) {
+            if (fused) {
                 fold(n, s0, per1{});
                 continue;
             }
@@ -1479,7 +1572,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 dst[(int64_t)s0 * a.nm + i] = cnt[lv * n + sid];
             }
         }
-        if (a.wfix)
+        if (fused)
             emit();
         return;
     } else if constexpr (PACK == 4) {
@@ -1516,7 +1609,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         uint8_t *hi = lo + n_all;
         uint8_t *flag = (uint8_t *)a.partial + 2 * n_all +
                         ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id;
-        if (carried == 0 && a.wfix) {
+        if (carried == 0 && fused) {
             fold(ns, 0, per4{});
             emit();
             return;
@@ -1552,7 +1645,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 atomicAdd(&cnt[(lv >> 1) * n + r], 1u << (16 * (lv & 1)));
             });
             __syncthreads();
-            if (a.wfix) {
+            if (fused) {
                 fold(n, s0, per2{});
                 continue;
             }
@@ -1564,7 +1657,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 hi[e] = (uint8_t)(c >> 8);
             }
         }
-        if (a.wfix) {
+        if (fused) {
             emit();
             return;
         }
@@ -1582,7 +1675,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 atomicAdd(&cnt[lv * ns + sid], add);
         });
         __syncthreads();
-        if (a.wfix) {
+        if (fused) {
             fold(ns, 0, std::integral_constant<int, PACK == 2 ? 2 : 1>{});
             emit();
         } else if (PACK == 2) {
@@ -1621,7 +1714,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
 // L2; a block whose queue is empty takes items from the others.  Item k of
 // queue x is pair x + 8 (k / n_tiles), tile k % n_tiles; its level sums go to
 // slot pair * n_tiles + tile.
-template <int PACK, int RK, bool W16 = false>
+template <int PACK, int RK, bool W16 = false, bool ISM = false>
 __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile_queue(IndexArgs a)
 {
     __shared__ int pick[4];
@@ -1659,7 +1752,7 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile_queue(IndexArgs a)
         __syncthreads();  // every thread has its item before thread 0 picks again
         if (rc < 0)
             return;
-        index_block<PACK, RK, W16>(a, row, chunk, tile, (int64_t)rc * a.n_tiles + tile);
+        index_block<PACK, RK, W16, ISM>(a, row, chunk, tile, (int64_t)rc * a.n_tiles + tile);
         __syncthreads();
     }
 }
@@ -2106,7 +2199,8 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
                              int n_tiles, int chunk_windows, int pack_counts, int64_t col_min,
                              int64_t row_base, void *partial, const long long *wfix,
                              int *count_bytes, void *stream, const int32_t *pair_first = nullptr,
-                             int n_pairs = 0, int *queue = nullptr, int n_blocks = 0)
+                             int n_pairs = 0, int *queue = nullptr, int n_blocks = 0,
+                             const IsmArgs *ism = nullptr)
 {
     int rc = check_l_nm(l, nm);
     if (rc)
@@ -2117,6 +2211,8 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
         return fail(-1, "%s: bad tiling (tile=%d n_tiles=%d S=%d chunk=%d)", who, tile,
                     n_tiles, S, chunk_windows);
     const bool persistent = pair_first != nullptr;
+    if (ism && !persistent)
+        return fail(-1, "%s: the ISM form is a persistent launch", who);
     if (B > 65535 && !persistent)
         return fail(-1, "%s: at most 65535 rows per call, got %d", who, B);
     if (pack_counts < 0 || pack_counts > 2)
@@ -2152,6 +2248,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     a.wfix = wfix;
     a.pair_first = pair_first;
     a.queue = queue;
+    a.ism = ism ? *ism : IsmArgs{};
     a.n_pairs = n_pairs;
     a.B = B;
     a.Wx = Wx;
@@ -2176,7 +2273,15 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
             : pack == 2 ? (rk ? k_index_tile<2, 1> : k_index_tile<2, 0>)
             : rk == 2   ? k_index_tile<1, 2>
             : rk == 1   ? k_index_tile<1, 1> : k_index_tile<1, 0>;
-    else
+    else if (ism) {  // the index formats of the ISM form: records, 8- or 16-bit counts
+        kfn = w16 && rk == 2             ? k_index_tile_queue<2, 2, true, true>
+            : !w16 && pack == 4 && rk == 1 ? k_index_tile_queue<4, 1, false, true>
+            : !w16 && pack == 2 && rk == 1 ? k_index_tile_queue<2, 1, false, true>
+                                          : nullptr;
+        if (kfn == nullptr)
+            return fail(-2, "%s: ISM through the index takes an index with bucket records "
+                            "and 8- or 16-bit counts", who);
+    } else
         kfn = w16       ? (rk ? k_index_tile_queue<2, 2, true> : k_index_tile_queue<2, 0, true>)
             : pack == 4 ? (rk ? k_index_tile_queue<4, 1> : k_index_tile_queue<4, 0>)
             : pack == 2 ? (rk ? k_index_tile_queue<2, 1> : k_index_tile_queue<2, 0>)
@@ -2426,6 +2531,173 @@ GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, con
                        (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
                        dx, shift, normalize, bias, out, pair_first);
     GKM_CHECK_LAUNCH("gkm_index_score_queue");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ISM through the l-mer index
+// ---------------------------------------------------------------------------
+
+// The rows of one query's ISM: old row p [l] holds the query's windows covering
+// position p (window i at slot i - max(0, p - l + 1), its position i kept for
+// the window weights; PAD past the last); new row 3 p + alt the same windows
+// with base p replaced by the alt-th base other than the reference (digit
+// p - i of window i).  Row lengths are windows + l - 1.
+__global__ void k_ism_rows(const uint32_t *__restrict__ xwin, const uint8_t *__restrict__ codes,
+                           int lx, int l, uint32_t *__restrict__ owin, int32_t *__restrict__ opos,
+                           int32_t *__restrict__ olen, uint32_t *__restrict__ nwin,
+                           int32_t *__restrict__ npos, int32_t *__restrict__ nlen)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n_old = (int64_t)lx * l;
+    if (t >= 4 * n_old)
+        return;
+    const int wx = lx - l + 1;
+    const bool old = t < n_old;
+    const int64_t u = old ? t : t - n_old;
+    const int row = (int)(u / l), k = (int)(u - (int64_t)row * l);
+    const int p = old ? row : row / 3;
+    const int lo = max(0, p - l + 1), hi = min(p, wx - 1);
+    const int cnt = hi - lo + 1;
+    uint32_t w = 0xFFFFFFFFu;
+    int pos = 0;
+    if (k < cnt) {
+        const int i = lo + k;
+        w = xwin[i];
+        pos = i;
+        if (!old) {
+            const int alt = row - 3 * p, ref = codes[p] & 3;
+            const uint32_t v = (uint32_t)(alt + (alt >= ref ? 1 : 0));
+            const int j = p - i;
+            w = (w & ~(3u << (2 * j))) | (v << (2 * j));
+        }
+    }
+    (old ? owin : nwin)[u] = w;
+    (old ? opos : npos)[u] = pos;
+    if (k == 0)
+        (old ? olen : nlen)[row] = cnt + l - 1;
+}
+
+// out[v][p] = the sum over tiles (in order) of new row (p, v)'s tile sums.
+__global__ void k_ism_reduce(const double *__restrict__ part, const int32_t *__restrict__ first,
+                             int n_tiles, const uint8_t *__restrict__ codes, int lx, int lmax,
+                             double *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * lx)
+        return;
+    const int p = r / 3, alt = r - 3 * p, ref = codes[p] & 3;
+    const int v = alt + (alt >= ref ? 1 : 0);
+    double t = 0.0;
+    for (int k = 0; k < n_tiles; ++k)
+        t += part[(int64_t)first[r] * n_tiles + k];
+    out[(int64_t)v * lmax + p] = t;
+}
+
+static int64_t align256(int64_t n)
+{
+    return (n + 255) / 256 * 256;
+}
+
+GKM_API int64_t gkm_index_ism_bytes(int lx, int l, int S, int n_tiles)
+{
+    if (lx < l || l < 1 || S < 1 || n_tiles < 1)
+        return 0;
+    const int64_t rows_old = lx, rows_new = 3 * (int64_t)lx;
+    return align256(rows_old * S * 8) +                      // R
+           align256(rows_old * l * 4) * 2 + align256(rows_old * 4) +  // old rows
+           align256(rows_new * l * 4) * 2 + align256(rows_new * 4) +  // new rows
+           queue_plan_bytes((int)rows_old) + queue_plan_bytes((int)rows_new) +
+           align256(rows_new * n_tiles * 8);                 // tile sums
+}
+
+GKM_API int gkm_index_ism(const uint32_t *xwin, const uint8_t *codes, int lx, int lmax,
+                          const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                          const uint32_t *records, int record_tables, const uint32_t *masks,
+                          const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                          int S, int tile, int n_tiles, int pack_counts, const double *tab,
+                          const double *svp, const double *dmut, int dmut_stride, int normalize,
+                          int rbf, double gamma, int n_blocks, void *workspace, double *out,
+                          void *stream)
+{
+    if (xwin == nullptr || codes == nullptr || tab == nullptr || svp == nullptr ||
+        workspace == nullptr || out == nullptr)
+        return fail(-1, "gkm_index_ism: xwin, codes, tab, svp, workspace and out are required");
+    if (normalize && dmut == nullptr)
+        return fail(-1, "gkm_index_ism: normalize needs dmut");
+    if ((uintptr_t)workspace % 256)
+        return fail(-1, "gkm_index_ism: workspace must be 256-byte aligned");
+    if (n_blocks < 1 || n_blocks > 65536)
+        return fail(-1, "gkm_index_ism: n_blocks must be 1..65536, got %d", n_blocks);
+    if (lx > lmax)
+        return fail(-1, "gkm_index_ism: length %d exceeds lmax %d", lx, lmax);
+    if (lx < l || S <= 0 || n_masks <= 0)
+        return 0;  // no windows (or no SVs): every delta is 0
+    if ((int64_t)lx * 3 > (1 << 28))
+        return fail(-1, "gkm_index_ism: a query of %d bp is too long", lx);
+    uint8_t *w = (uint8_t *)workspace;
+    const int rows_old = lx, rows_new = 3 * lx;
+    double *R = (double *)w;
+    w += align256((int64_t)rows_old * S * 8);
+    uint32_t *owin = (uint32_t *)w;
+    w += align256((int64_t)rows_old * l * 4);
+    int32_t *opos = (int32_t *)w;
+    w += align256((int64_t)rows_old * l * 4);
+    int32_t *olen = (int32_t *)w;
+    w += align256((int64_t)rows_old * 4);
+    uint32_t *nwin = (uint32_t *)w;
+    w += align256((int64_t)rows_new * l * 4);
+    int32_t *npos = (int32_t *)w;
+    w += align256((int64_t)rows_new * l * 4);
+    int32_t *nlen = (int32_t *)w;
+    w += align256((int64_t)rows_new * 4);
+    uint8_t *plan_old = w;
+    w += queue_plan_bytes(rows_old);
+    uint8_t *plan_new = w;
+    w += queue_plan_bytes(rows_new);
+    double *part = (double *)w;
+    const int64_t nth = 4 * (int64_t)lx * l;
+    hipLaunchKernelGGL(k_ism_rows, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0,
+                       S_(stream), xwin, codes, lx, l, owin, opos, olen, nwin, npos, nlen);
+    GKM_CHECK_LAUNCH("gkm_index_ism");
+    IsmArgs m{};
+    m.tab = tab;
+    m.svp = svp;
+    m.R = R;
+    m.dmut = dmut;
+    m.codes = codes;
+    m.part = part;
+    m.lmax = lmax;
+    m.dmut_stride = dmut_stride;
+    m.wx = lx - l + 1;
+    m.normalize = normalize;
+    m.rbf = rbf;
+    m.gamma = gamma;
+    // every row holds 1..l windows: one work item (pair) per row and tile
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool old = pass == 0;
+        const int B = old ? rows_old : rows_new;
+        uint8_t *plan = old ? plan_old : plan_new;
+        int *queue = (int *)plan;
+        int32_t *first = (int32_t *)(plan + 256);
+        const int32_t *len = old ? olen : nlen;
+        hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), len, B, l, l,
+                           1, queue, first);
+        GKM_CHECK_LAUNCH("gkm_index_ism");
+        m.old_rows = old ? 1 : 0;
+        int count_bytes = 4;
+        int rc = launch_index_tile("gkm_index_ism", old ? owin : nwin, old ? opos : npos, len, B,
+                                   l, offsets, ids, vals, records, record_tables, masks, levels,
+                                   n_masks, l, nm, wtab, S, tile, n_tiles, l, pack_counts, -1, 0,
+                                   nullptr, nullptr, &count_bytes, stream, first, B, queue,
+                                   n_blocks, &m);
+        if (rc)
+            return rc;
+    }
+    hipLaunchKernelGGL(k_ism_reduce, dim3((unsigned)((rows_new + 255) / 256)), dim3(256), 0,
+                       S_(stream), part, (const int32_t *)(plan_new + 256), n_tiles, codes, lx,
+                       lmax, out);
+    GKM_CHECK_LAUNCH("gkm_index_ism");
     return 0;
 }
 

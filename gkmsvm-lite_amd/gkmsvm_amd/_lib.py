@@ -46,6 +46,9 @@ SIGNATURES = {
     "gkm_index_score": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P, P, P,
                         P, I, I, D, P, P],
     "gkm_index_queue_bytes": [I, I, I, I],
+    "gkm_index_ism_bytes": [I, I, I, I],
+    "gkm_index_ism": [P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, P, P, P, I, I, I,
+                      D, I, P, P, P],
     "gkm_index_score_queue": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P,
                               I, I, P, P, P, I, I, D, P, P],
     "gkm_scratch_bytes": [],
@@ -68,7 +71,7 @@ SIGNATURES = {
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p, "gkm_scratch_bytes": ctypes.c_int64,
             "gkm_index_partial_bytes": ctypes.c_int64, "gkm_index_psum_bytes": ctypes.c_int64,
-            "gkm_index_queue_bytes": ctypes.c_int64}
+            "gkm_index_queue_bytes": ctypes.c_int64, "gkm_index_ism_bytes": ctypes.c_int64}
 
 _lib = None
 

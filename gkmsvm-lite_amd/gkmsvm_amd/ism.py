@@ -1,7 +1,10 @@
 """In silico mutagenesis on the GPU (gkmsvm-lite ism.py:21-320).
 
 For every position p and base v, the score change f(x with x_p = v) - f(x).
-The kernel change of a substitution touches only the windows covering p: on a
+Three paths: the l-mer table (kernels without RBF), the l-mer index (each
+SV's K_raw(mutant, s) = K_raw(x, s) - K_raw(old windows, s) + K_raw(new
+windows, s), the windows covering p probed as they are and substituted), and
+the sweep.  The kernel change of a substitution touches only the windows covering p: on a
 diagonal of the (x, SV) pair, a window with c mismatches gains one
 (h[c+1] - h[c]) where x_p matched the SV, and loses one (h[c-1] - h[c]) for the
 base v equal to the SV's where it did not.  One sweep per (x, SV) pair
@@ -24,10 +27,12 @@ def _sv_chunk(B: int, S: int) -> int:
 
 
 def ism(model, x, *, lengths=None, batch_size: int | None = None,
-        verbose: bool = False) -> torch.Tensor:
+        verbose: bool = False, use_index=None) -> torch.Tensor:
     """[B, 4, L] float64 score deltas for every single-base substitution (0 for
     the reference base and past each sequence's length, NaN rows for sequences
-    shorter than l)."""
+    shorter than l).  Without the l-mer table the deltas come from the l-mer
+    index (use_index True, or "auto" -- the model's use_lmer_index by default
+    -- where the cost model prefers it) or from the sweep kernel."""
     dev = model._require_device()
     batch = read_batch(x, lengths)
     kp = model._params()
@@ -47,10 +52,13 @@ def ism(model, x, *, lengths=None, batch_size: int | None = None,
             from tqdm import tqdm
 
             starts = tqdm(starts, desc="ISM")
+        mode = model.use_lmer_index if use_index is None else use_index
         for s in starts:
             e = min(B, s + bs)
             if table is not None:
                 _ism_rows_table(model, codes[s:e], lens[s:e], table, out[s:e])
+            elif E.ism_use_index(model._svs(), lens[s:e], mode):
+                _ism_rows_index(model, codes[s:e], lens[s:e], out[s:e])
             else:
                 _ism_rows(model, codes[s:e], lens[s:e], out[s:e])
     return mask_positions(out, batch, short)
@@ -84,6 +92,38 @@ def _ism_rows_table(model, codes, lens, table, out):
     _lib.call("gkm_table_ism", E._p(fwd), E._p(xs.len), E._p(xs.codes), B, fwd.shape[1],
               xs.lmax, kp.l, E._p(wt), E._p(table), E._p(raw), E._p(dx), E._p(dmut),
               int(kp.normalize), E._p(out), E.stream_handle())
+
+
+def _ism_rows_index(model, codes, lens, out, group: int = 64):
+    """ISM through the l-mer index (kernels/lmer_index.py:326-376): each
+    query's K_raw(x, s) from its index histogram, then gkm_index_ism per query
+    (the windows covering each position, as they are and substituted, probed
+    in one persistent launch each)."""
+    kp = model._params()
+    sv = model._svs()
+    xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
+    dx = dmut = None
+    if kp.normalize:
+        dx = E.self_kraw(kp, xs)
+        dmut = _mutant_diag(kp, xs, dx)
+    tab = kp.tab[: kp.nm]
+    for g0 in range(0, xs.n, group):
+        g1 = min(xs.n, g0 + group)
+        hist = E.index_hist(kp, sv.index(), xs, g0, g1, sv.S)
+        kref = torch.zeros((g1 - g0, sv.S), dtype=torch.float64, device=kp.device)
+        for lv in range(kp.nm):  # level order, as a score's combine
+            kref += tab[lv] * hist[:, :, lv].to(torch.float64)
+        del hist
+        fref = kref
+        if kp.normalize:
+            fref = kref / torch.sqrt(dx[g0:g1, None] * sv.ds[None, :]).clamp(min=1e-10)
+        if kp.rbf:
+            fref = torch.exp(kp.gamma * (fref - 1.0))
+        for b in range(g0, g1):
+            if lens[b] < kp.l:
+                continue
+            svp = torch.stack([kref[b - g0], fref[b - g0], sv.coef, sv.ds], dim=1).contiguous()
+            E.index_ism(sv, xs, b, svp, dmut, out[b])
 
 
 def _ism_rows(model, codes, lens, out):
