@@ -1057,19 +1057,20 @@ struct __attribute__((aligned(4))) bounds2 {
 };
 
 // In silico mutagenesis through the index (gkm_index_ism): the rows are the
-// windows covering one position -- as they are (old rows, one per position) or
-// with the position's base substituted (new rows 3 p + alt) -- and each block's
-// counts finish per SV instead of going out.
+// query's windows one by one (window rows) or the windows covering position p
+// with its base substituted (new rows 3 p + alt), and each block's counts
+// finish per SV instead of going out.
 struct IsmArgs {
     const double *tab;    // h[m], m < nm
     const double *svp;    // [S][4] per SV: K_raw(x, s), its finished value, coefficient,
                           // K_raw(s, s)
-    double *R;            // [positions][S]: each SV's K_raw part from the windows covering p
+    double *R;            // window rows: C [windows][S] out (each window's K_raw part);
+                          // new rows: R [positions][S] in (the windows covering p)
     const double *dmut;   // K_raw(mutant, mutant) at ((v lmax + p) stride), or NULL
     const uint8_t *codes; // the query's base codes
     double *part;         // new rows: [pairs][n_tiles] sums over the tile's SVs
     int lmax, dmut_stride, wx;  // wx: the query's windows (window weights)
-    int old_rows, normalize, rbf;
+    int old_rows, normalize, rbf;  // old_rows: the window rows
     double gamma;
 };
 
@@ -1100,7 +1101,7 @@ struct IndexArgs {
 
 // ISM epilogue over n SVs of a block's tile from SV s_first (counts laid out
 // [lv / PER][n], PER counts per word): K_raw's part from the block's windows,
-// k = sum_m h[m] N[m] in level order.  Old rows store it as R[p][s]; new rows
+// k = sum_m h[m] N[m] in level order.  Window rows store it as C[i][s]; new rows
 // (p, v) finish K_raw(x, s) - R[p][s] + k like a score (normalised by the
 // mutant's self-kernel zmut and the SV's, RBF) and return this thread's sum
 // of coef_s (f(mutant, s) - f(x, s)) (ism.py:49-52 rescoring, restated per SV).
@@ -1259,9 +1260,36 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     auto walk = [&](auto count) {
         const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
         // two (window, mask group) pairs' records in flight: the pair walked
-        // now was fetched two steps ago, into the buffer (A or B) it is read from
-        int ci = i0 + wave, cg = 0;  // the pair being walked
-        int ni = ci, ng = 0;         // the pair being fetched
+        // now was fetched two steps ago, into the buffer (A or B) it is read from.
+        // ISM rows hold 1..l windows: their pairs are dealt to the waves in turn
+        // (pair k: window k / n_groups, group k % n_groups), so every wave has
+        // work; otherwise a wave walks every group of one window, then the
+        // window 16 on.
+        constexpr bool FLAT = ISM;
+        constexpr int WSTEP = FLAT ? 1 : INDEX_THREADS / WAVE;  // a wave's next window
+        int ci = FLAT ? i0 + wave / n_groups : i0 + wave;  // the pair being walked
+        int cg = FLAT ? wave % n_groups : 0;
+        int ni = ci, ng = cg;  // the pair being fetched
+        // (i, g) to this wave's next pair; true when the window changed
+        auto advance = [&](int &i, int &g) -> bool {
+            if constexpr (FLAT) {
+                g += INDEX_THREADS / WAVE;
+                bool moved = false;
+                while (g >= n_groups) {
+                    g -= n_groups;
+                    ++i;
+                    moved = true;
+                }
+                return moved;
+            } else {
+                if (++g == n_groups) {
+                    g = 0;
+                    i += INDEX_THREADS / WAVE;
+                    return true;
+                }
+                return false;
+            }
+        };
         uint4 nra[INDEX_PROBES][2], nrc[INDEX_PROBES][2];  // record (RK 1, 2), or (start, end)
         int nlva[INDEX_PROBES], nlvc[INDEX_PROBES];
         // Loads complete in order, so a wait for a load also waits for every
@@ -1282,7 +1310,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         auto load_window = [&](int i) -> uint32_t {  // windows past the chunk: no probes
             return a.xwin[(int64_t)b * a.Wx + min(i, a.Wx - 1)];
         };
-        uint32_t fw = load_window(ni), fw_next = load_window(ni + INDEX_THREADS / WAVE);
+        uint32_t fw = load_window(ni), fw_next = load_window(ni + WSTEP);
         // the walked window's position in its row (window weights), a window ahead
         auto load_pos = [&](int i) -> int {
             return a.xpos[(int64_t)b * a.Wx + min(i, a.Wx - 1)];
@@ -1290,9 +1318,9 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         int cpos = 0, cpos_next = 0;
         if (a.xpos) {
             cpos = load_pos(ci);
-            cpos_next = load_pos(ci + INDEX_THREADS / WAVE);
+            cpos_next = load_pos(ci + WSTEP);
         }
-        load_masks(0);
+        load_masks(cg);
         auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
             const uint32_t w = ni < i1 ? fw : nl;  // PAD past the chunk
             uint32_t mk[INDEX_PROBES], lb[INDEX_PROBES];
@@ -1301,7 +1329,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 mk[p] = pm[p];
                 lb[p] = pl[p];
             }
-            load_masks(ng + 1 == n_groups ? 0 : ng + 1);
+            if constexpr (FLAT) {
+                int ti = ni, tg = ng;
+                advance(ti, tg);
+                load_masks(tg);
+            } else {
+                load_masks(ng + 1 == n_groups ? 0 : ng + 1);
+            }
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
                 const int t = (ng * INDEX_PROBES + p) * WAVE + lane;
@@ -1335,7 +1369,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                     }
                 }
             }
-            if (++ng == n_groups) {
+            if constexpr (FLAT) {
+                const int prev = ni;
+                if (advance(ni, ng)) {
+                    fw = ni == prev + 1 ? fw_next : load_window(ni);
+                    fw_next = load_window(ni + 1);
+                }
+            } else if (++ng == n_groups) {
                 ng = 0;
                 ni += INDEX_THREADS / WAVE;
                 fw = fw_next;
@@ -1385,7 +1425,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 }
             fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
             const int pos = a.xpos ? cpos : i;  // loaded a window ahead
-            if (++cg == n_groups) {
+            if constexpr (FLAT) {
+                const int prev = ci;
+                if (advance(ci, cg) && a.xpos) {
+                    cpos = ci == prev + 1 ? cpos_next : load_pos(ci);
+                    cpos_next = load_pos(ci + 1);
+                }
+            } else if (++cg == n_groups) {
                 cg = 0;
                 ci += INDEX_THREADS / WAVE;
                 if (a.xpos) {
@@ -2538,44 +2584,101 @@ GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, con
 // ISM through the l-mer index
 // ---------------------------------------------------------------------------
 
-// The rows of one query's ISM: old row p [l] holds the query's windows covering
-// position p (window i at slot i - max(0, p - l + 1), its position i kept for
-// the window weights; PAD past the last); new row 3 p + alt the same windows
-// with base p replaced by the alt-th base other than the reference (digit
-// p - i of window i).  Row lengths are windows + l - 1.
+// The rows of one query's ISM.  Window row i: the query's window i alone
+// (its position i kept for the window weight).  New row 3 p + alt [l]: the
+// windows covering position p (window i at slot i - max(0, p - l + 1); PAD
+// past the last) with base p replaced by the alt-th base other than the
+// reference (digit p - i of window i).  Row lengths are windows + l - 1.
 __global__ void k_ism_rows(const uint32_t *__restrict__ xwin, const uint8_t *__restrict__ codes,
-                           int lx, int l, uint32_t *__restrict__ owin, int32_t *__restrict__ opos,
-                           int32_t *__restrict__ olen, uint32_t *__restrict__ nwin,
-                           int32_t *__restrict__ npos, int32_t *__restrict__ nlen)
+                           int lx, int l, int32_t *__restrict__ wpos, int32_t *__restrict__ wlen,
+                           uint32_t *__restrict__ nwin, int32_t *__restrict__ npos,
+                           int32_t *__restrict__ nlen)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n_old = (int64_t)lx * l;
-    if (t >= 4 * n_old)
-        return;
     const int wx = lx - l + 1;
-    const bool old = t < n_old;
-    const int64_t u = old ? t : t - n_old;
-    const int row = (int)(u / l), k = (int)(u - (int64_t)row * l);
-    const int p = old ? row : row / 3;
+    const int64_t n_new = 3 * (int64_t)lx * l;
+    if (t < wx) {
+        wpos[t] = (int32_t)t;
+        wlen[t] = l;
+    }
+    if (t >= n_new)
+        return;
+    const int row = (int)(t / l), k = (int)(t - (int64_t)row * l);
+    const int p = row / 3;
     const int lo = max(0, p - l + 1), hi = min(p, wx - 1);
     const int cnt = hi - lo + 1;
     uint32_t w = 0xFFFFFFFFu;
     int pos = 0;
     if (k < cnt) {
         const int i = lo + k;
-        w = xwin[i];
+        const int alt = row - 3 * p, ref = codes[p] & 3;
+        const uint32_t v = (uint32_t)(alt + (alt >= ref ? 1 : 0));
+        const int j = p - i;
+        w = (xwin[i] & ~(3u << (2 * j))) | (v << (2 * j));
         pos = i;
-        if (!old) {
-            const int alt = row - 3 * p, ref = codes[p] & 3;
-            const uint32_t v = (uint32_t)(alt + (alt >= ref ? 1 : 0));
-            const int j = p - i;
-            w = (w & ~(3u << (2 * j))) | (v << (2 * j));
-        }
     }
-    (old ? owin : nwin)[u] = w;
-    (old ? opos : npos)[u] = pos;
+    nwin[t] = w;
+    npos[t] = pos;
     if (k == 0)
-        (old ? olen : nlen)[row] = cnt + l - 1;
+        nlen[row] = cnt + l - 1;
+}
+
+// From the windows' parts C [wx][S]: R[p][s] = the sum over the windows
+// covering p for the positions of this block's segment.
+constexpr int ISM_SEG = 64;  // positions per block row
+__global__ __launch_bounds__(256) void k_ism_cover(const double *__restrict__ C, int wx, int lx,
+                                                   int l, int S, double *__restrict__ R)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S)
+        return;
+    const int p0 = blockIdx.y * ISM_SEG, p1 = min(lx, p0 + ISM_SEG);
+    // R at p0 summed directly, then slid along the segment: + the window
+    // entering, - the window leaving (a fixed order: deterministic)
+    int lo = max(0, p0 - l + 1), hi = min(p0, wx - 1);
+    double r = 0.0;
+    for (int i = lo; i <= hi; ++i)
+        r += C[(int64_t)i * S + s];
+    R[(int64_t)p0 * S + s] = r;
+#pragma unroll 4
+    for (int p = p0 + 1; p < p1; ++p) {
+        const int nlo = max(0, p - l + 1), nhi = min(p, wx - 1);
+        const double add = nhi > hi ? C[(int64_t)nhi * S + s] : 0.0;
+        const double sub = nlo > lo ? C[(int64_t)lo * S + s] : 0.0;
+        r = r + add - sub;
+        lo = nlo;
+        hi = nhi;
+        R[(int64_t)p * S + s] = r;
+    }
+}
+
+// Per SV into svp [S][4]: K_raw(x, s) -- the windows fall into the disjoint
+// groups [k l, k l + l) that R[k l + l - 1] covers -- its finished value
+// (normalised by K_raw(x, x) and K_raw(s, s), RBF), the coefficient and
+// K_raw(s, s).
+__global__ __launch_bounds__(256) void k_ism_ref(const double *__restrict__ R, int wx, int l,
+                                                 int S, const double *__restrict__ coef,
+                                                 const double *__restrict__ ds,
+                                                 const double *__restrict__ dx, int normalize,
+                                                 int rbf, double gamma, double *__restrict__ svp)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S)
+        return;
+    double k = 0.0;
+#pragma unroll 8
+    for (int g = 0; g < wx; g += l)
+        k += R[(int64_t)(g + l - 1) * S + s];
+    double f = k;
+    if (normalize)
+        f = k / fmax(sqrt(dx[0] * ds[s]), 1e-10);
+    if (rbf)
+        f = exp(gamma * (f - 1.0));
+    double *q = svp + (int64_t)s * 4;
+    q[0] = k;
+    q[1] = f;
+    q[2] = coef[s];
+    q[3] = ds[s];
 }
 
 // out[v][p] = the sum over tiles (in order) of new row (p, v)'s tile sums.
@@ -2603,12 +2706,13 @@ GKM_API int64_t gkm_index_ism_bytes(int lx, int l, int S, int n_tiles)
 {
     if (lx < l || l < 1 || S < 1 || n_tiles < 1)
         return 0;
-    const int64_t rows_old = lx, rows_new = 3 * (int64_t)lx;
-    return align256(rows_old * S * 8) +                      // R
-           align256(rows_old * l * 4) * 2 + align256(rows_old * 4) +  // old rows
+    const int64_t wx = lx - l + 1, rows_new = 3 * (int64_t)lx;
+    return align256(wx * S * 8) + align256((int64_t)lx * S * 8) +  // C, R
+           align256((int64_t)S * 32) +                              // svp
+           align256(wx * 4) * 2 +                                   // window rows
            align256(rows_new * l * 4) * 2 + align256(rows_new * 4) +  // new rows
-           queue_plan_bytes((int)rows_old) + queue_plan_bytes((int)rows_new) +
-           align256(rows_new * n_tiles * 8);                 // tile sums
+           queue_plan_bytes((int)wx) + queue_plan_bytes((int)rows_new) +
+           align256(rows_new * n_tiles * 8);                        // tile sums
 }
 
 GKM_API int gkm_index_ism(const uint32_t *xwin, const uint8_t *codes, int lx, int lmax,
@@ -2616,15 +2720,16 @@ GKM_API int gkm_index_ism(const uint32_t *xwin, const uint8_t *codes, int lx, in
                           const uint32_t *records, int record_tables, const uint32_t *masks,
                           const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
                           int S, int tile, int n_tiles, int pack_counts, const double *tab,
-                          const double *svp, const double *dmut, int dmut_stride, int normalize,
-                          int rbf, double gamma, int n_blocks, void *workspace, double *out,
-                          void *stream)
+                          const double *coef, const double *ds, const double *dx,
+                          const double *dmut, int dmut_stride, int normalize, int rbf,
+                          double gamma, int n_blocks, void *workspace, double *out, void *stream)
 {
-    if (xwin == nullptr || codes == nullptr || tab == nullptr || svp == nullptr ||
-        workspace == nullptr || out == nullptr)
-        return fail(-1, "gkm_index_ism: xwin, codes, tab, svp, workspace and out are required");
-    if (normalize && dmut == nullptr)
-        return fail(-1, "gkm_index_ism: normalize needs dmut");
+    if (xwin == nullptr || codes == nullptr || tab == nullptr || coef == nullptr ||
+        ds == nullptr || workspace == nullptr || out == nullptr)
+        return fail(-1, "gkm_index_ism: xwin, codes, tab, coef, ds, workspace and out are "
+                        "required");
+    if (normalize && (dmut == nullptr || dx == nullptr))
+        return fail(-1, "gkm_index_ism: normalize needs dx and dmut");
     if ((uintptr_t)workspace % 256)
         return fail(-1, "gkm_index_ism: workspace must be 256-byte aligned");
     if (n_blocks < 1 || n_blocks > 65536)
@@ -2633,66 +2738,79 @@ GKM_API int gkm_index_ism(const uint32_t *xwin, const uint8_t *codes, int lx, in
         return fail(-1, "gkm_index_ism: length %d exceeds lmax %d", lx, lmax);
     if (lx < l || S <= 0 || n_masks <= 0)
         return 0;  // no windows (or no SVs): every delta is 0
-    if ((int64_t)lx * 3 > (1 << 28))
+    if ((int64_t)lx * 3 * l > (1 << 30) || (lx + ISM_SEG - 1) / ISM_SEG > 65535)
         return fail(-1, "gkm_index_ism: a query of %d bp is too long", lx);
     uint8_t *w = (uint8_t *)workspace;
-    const int rows_old = lx, rows_new = 3 * lx;
+    const int wx = lx - l + 1, rows_new = 3 * lx;
+    double *C = (double *)w;
+    w += align256((int64_t)wx * S * 8);
     double *R = (double *)w;
-    w += align256((int64_t)rows_old * S * 8);
-    uint32_t *owin = (uint32_t *)w;
-    w += align256((int64_t)rows_old * l * 4);
-    int32_t *opos = (int32_t *)w;
-    w += align256((int64_t)rows_old * l * 4);
-    int32_t *olen = (int32_t *)w;
-    w += align256((int64_t)rows_old * 4);
+    w += align256((int64_t)lx * S * 8);
+    double *svp = (double *)w;
+    w += align256((int64_t)S * 32);
+    int32_t *wpos = (int32_t *)w;
+    w += align256((int64_t)wx * 4);
+    int32_t *wlen = (int32_t *)w;
+    w += align256((int64_t)wx * 4);
     uint32_t *nwin = (uint32_t *)w;
     w += align256((int64_t)rows_new * l * 4);
     int32_t *npos = (int32_t *)w;
     w += align256((int64_t)rows_new * l * 4);
     int32_t *nlen = (int32_t *)w;
     w += align256((int64_t)rows_new * 4);
-    uint8_t *plan_old = w;
-    w += queue_plan_bytes(rows_old);
+    uint8_t *plan_w = w;
+    w += queue_plan_bytes(wx);
     uint8_t *plan_new = w;
     w += queue_plan_bytes(rows_new);
     double *part = (double *)w;
-    const int64_t nth = 4 * (int64_t)lx * l;
+    const int64_t nth = 3 * (int64_t)lx * l;
     hipLaunchKernelGGL(k_ism_rows, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0,
-                       S_(stream), xwin, codes, lx, l, owin, opos, olen, nwin, npos, nlen);
+                       S_(stream), xwin, codes, lx, l, wpos, wlen, nwin, npos, nlen);
     GKM_CHECK_LAUNCH("gkm_index_ism");
     IsmArgs m{};
     m.tab = tab;
     m.svp = svp;
-    m.R = R;
     m.dmut = dmut;
     m.codes = codes;
     m.part = part;
     m.lmax = lmax;
     m.dmut_stride = dmut_stride;
-    m.wx = lx - l + 1;
+    m.wx = wx;
     m.normalize = normalize;
     m.rbf = rbf;
     m.gamma = gamma;
-    // every row holds 1..l windows: one work item (pair) per row and tile
+    // pass 0: the window rows (one window each: xwin itself, one window per
+    // row) into C; pass 1: the new rows (1..l windows each) against R and svp.
+    // One work item (pair) per row and tile.
     for (int pass = 0; pass < 2; ++pass) {
-        const bool old = pass == 0;
-        const int B = old ? rows_old : rows_new;
-        uint8_t *plan = old ? plan_old : plan_new;
+        const bool win = pass == 0;
+        const int B = win ? wx : rows_new;
+        const int W = win ? 1 : l;
+        uint8_t *plan = win ? plan_w : plan_new;
         int *queue = (int *)plan;
         int32_t *first = (int32_t *)(plan + 256);
-        const int32_t *len = old ? olen : nlen;
-        hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), len, B, l, l,
+        const int32_t *len = win ? wlen : nlen;
+        hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), len, B, l, W,
                            1, queue, first);
         GKM_CHECK_LAUNCH("gkm_index_ism");
-        m.old_rows = old ? 1 : 0;
+        m.old_rows = win ? 1 : 0;
+        m.R = win ? C : R;
         int count_bytes = 4;
-        int rc = launch_index_tile("gkm_index_ism", old ? owin : nwin, old ? opos : npos, len, B,
-                                   l, offsets, ids, vals, records, record_tables, masks, levels,
-                                   n_masks, l, nm, wtab, S, tile, n_tiles, l, pack_counts, -1, 0,
+        int rc = launch_index_tile("gkm_index_ism", win ? xwin : nwin, win ? wpos : npos, len, B,
+                                   W, offsets, ids, vals, records, record_tables, masks, levels,
+                                   n_masks, l, nm, wtab, S, tile, n_tiles, W, pack_counts, -1, 0,
                                    nullptr, nullptr, &count_bytes, stream, first, B, queue,
                                    n_blocks, &m);
         if (rc)
             return rc;
+        if (win) {
+            dim3 grid((unsigned)((S + 255) / 256), (unsigned)((lx + ISM_SEG - 1) / ISM_SEG));
+            hipLaunchKernelGGL(k_ism_cover, grid, dim3(256), 0, S_(stream), C, wx, lx, l, S, R);
+            GKM_CHECK_LAUNCH("gkm_index_ism");
+            hipLaunchKernelGGL(k_ism_ref, dim3((unsigned)((S + 255) / 256)), dim3(256), 0,
+                               S_(stream), R, wx, l, S, coef, ds, dx, normalize, rbf, gamma, svp);
+            GKM_CHECK_LAUNCH("gkm_index_ism");
+        }
     }
     hipLaunchKernelGGL(k_ism_reduce, dim3((unsigned)((rows_new + 255) / 256)), dim3(256), 0,
                        S_(stream), part, (const int32_t *)(plan_new + 256), n_tiles, codes, lx,

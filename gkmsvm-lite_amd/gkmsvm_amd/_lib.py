@@ -47,8 +47,8 @@ SIGNATURES = {
                         P, I, I, D, P, P],
     "gkm_index_queue_bytes": [I, I, I, I],
     "gkm_index_ism_bytes": [I, I, I, I],
-    "gkm_index_ism": [P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, P, P, P, I, I, I,
-                      D, I, P, P, P],
+    "gkm_index_ism": [P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, P, P, P, P, P, I,
+                      I, I, D, I, P, P, P],
     "gkm_index_score_queue": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P,
                               I, I, P, P, P, I, I, D, P, P],
     "gkm_scratch_bytes": [],

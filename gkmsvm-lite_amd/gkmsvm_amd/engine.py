@@ -841,6 +841,7 @@ def index_score_queue(sv: SVModel, xs: DeviceSeqs, dx, bias: float, out: torch.T
 # ISM sweep (gkm_ism): window pairs of (query, SV) per second, measured on the
 # -t 5 configuration (1 kb query against 50,000 SVs of 500 bp: 47.6 ms)
 SWEEP_PAIRS_PER_S = 1.0e12
+ISM_MAX_BYTES = 16 << 30  # workspace of one query's ISM through the index
 
 
 def index_ism_ok(kp: KernelParams, idx) -> bool:
@@ -854,31 +855,33 @@ def index_ism_ok(kp: KernelParams, idx) -> bool:
 
 
 def ism_use_index(sv: "SVModel", x_lengths, mode="auto") -> bool:
-    """Whether ISM of rows of these lengths goes through the l-mer index (4 l
-    probed windows per query window: the windows covering each position, as
-    they are and with each of 3 substitutions) rather than the sweep: forced
+    """Whether ISM of rows of these lengths goes through the l-mer index (3 l + 1
+    probed windows per query window: the window itself, and the windows
+    covering each position with each of 3 substitutions) rather than the sweep: forced
     by mode True/False where the index applies, else the cheaper by the cost
     model."""
     if mode is False or sv.kp.l > INDEX_MAX_L or sv.kp.dmax < 0:
         return False
     if not index_ism_ok(sv.kp, sv.index()):
         return False
+    lmax = int(np.max(x_lengths)) if len(x_lengths) else 0
+    if _lib.load().gkm_index_ism_bytes(lmax, sv.kp.l, sv.S, sv.index().n_tiles) > ISM_MAX_BYTES:
+        return False  # the per-window and per-position SV parts of one query
     if mode is True:
         return True
     wx = float(np.clip(np.asarray(x_lengths) - sv.kp.l + 1, 0, None).sum())
-    index_s, _ = sv._path_seconds(4.0 * sv.kp.l * wx)
+    index_s, _ = sv._path_seconds((3.0 * sv.kp.l + 1.0) * wx)
     norient = 2 if sv.kp.rc else 1
     ws = float(np.clip(sv.seqs.lengths - sv.kp.l + 1, 0, None).sum())
     return index_s < wx * norient * ws / SWEEP_PAIRS_PER_S
 
 
-def index_ism(sv: "SVModel", xs: DeviceSeqs, b: int, svp: torch.Tensor, dmut, out: torch.Tensor,
+def index_ism(sv: "SVModel", xs: DeviceSeqs, b: int, dx, dmut, out: torch.Tensor,
               stream=None) -> torch.Tensor:
     """out [4, lmax] (contiguous) = the ISM deltas of row b through the l-mer
-    index (gkm_index_ism): svp [S, 4] per SV (K_raw(x_b, s), its finished
-    value, coefficient, K_raw(s, s)); dmut [4, lmax, B] the mutants'
-    self-kernels (None without normalisation).  Entries of the reference base
-    and past the row's length are left as they are."""
+    index (gkm_index_ism); dx [B] the rows' and dmut [4, lmax, B] the
+    mutants' self-kernels (None without normalisation).  Entries of the
+    reference base and past the row's length are left as they are."""
     kp, idx = sv.kp, sv.index()
     lx = int(xs.lengths[b])
     fwd, _ = xs.windows(kp.l, stream)
@@ -886,15 +889,16 @@ def index_ism(sv: "SVModel", xs: DeviceSeqs, b: int, svp: torch.Tensor, dmut, ou
     ws = torch.empty(max(1, _lib.load().gkm_index_ism_bytes(lx, kp.l, sv.S, idx.n_tiles)),
                      dtype=torch.uint8, device=kp.device)
     wx = max(0, lx - kp.l + 1)
-    _timed("k_index_ism", lambda: index_bytes(kp, idx, [4 * kp.l * wx + kp.l - 1]), stream,
-           lambda: _lib.call(
+    norm = kp.normalize
+    _timed("k_index_ism", lambda: index_bytes(kp, idx, [(3 * kp.l + 1) * wx + kp.l - 1]),
+           stream, lambda: _lib.call(
                "gkm_index_ism", _p(fwd[b]), _p(xs.codes[b]), lx, xs.lmax, _p(idx.offsets),
                _p(idx.ids), _p(idx.vals), _p(idx.records), idx.record_tables, _p(idx.masks),
                _p(idx.levels), idx.n_masks, kp.l, kp.nm, _p(wt), sv.S, idx.tile, idx.n_tiles,
-               {1: 0, 2: 1, 4: 2}[idx.pack], _p(kp.tab), _p(svp),
-               None if dmut is None else dmut.data_ptr() + 8 * b, xs.n, int(kp.normalize),
-               int(kp.rbf), float(kp.gamma), n_cus(kp.device), _p(ws), _p(out),
-               stream_handle(stream)))
+               {1: 0, 2: 1, 4: 2}[idx.pack], _p(kp.tab), _p(sv.coef), _p(sv.ds),
+               dx.data_ptr() + 8 * b if norm else None,
+               dmut.data_ptr() + 8 * b if norm else None, xs.n, int(norm), int(kp.rbf),
+               float(kp.gamma), n_cus(kp.device), _p(ws), _p(out), stream_handle(stream)))
     return out
 
 

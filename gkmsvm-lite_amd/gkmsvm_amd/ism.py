@@ -94,11 +94,11 @@ def _ism_rows_table(model, codes, lens, table, out):
               int(kp.normalize), E._p(out), E.stream_handle())
 
 
-def _ism_rows_index(model, codes, lens, out, group: int = 64):
-    """ISM through the l-mer index (kernels/lmer_index.py:326-376): each
-    query's K_raw(x, s) from its index histogram, then gkm_index_ism per query
-    (the windows covering each position, as they are and substituted, probed
-    in one persistent launch each)."""
+def _ism_rows_index(model, codes, lens, out):
+    """ISM through the l-mer index (kernels/lmer_index.py:326-376):
+    gkm_index_ism per query (its windows probed one by one, then the windows
+    covering each position with each substitution, in two persistent
+    launches)."""
     kp = model._params()
     sv = model._svs()
     xs = E.DeviceSeqs(codes, lens, kp.device, rc=kp.rc)
@@ -106,24 +106,9 @@ def _ism_rows_index(model, codes, lens, out, group: int = 64):
     if kp.normalize:
         dx = E.self_kraw(kp, xs)
         dmut = _mutant_diag(kp, xs, dx)
-    tab = kp.tab[: kp.nm]
-    for g0 in range(0, xs.n, group):
-        g1 = min(xs.n, g0 + group)
-        hist = E.index_hist(kp, sv.index(), xs, g0, g1, sv.S)
-        kref = torch.zeros((g1 - g0, sv.S), dtype=torch.float64, device=kp.device)
-        for lv in range(kp.nm):  # level order, as a score's combine
-            kref += tab[lv] * hist[:, :, lv].to(torch.float64)
-        del hist
-        fref = kref
-        if kp.normalize:
-            fref = kref / torch.sqrt(dx[g0:g1, None] * sv.ds[None, :]).clamp(min=1e-10)
-        if kp.rbf:
-            fref = torch.exp(kp.gamma * (fref - 1.0))
-        for b in range(g0, g1):
-            if lens[b] < kp.l:
-                continue
-            svp = torch.stack([kref[b - g0], fref[b - g0], sv.coef, sv.ds], dim=1).contiguous()
-            E.index_ism(sv, xs, b, svp, dmut, out[b])
+    for b in range(xs.n):
+        if lens[b] >= kp.l:
+            E.index_ism(sv, xs, b, dx, dmut, out[b])
 
 
 def _ism_rows(model, codes, lens, out):

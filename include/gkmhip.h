@@ -149,28 +149,30 @@ int gkm_ism(const void *xplanes, const int32_t *xlen, const uint8_t *xcodes, int
 /* ISM of one query through the l-mer index: out [4][lmax] (the query's
  * slice of [B][4][lmax]) SET to sum_s coef_s (F(k_mut) - F(k_ref)) at every
  * substitution of positions 0..lx-1 (the reference base's entries and those
- * past lx are not written).  K_raw(mutant, s) = K_raw(x, s) - K_raw(old, s) +
- * K_raw(new, s), where old holds the query's windows covering the position
- * and new the same windows with the base substituted: lx old and 3 lx new
- * rows of at most l windows go through the persistent index query (the
- * format of gkm_index_score_queue; indexes with bucket records and 8- or
- * 16-bit counts, else -2), whose blocks finish their counts per SV -- old
- * rows into R [lx][S], new rows into per-tile sums of coef_s (F(k_mut) -
- * F(k_ref)) -- summed over tiles in order (deterministic).  xwin: the
- * query's packed windows; svp [S][4]: K_raw(x, s), F(k_ref(s)) (normalised,
- * RBF), coef_s, K_raw(s, s) for SVs in index order; dmut: K_raw(mutant,
- * mutant) of this query, entry (v lmax + p) dmut_stride (gkm_mutant_diag's
- * [4][lmax][B] from column b: stride B); workspace gkm_index_ism_bytes
- * bytes, 256-byte aligned.  kernels/lmer_index_cpu.py:203-285
- * (_query_mutants_cpu) and kernels/lmer_index.py:326-376 (LmerIndex.mutants). */
+ * past lx are not written).  K_raw(mutant, s) = K_raw(x, s) - R[p][s] +
+ * K_raw(new, s): R[p][s] sums the parts C[i][s] of the windows covering p,
+ * new holds those windows with the base substituted.  Two persistent index
+ * queries (the format of gkm_index_score_queue; indexes with bucket records
+ * and 8- or 16-bit counts, else -2) whose blocks finish their counts per SV:
+ * the lx - l + 1 windows one by one into C [windows][S], then the 3 lx new
+ * rows of at most l windows into per-tile sums of coef_s (F(k_mut) -
+ * F(k_ref)), summed over tiles in order; every sum has a fixed order
+ * (deterministic).  xwin: the query's packed windows; coef, ds: the SVs'
+ * coefficients and K_raw(s, s) in index order; dx: the query's K_raw(x, x);
+ * dmut: K_raw(mutant, mutant) of this query, entry (v lmax + p) dmut_stride
+ * (gkm_mutant_diag's [4][lmax][B] from column b: stride B); dx and dmut may
+ * be NULL without normalize; workspace gkm_index_ism_bytes bytes, 256-byte
+ * aligned.  kernels/lmer_index_cpu.py:203-285 (_query_mutants_cpu) and
+ * kernels/lmer_index.py:326-376 (LmerIndex.mutants). */
 int64_t gkm_index_ism_bytes(int lx, int l, int S, int n_tiles);
 int gkm_index_ism(const uint32_t *xwin, const uint8_t *codes, int lx, int lmax,
                   const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
                   const uint32_t *records, int record_tables, const uint32_t *masks,
                   const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab, int S,
-                  int tile, int n_tiles, int pack_counts, const double *tab, const double *svp,
-                  const double *dmut, int dmut_stride, int normalize, int rbf, double gamma,
-                  int n_blocks, void *workspace, double *out, void *stream);
+                  int tile, int n_tiles, int pack_counts, const double *tab, const double *coef,
+                  const double *ds, const double *dx, const double *dmut, int dmut_stride,
+                  int normalize, int rbf, double gamma, int n_blocks, void *workspace,
+                  double *out, void *stream);
 
 /* GkmExplain through the l-mer index, ADDED into out [B][4][lmax]: every
  * query window u (xwin [B][Wx], PAD past each row's windows) visits the l-mers
