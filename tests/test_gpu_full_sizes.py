@@ -162,9 +162,11 @@ def test_batched_uneven_workload_20k_svs(gpu):
 def test_explain_ism_workload_50k_svs(gpu):
     """-t 5 (weighted RBF, gamma 2), 50,000 SVs x 500 bp with Laplace(0, 0.5)
     coefficients, one 1 kb query: GkmExplain through the index equals the
-    per-pair sweep (1e-10), ISM is zero at the reference base and equals 12
-    rescored mutants within 1e-12 of the coefficients' absolute sum (the
-    rounding of two 50,000-term score sums)."""
+    per-pair sweep (1e-10); ISM (through the l-mer index, as the cost model
+    picks at this size) equals the sweep's ISM (1e-10), is zero at the
+    reference base and equals 12 rescored mutants within 1e-12 of the
+    coefficients' absolute sum (the rounding of two 50,000-term score sums)."""
+    from gkmsvm_amd import engine as E
     from gkmsvm_amd import explain as X
     from gkmsvm_amd import gkmexplain, ism
 
@@ -183,7 +185,10 @@ def test_explain_ism_workload_50k_svs(gpu):
             X.EXPLAIN_KERNEL = "auto"
     H.assert_close(res["index"], res["sweep"], 1e-10, "index vs sweep explain")
     assert np.abs(res["sweep"]).max() > 0
+    assert E.ism_use_index(m._svs(), [len(q)], "auto")
     d = ism(m, [q]).cpu().numpy()[0]
+    H.assert_close(d, ism(m, [q], use_index=False).cpu().numpy()[0], 1e-10,
+                   "index vs sweep ISM")
     codes = O.encode(q)
     assert (d[codes, np.arange(len(q))] == 0.0).all()
     pos = rng.choice(len(q), 12, replace=False)
