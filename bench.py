@@ -357,6 +357,32 @@ def kernel_only(args):
         torch.cuda.synchronize()
         print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
         return
+    if args.mode == "score":
+        # the serving step's persistent launch: the scoring and variant rows of
+        # one 96-request burst of the stream (seed 5) scored through the index
+        # in one gkm_index_score_queue launch (k_queue_plan, k_index_tile_queue,
+        # k_index_score_reduce)
+        rows = []
+        for kind, payload in make_requests(96, 5):
+            if kind == "score":
+                rows += payload
+            elif kind == "variant":
+                rows += [payload[0][0], payload[0][1]]
+        xs, _ = E.DeviceSeqs.from_strings(rows, kp.device)
+        dx = E.self_kraw(kp, xs)
+        out = torch.empty(xs.n, dtype=torch.float64, device=kp.device)
+        sv.index()
+        E.index_score_queue(sv, xs, dx, 0.0, out)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.launches):
+            E.index_score_queue(sv, xs, dx, 0.0, out)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": args.mode, "launches": args.launches, "rows": xs.n,
+                          "windows": int(np.clip(xs.lengths - kp.l + 1, 0, None).sum()),
+                          "ms_per_launch": (time.perf_counter() - t) / args.launches * 1e3,
+                          "checksum": float(out.double().sum().item())}), flush=True)
+        return
     q = rng.integers(0, 4, (16, 1000)).astype(np.uint8)
     xs = E.DeviceSeqs(q, np.full(16, 1000), kp.device)
     out = torch.empty((16, sv.S, kp.nm), dtype=torch.int64, device=kp.device)
@@ -557,11 +583,13 @@ def main():
     ap.add_argument("--no-tables", action="store_true",
                     help="skip the l-mer table runs (profiles of the per-SV kernels)")
     ap.add_argument("--mode", default="serve",
-                    choices=["serve", "cross", "index", "table", "copy", "peaks", "ism",
-                             "configs"],
-                    help="serve: the benchmark; cross/index/table/copy: one kernel family on "
-                         "the serving shape, for profiling; ism: the -t 5 ISM sample, for "
-                         "profiling; configs: bounded timings of the other configurations")
+                    choices=["serve", "cross", "index", "score", "table", "copy", "peaks",
+                             "ism", "configs"],
+                    help="serve: the benchmark; cross/index/score/table/copy: one kernel "
+                         "family on the serving shape, for profiling (score: one step's "
+                         "scoring rows in the persistent index launch); ism: the -t 5 ISM "
+                         "sample, for profiling; configs: bounded timings of the other "
+                         "configurations")
     ap.add_argument("--c3-queries", type=int, default=100_000)
     ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)

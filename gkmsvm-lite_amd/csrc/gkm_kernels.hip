@@ -1014,13 +1014,19 @@ __device__ __noinline__ void fold_level_sums_call(const uint32_t *cnt, int n,
 constexpr int INDEX_THREADS = 1024;  // 16 waves share one LDS tile (one block per CU)
 constexpr int INDEX_PROBES = 1;      // probes per lane per group (64 INDEX_PROBES masks)
 constexpr int INDEX_WALK = 12;       // entries per probe per round of the lane walk
-constexpr int INDEX_WALK_MAX_C = 48; // buckets up to this size are walked by their lane
+static_assert(INDEX_WALK <= GKM_INDEX_PAD, "the first round reads INDEX_WALK entries ahead");
 constexpr int INDEX_PASSES = 4;      // 64-entry passes of a balanced walk issued together
-constexpr int RECORD_IDS = 12;       // ids held in a 32-byte bucket record (unweighted)
+constexpr int RECORD_IDS = 15;       // ids a 32-byte bucket record holds (unweighted, short)
+constexpr int RECORD_IDS_BIG = 11;   // ids it holds beside the start and count (long)
 constexpr int RECORD_IDS_W = 8;      // (id, weight) pairs held in one (weighted index)
 
-// Bucket records, 8 words: start, count, then (RK 1) RECORD_IDS uint16 ids or
-// (RK 2) RECORD_IDS_W uint16 ids and their uint8 weights; wd = words 2..7.
+// Bucket records, 8 words (16 halfwords), one per key.
+// RK 1 (unweighted): halfword 0 = min(count, 0xFFFF); a bucket of at most
+// RECORD_IDS entries has its ids in halfwords 1..count (no start: nothing past
+// them); a longer one its first RECORD_IDS_BIG ids in halfwords 1..11, its
+// start in word 6 and its count in word 7 (the rest at ids[start + 11 ..]).
+// RK 2 (weighted): word 0 start, word 1 count, RECORD_IDS_W uint16 ids in
+// halfwords 4..11 and their uint8 weights in bytes 24..31.
 // RK 3: one 64-byte record per l-mer for unweighted indexes of at most 4 SV
 // tiles, whose buckets are consecutive in ids: word 0 the start of the l-mer's
 // run, word 1 the tiles' counts as bytes (255: saturated, read the offsets),
@@ -1028,19 +1034,40 @@ constexpr int RECORD_IDS_W = 8;      // (id, weight) pairs held in one (weighted
 constexpr int LREC_IDS = 28;
 constexpr int LREC_MAX_TILES = 4;
 
+// ids a record can hold (the lane walk's unrolled reads)
 template <int RK>
 constexpr int record_ids() { return RK == 2 ? RECORD_IDS_W : RK == 1 ? RECORD_IDS : 0; }
 __device__ __forceinline__ uint32_t lrec_id(const uint32_t *wd, int k)
 {
     return (wd[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
 }
-__device__ __forceinline__ uint32_t rec_id(const uint32_t *wd, int k)
+// id k of a record, w = its 8 words
+template <int RK>
+__device__ __forceinline__ uint32_t rec_id(const uint32_t *w, int k)
 {
-    return (wd[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    constexpr int H = RK == 2 ? 4 : 1;  // halfword of id 0
+    return (w[(k + H) >> 1] >> (16 * ((k + H) & 1))) & 0xFFFFu;
 }
-__device__ __forceinline__ uint32_t rec_val(const uint32_t *wd, int k)
+__device__ __forceinline__ uint32_t rec_val(const uint32_t *w, int k)
 {
-    return (wd[4 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
+    return (w[6 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
+}
+// A record's bucket: its entry count, the ids it holds, and where the rest
+// begin in the ids array (e0 + n_in; e0 = the bucket's start).
+struct RecBucket {
+    int c, n_in;
+    uint32_t e0;
+};
+template <int RK>
+__device__ __forceinline__ RecBucket rec_bucket(uint4 ra, uint4 rb)
+{
+    if constexpr (RK == 1) {
+        const uint32_t h0 = ra.x & 0xFFFFu;
+        const bool big = h0 > (uint32_t)RECORD_IDS;
+        return {big ? (int)rb.w : (int)h0, big ? RECORD_IDS_BIG : (int)h0, big ? rb.z : 0u};
+    } else {
+        return {(int)ra.y, min((int)ra.y, RECORD_IDS_W), ra.x};
+    }
 }
 
 // The layout of the bucket records gkm_index_explain is passed: none (0), 32
@@ -1161,11 +1188,12 @@ __device__ __noinline__ double ism_fold(const double *tab, const double *svp, do
 // together (same XCD, see below).  One block fills a CU's LDS, so a CU runs 16
 // waves; each wave keeps the records (or bounds) of its next two groups in
 // flight while it walks the current group's buckets (two register buffers,
-// the loop unrolled by two, so a buffer is read two steps after its load).  When no bucket of the group holds more than
-// INDEX_WALK_MAX_C entries each lane walks its own bucket, the ids past the
-// record INDEX_WALK per round with the round's loads issued together.  Larger
-// buckets (repeats) are walked wave-balanced: the wave's buckets laid end to
-// end, each pass's entries located by a binary search over the prefix sums.
+// the loop unrolled by two, so a buffer is read two steps after its load).
+// When every bucket of the group ends within INDEX_WALK entries past its
+// record, each lane walks its own bucket: the record's ids, then the rest with
+// loads issued before the next fetch.  Larger buckets (repeats) are walked
+// wave-balanced: the wave's buckets laid end to end, each pass's entries
+// located by a binary search over the prefix sums.
 // Block order: the XCDs take blocks round-robin by linear id (gridDim.x is a
 // multiple of 8), so block p runs on XCD p % 8.  The blocks of one window
 // chunk -- one per SV tile, probing the same offsets lines -- are given ids
@@ -1174,7 +1202,15 @@ __device__ __noinline__ double ism_fold(const double *tab, const double *svp, do
 // the row's last window, exit at once.
 // One work item of the index query: window chunk `chunk` of row b against SV
 // tile tile_id; with wfix its level sums go to slot `slot` of the psum output.
-template <int PACK, int RK, bool W16, bool ISM = false>
+// The LDS count of one level in the index query: word offset of the level's
+// counts (level / counts per word times the SVs counted) and the increment
+// (or shift) that adds to its field.
+struct CntKey {
+    int off;
+    uint32_t inc;
+};
+
+template <int PACK, int RK, bool W16, bool ISM = false, bool DIAG = false>
 __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk, int tile_id,
                                             int64_t slot)
 {
@@ -1251,14 +1287,22 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     };
     const uint32_t nl = 1u << (2 * a.l);
     const int i1 = min(wx, i0 + a.chunk_windows);
-    const int64_t lo_col = a.col_min >= 0 ? a.row_base + b + a.col_min - s_base : -1;
+    // DIAG (col_min >= 0: the Gram's upper triangle): entries of SVs below
+    // lo_col (relative to the tile, clamped to -1 .. 2^16) are skipped
+    const int lo_col = DIAG ? (int)min(max(a.row_base + b + a.col_min - s_base, (int64_t)-1),
+                                       (int64_t)0x10000)
+                            : -1;
     const int64_t off = (((int64_t)chunk * a.B + b) * a.S + s_base) * a.nm;
     // One walk of the block's (window, mask group) pairs, every bucket entry
-    // handed to count(sid, level, weight).  Probe p of lane t in group g takes
-    // mask m = (INDEX_PROBES g + p) 64 + t, neighbour v = w ^ mask[m], whose
-    // bucket is [e0, e1).
-    auto walk = [&](auto count) {
+    // handed to count(sid, key, weight), key = keyf(level) computed once per
+    // probe (the LDS word offset and increment of that level).  Probe p of
+    // lane t in group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
+    // v = w ^ mask[m], whose bucket is [e0, e0 + c).  Returns this lane's
+    // number of entries walked (every one of them was handed to count once).
+    auto walk = [&](auto keyf, auto count) -> uint32_t {
+        using Key = decltype(keyf(0));
         const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
+        uint32_t walked = 0;
         // two (window, mask group) pairs' records in flight: the pair walked
         // now was fetched two steps ago, into the buffer (A or B) it is read from.
         // ISM rows hold 1..l windows: their pairs are dealt to the waves in turn
@@ -1321,6 +1365,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             cpos_next = load_pos(ci + WSTEP);
         }
         load_masks(cg);
+        // record byte offsets in 32 bits (the host keeps the record tables
+        // below 4 GiB): table q at q * tbytes, key (l-mer, tile) at
+        // l-mer * n_tiles * 32 + tile * 32
+        const uint32_t tbytes = nl * (uint32_t)a.n_tiles * 32u;
+        const uint32_t vbytes = (uint32_t)a.n_tiles * 32u;
+        const uint32_t tile_bytes = (uint32_t)tile_id * 32u;
+        const uint32_t last_table = (uint32_t)(a.record_tables - 1);
         auto fetch = [&](uint4 (&nr)[INDEX_PROBES][2], int (&nlv)[INDEX_PROBES]) {
             const uint32_t w = ni < i1 ? fw : nl;  // PAD past the chunk
             uint32_t mk[INDEX_PROBES], lb[INDEX_PROBES];
@@ -1353,12 +1404,11 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                         // the l-mer swapped, so the two masks of a pair (differing
                         // in bit 0 of digit q) read one 64-byte-aligned pair of
                         // records, a 128-byte line, in one instruction
-                        const uint32_t q = min(lvb >> 4, (uint32_t)(a.record_tables - 1));
+                        const uint32_t q = min(lvb >> 4, last_table);
                         const uint32_t x = (v ^ (v >> (2u * q))) & 3u;
                         const uint32_t vq = v ^ x ^ (x << (2u * q));
-                        const size_t rk = (size_t)q * ((size_t)nl * (uint32_t)a.n_tiles) +
-                                          (size_t)vq * (uint32_t)a.n_tiles + (uint32_t)tile_id;
-                        const uint4 *r = (const uint4 *)(a.records + rk * 8);
+                        const uint32_t boff = q * tbytes + vq * vbytes + tile_bytes;
+                        const uint4 *r = (const uint4 *)((const char *)a.records + boff);
                         nr[p][0] = r[0];
                         nr[p][1] = r[1];
                     } else {  // the bucket's bounds, its count taken when it is walked
@@ -1386,43 +1436,57 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         fetch(nrc, nlvc);
         auto step = [&](uint4 (&nrb)[INDEX_PROBES][2], int (&nlvb)[INDEX_PROBES]) {
             const int i = ci;
-            uint32_t e0[INDEX_PROBES], rid[INDEX_PROBES][6];
+            uint32_t rw[INDEX_PROBES][8], rb0[INDEX_PROBES];
             int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
+            Key key[INDEX_PROBES];
             bool small = true;
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
-                e0[p] = nrb[p][0].x;
-                c[p] = nlvb[p] < 0 ? 0 : RK ? (int)nrb[p][0].y : (int)(nrb[p][0].y - nrb[p][0].x);
-                lv[p] = max(nlvb[p], 0);
-                n_in[p] = min(c[p], record_ids<RK>());  // ids in the record
-                small = small && c[p] <= INDEX_WALK_MAX_C;
-                rid[p][0] = nrb[p][0].z;
-                rid[p][1] = nrb[p][0].w;
-                rid[p][2] = nrb[p][1].x;
-                rid[p][3] = nrb[p][1].y;
-                rid[p][4] = nrb[p][1].z;
-                rid[p][5] = nrb[p][1].w;
+                rw[p][0] = nrb[p][0].x;
+                rw[p][1] = nrb[p][0].y;
+                rw[p][2] = nrb[p][0].z;
+                rw[p][3] = nrb[p][0].w;
+                rw[p][4] = nrb[p][1].x;
+                rw[p][5] = nrb[p][1].y;
+                rw[p][6] = nrb[p][1].z;
+                rw[p][7] = nrb[p][1].w;
+                RecBucket bk;
+                if constexpr (RK != 0)
+                    bk = rec_bucket<RK>(nrb[p][0], nrb[p][1]);
+                else
+                    bk = {(int)(nrb[p][0].y - nrb[p][0].x), 0, nrb[p][0].x};
+                c[p] = nlvb[p] < 0 ? 0 : bk.c;
+                n_in[p] = nlvb[p] < 0 ? 0 : bk.n_in;  // ids in the record
+                rb0[p] = bk.e0 + (uint32_t)n_in[p];    // the rest of the bucket from here
+                lv[p] = max(nlvb[p], 0);  // (nlvb is the next fetch's once fetched)
+                key[p] = keyf(lv[p]);
+                walked += (uint32_t)c[p];
+                // a group whose buckets' rests all fit INDEX_WALK entries takes
+                // them from the loads below; a longer rest (repeats) is walked
+                // balanced
+                small = small && c[p] - n_in[p] <= INDEX_WALK;
             }
-            // the lane walk's first round of ids past the records, issued before
-            // the fetch below, so waiting for it leaves that fetch in flight
-            int kmin = n_in[0];
-#pragma unroll
-            for (int p = 1; p < INDEX_PROBES; ++p)
-                kmin = min(kmin, n_in[p]);
+            // the rests, issued before the fetch below, so waiting for them
+            // leaves that fetch in flight.  A lane with a rest reads INDEX_WALK
+            // entries from its first (up to INDEX_WALK - 1 past the bucket's end:
+            // the ids array holds INDEX_WALK padding entries past its last
+            // bucket), the others the array's first entries -- one address per
+            // probe, the entries at immediate offsets from it, no branch around
+            // a load
             int sid0[INDEX_PROBES][INDEX_WALK];
             uint32_t wv0[INDEX_PROBES][INDEX_WALK];
 #pragma unroll
-            for (int p = 0; p < INDEX_PROBES; ++p)
+            for (int p = 0; p < INDEX_PROBES; ++p) {
+                const uint32_t ob = small && n_in[p] < c[p] ? rb0[p] : 0u;
+                const uint16_t *op = a.ids + ob;
+                const uint8_t *ov = a.vals ? a.vals + ob : nullptr;
 #pragma unroll
                 for (int k = 0; k < INDEX_WALK; ++k) {
-                    // unconditional too (entry 0 where there is none; the index
-                    // holds at least one)
-                    const bool in = small && kmin + k >= n_in[p] && kmin + k < c[p];
-                    const uint32_t e = in ? e0[p] + (uint32_t)(kmin + k) : 0u;
-                    sid0[p][k] = (int)a.ids[e];
+                    sid0[p][k] = (int)op[k];
                     if (PACK == 1 || W16)
-                        wv0[p][k] = a.vals ? a.vals[e] : 1u;
+                        wv0[p][k] = ov ? ov[k] : 1u;
                 }
+            }
             fetch(nrb, nlvb);  // the pair two ahead, into the registers just read
             const int pos = a.xpos ? cpos : i;  // loaded a window ahead
             if constexpr (FLAT) {
@@ -1440,58 +1504,49 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 }
             }
             const uint32_t aw = a.wtab ? window_weight(a.wtab, wxw, pos) : 1u;
+            // the ids in the records
+#pragma unroll
+            for (int p = 0; p < INDEX_PROBES; ++p)
+#pragma unroll
+                for (int k = 0; k < record_ids<RK>(); ++k)
+                    if (k < n_in[p])
+                        count((int)rec_id<RK>(rw[p], k), key[p],
+                              PACK > 1 && !W16 ? 1u : aw * (RK == 2 ? rec_val(rw[p], k) : 1u));
             if (__ballot(!small) == 0) {
-                // lane walk: every probe's next INDEX_WALK entries per round
-                int cmax = c[0];
+                // the rests, from the loads above: entry k of probe p's rest for
+                // k < c - n_in; the round stops at the first k past every
+                // lane's rest (few buckets reach past their records).  No loads
+                // here beyond those: a loop of loads in this path would make
+                // the compiler drain the fetches in flight at every step.
+                int d[INDEX_PROBES], dtop = 0;
 #pragma unroll
-                for (int p = 1; p < INDEX_PROBES; ++p)
-                    cmax = max(cmax, c[p]);
+                for (int p = 0; p < INDEX_PROBES; ++p) {
+                    d[p] = c[p] - n_in[p];
+                    dtop = max(dtop, d[p]);
+                }
 #pragma unroll
-                for (int p = 0; p < INDEX_PROBES; ++p)  // the entries in the records
-#pragma unroll
-                    for (int k = 0; k < record_ids<RK>(); ++k)
-                        if (k < c[p])
-                            count((int)rec_id(rid[p], k), lv[p],
-                                  PACK > 1 && !W16 ? 1u : aw * (RK == 2 ? rec_val(rid[p], k) : 1u));
-                for (int kb = kmin; __ballot(kb < cmax) != 0; kb += INDEX_WALK) {
-                    int sid[INDEX_PROBES][INDEX_WALK];
-                    uint32_t wv[INDEX_PROBES][INDEX_WALK];
-#pragma unroll
-                    for (int p = 0; p < INDEX_PROBES; ++p)
-#pragma unroll
-                        for (int k = 0; k < INDEX_WALK; ++k) {
-                            const bool in = kb + k >= n_in[p] && kb + k < c[p];
-                            const uint32_t e = e0[p] + (uint32_t)(kb + k);
-                            if (kb == kmin) {  // loaded before the fetch
-                                sid[p][k] = sid0[p][k];
-                                if (PACK == 1 || W16)
-                                    wv[p][k] = wv0[p][k];
-                            } else {
-                                sid[p][k] = in ? (int)a.ids[e] : 0;
-                                if (PACK == 1 || W16)
-                                    wv[p][k] = in && a.vals ? a.vals[e] : 1u;
-                            }
-                        }
+                for (int k = 0; k < INDEX_WALK; ++k) {
+                    if (__ballot(k < dtop) == 0)
+                        break;
 #pragma unroll
                     for (int p = 0; p < INDEX_PROBES; ++p)
-#pragma unroll
-                        for (int k = 0; k < INDEX_WALK; ++k)
-                            if (kb + k >= n_in[p] && kb + k < c[p])
-                                count(sid[p][k], lv[p], PACK > 1 && !W16 ? 1u : aw * wv[p][k]);
+                        if (k < d[p])
+                            count(sid0[p][k], key[p], PACK > 1 && !W16 ? 1u : aw * wv0[p][k]);
                 }
                 return;
             }
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
-                // balanced walk of this probe's buckets: laid end to end (exclusive
+                // balanced walk of this probe's rests: laid end to end (exclusive
                 // prefix sums) and walked 64 entries per pass: lane j of a pass takes
-                // entry base + j, in the bucket of the last lane r with excl[r] <= base + j
-                int incl = c[p];
+                // entry base + j, in the rest of the last lane r with excl[r] <= base + j
+                const int rest = c[p] - n_in[p];
+                int incl = rest;
                 for (int o = 1; o < WAVE; o <<= 1) {
                     const int u = __shfl_up(incl, o, WAVE);
                     incl += lane >= o ? u : 0;
                 }
-                const int excl = incl - c[p];
+                const int excl = incl - rest;
                 const int total = __shfl(incl, WAVE - 1, WAVE);
                 // INDEX_PASSES passes at a time: their searches, gathers and id
                 // loads are independent, so their latencies overlap
@@ -1513,7 +1568,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
 #pragma unroll
                     for (int u = 0; u < INDEX_PASSES; ++u) {
                         const int j = base + u * WAVE + lane;
-                        const uint32_t pe0 = (uint32_t)__shfl((int)e0[p], pr[u], WAVE);
+                        const uint32_t pe0 = (uint32_t)__shfl((int)rb0[p], pr[u], WAVE);
                         plv[u] = __shfl(lv[p], pr[u], WAVE);
                         const int pex = __shfl(excl, pr[u], WAVE);
                         e[u] = pe0 + (uint32_t)(j - pex);
@@ -1522,7 +1577,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
 #pragma unroll
                     for (int u = 0; u < INDEX_PASSES; ++u)
                         if (sid[u] >= 0)
-                            count(sid[u], plv[u],
+                            count(sid[u], keyf(plv[u]),
                                   PACK > 1 && !W16 ? 1u : aw * (a.vals ? a.vals[e[u]] : 1u));
                 }
             }
@@ -1533,6 +1588,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 break;
             step(nrc, nlvc);
         }
+        return walked;
     };
     // 16-bit counts [(nm + 1) / 2][n] of n SVs from SV s0 of the tile out as
     // the [n][nm] uint16 partials
@@ -1560,12 +1616,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         // block compares the two (64-bit sums) and, where they differ, walks
         // its chunk again with 32-bit counts, sub SVs per pass
         unsigned long long adds = 0;
-        walk([&](int sid, int lv, uint32_t add) {
-            if (sid < lo_col)
-                return;
-            adds += add;
-            atomicAdd(&cnt[(lv >> 1) * ns + sid], add << (16 * (lv & 1)));
-        });
+        walk([&](int lv) { return CntKey{(lv >> 1) * ns, 16u * (uint32_t)(lv & 1)}; },
+             [&](int sid, CntKey k, uint32_t add) {
+                 if (DIAG && sid < lo_col)
+                     return;
+                 adds += add;
+                 atomicAdd(&cnt[k.off + sid], add << k.inc);
+             });
         __syncthreads();
         unsigned long long d = adds;
         for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS) {
@@ -1602,12 +1659,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             for (int i = threadIdx.x; i < n * a.nm; i += INDEX_THREADS)
                 cnt[i] = 0u;
             __syncthreads();
-            walk([&](int sid, int lv, uint32_t add) {
-                const int r = sid - s0;
-                if (sid < lo_col || (unsigned)r >= (unsigned)n)
-                    return;
-                atomicAdd(&cnt[lv * n + r], add);
-            });
+            walk([&](int lv) { return CntKey{lv * n, 0u}; },
+                 [&](int sid, CntKey k, uint32_t add) {
+                     const int r = sid - s0;
+                     if ((DIAG && sid < lo_col) || (unsigned)r >= (unsigned)n)
+                         return;
+                     atomicAdd(&cnt[k.off + r], add);
+                 });
             __syncthreads();
             if (fused) {
                 fold(n, s0, per1{});
@@ -1622,13 +1680,20 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             emit();
         return;
     } else if constexpr (PACK == 4) {
+        // every entry walked is one add, except (DIAG) those skipped
         uint32_t adds = 0;
-        walk([&](int sid, int lv, uint32_t) {
-            if (sid < lo_col)
-                return;
-            ++adds;
-            atomicAdd(&cnt[(lv >> 2) * ns + sid], 1u << (8 * (lv & 3)));
-        });
+        const uint32_t walked = walk(
+            [&](int lv) { return CntKey{(lv >> 2) * ns, 1u << (8 * (lv & 3))}; },
+            [&](int sid, CntKey k, uint32_t) {
+                if constexpr (DIAG) {
+                    if (sid < lo_col)
+                        return;
+                    ++adds;
+                }
+                atomicAdd(&cnt[k.off + sid], k.inc);
+            });
+        if constexpr (!DIAG)
+            adds = walked;
         __syncthreads();
         // adds - byte sums over the block: 0 exactly when no add carried (the
         // block's adds stay below 2^32: at most 65,535 per SV, see the host cap)
@@ -1684,12 +1749,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             for (int i = threadIdx.x; i < n * ((a.nm + 1) / 2); i += INDEX_THREADS)
                 cnt[i] = 0u;
             __syncthreads();
-            walk([&](int sid, int lv, uint32_t) {
-                const int r = sid - s0;
-                if (sid < lo_col || (unsigned)r >= (unsigned)n)
-                    return;
-                atomicAdd(&cnt[(lv >> 1) * n + r], 1u << (16 * (lv & 1)));
-            });
+            walk([&](int lv) { return CntKey{(lv >> 1) * n, 1u << (16 * (lv & 1))}; },
+                 [&](int sid, CntKey k, uint32_t) {
+                     const int r = sid - s0;
+                     if ((DIAG && sid < lo_col) || (unsigned)r >= (unsigned)n)
+                         return;
+                     atomicAdd(&cnt[k.off + r], k.inc);
+                 });
             __syncthreads();
             if (fused) {
                 fold(n, s0, per2{});
@@ -1712,14 +1778,16 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
         return;
     } else {
         // counts laid out [word][SV], so that a wave's atomics spread over the banks
-        walk([&](int sid, int lv, uint32_t add) {
-            if (sid < lo_col)
-                return;
-            if (PACK == 2)
-                atomicAdd(&cnt[(lv >> 1) * ns + sid], 1u << (16 * (lv & 1)));
-            else
-                atomicAdd(&cnt[lv * ns + sid], add);
-        });
+        walk(
+            [&](int lv) {
+                return PACK == 2 ? CntKey{(lv >> 1) * ns, 1u << (16 * (lv & 1))}
+                                 : CntKey{lv * ns, 0u};
+            },
+            [&](int sid, CntKey k, uint32_t add) {
+                if (DIAG && sid < lo_col)
+                    return;
+                atomicAdd(&cnt[k.off + sid], PACK == 2 ? k.inc : add);
+            });
         __syncthreads();
         if (fused) {
             fold(ns, 0, std::integral_constant<int, PACK == 2 ? 2 : 1>{});
@@ -1745,8 +1813,11 @@ __global__ __launch_bounds__(INDEX_THREADS) void k_index_tile(IndexArgs a)
     if (chunk >= a.nchunks)
         return;
     const int b = blockIdx.y;
-    index_block<PACK, RK, W16>(a, b, chunk, tile_id,
-                               ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id);
+    const int64_t slot = ((int64_t)chunk * a.B + b) * a.n_tiles + tile_id;
+    if (a.col_min >= 0)  // the Gram's upper triangle
+        index_block<PACK, RK, W16, false, true>(a, b, chunk, tile_id, slot);
+    else
+        index_block<PACK, RK, W16>(a, b, chunk, tile_id, slot);
 }
 
 // The persistent form: as many blocks as the chip holds at once (one per CU),
@@ -1948,28 +2019,32 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                 constexpr int XT = RT ? IXEXP_TILES_REC : IXEXP_TILES;
                 for (int tl0 = 0; tl0 < (walked ? 0 : nt); tl0 += XT) {
                     uint32_t e0[XT], rid[XT][R0], rwt[XT][R0];
-                    int c[XT];
+                    int c[XT], n0[XT];  // entries, and those of the first round
 #pragma unroll
                     for (int x = 0; x < XT; ++x) {
                         const uint32_t key = v * (uint32_t)nt + (uint32_t)(tl0 + x);
                         e0[x] = 0u;
-                        c[x] = 0;
+                        c[x] = n0[x] = 0;
                         if (tl0 + x < nt) {
                             if (RT) {
                                 const uint4 *r = (const uint4 *)(a.records + (size_t)key * 8);
                                 const uint4 ra = r[0], rb = r[1];
-                                const uint32_t wd[6] = {ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
-                                e0[x] = ra.x;
-                                c[x] = (int)ra.y;
+                                const uint32_t w[8] = {ra.x, ra.y, ra.z, ra.w,
+                                                       rb.x, rb.y, rb.z, rb.w};
+                                const RecBucket bk = rec_bucket<RT>(ra, rb);
+                                e0[x] = bk.e0;
+                                c[x] = bk.c;
+                                n0[x] = bk.n_in;
 #pragma unroll
                                 for (int k = 0; k < R0; ++k) {
-                                    rid[x][k] = rec_id(wd, k);
-                                    rwt[x][k] = RT == 2 ? rec_val(wd, k) : 1u;
+                                    rid[x][k] = rec_id<RT>(w, k);
+                                    rwt[x][k] = RT == 2 ? rec_val(w, k) : 1u;
                                 }
                             } else {
                                 const bounds2 r = *(const bounds2 *)(a.offsets + key);
                                 e0[x] = r.x;
                                 c[x] = (int)(r.y - r.x);
+                                n0[x] = min(c[x], R0);
                             }
                         }
                     }
@@ -1979,7 +2054,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                         const double *ect = ecb + (size_t)(tl0 + x) * a.tile;
 #pragma unroll
                         for (int k = 0; k < R0; ++k) {
-                            const bool in = k < c[x];
+                            const bool in = k < n0[x];
                             if (!RT) {
                                 rid[x][k] = in ? a.ids[e0[x] + k] : 0u;
                                 rwt[x][k] = a.vals && in ? a.vals[e0[x] + k] : 1u;
@@ -1995,7 +2070,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
 #pragma unroll
                     for (int x = 0; x < XT; ++x) {
                         const double *ect = ecb + (size_t)(tl0 + x) * a.tile;
-                        for (int kb = R0; kb < c[x]; kb += IXEXP_ROUND) {
+                        for (int kb = n0[x]; kb < c[x]; kb += IXEXP_ROUND) {
                             double cw[IXEXP_ROUND];
 #pragma unroll
                             for (int k = 0; k < IXEXP_ROUND; ++k) {
@@ -2265,6 +2340,14 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
         return fail(-1, "%s: pack_counts must be 0, 1 or 2, got %d", who, pack_counts);
     if (records != nullptr && (record_tables < 1 || record_tables > l))
         return fail(-1, "%s: record_tables must be 1..l (%d), got %d", who, l, record_tables);
+    // the kernels address the record tables with 32-bit byte offsets
+    if (records != nullptr &&
+        (uint64_t)record_tables * ((uint64_t)1 << (2 * l)) * (uint64_t)n_tiles * 32u >
+            ((uint64_t)1 << 32))
+        return fail(-1, "%s: record tables over 4 GiB (%d tables, l=%d, %d tiles)", who,
+                    record_tables, l, n_tiles);
+    if (persistent && col_min >= 0)
+        return fail(-1, "%s: the persistent launch takes no col_min", who);
     if (wfix != nullptr && nm > 16)
         return fail(-1, "%s: exact scoring takes at most 16 levels, got %d", who, nm);
     // counts per LDS word: 1 (uint32), 2 (16-bit), 4 (8-bit); weighted

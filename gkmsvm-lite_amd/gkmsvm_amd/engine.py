@@ -448,26 +448,41 @@ class DeviceIndex:
         if self.n_entries >= 2**31:
             raise MemoryError("l-mer index over 2^31 entries")
         self.offsets = offsets.to(torch.int32)  # uint32 bits
-        # bucket records, 32 bytes per key (the l-mer index query): start,
-        # count, the bucket's first RECORD_IDS ids (or RECORD_IDS_W ids and their
-        # uint8 weights) -- bounds and ids in one load, the n_tiles records of
-        # an l-mer in one cache line
+        # bucket records, 32 bytes per key (the l-mer index query) -- the count
+        # and ids in one load, the n_tiles records of an l-mer in one cache
+        # line.  Unweighted: halfword 0 = min(count, 0xFFFF); a bucket of at
+        # most RECORD_IDS entries holds its ids in halfwords 1..count, a longer
+        # one its first RECORD_IDS_BIG ids in halfwords 1..11, its start in
+        # word 6 and its count in word 7.  Weighted: start, count, the first
+        # RECORD_IDS_W ids (halfwords 4..11) and their uint8 weights (bytes
+        # 24..31).
         self.records = None
         nk = nl * self.n_tiles
         ks = key[order]
         if (self.pack16 or self.vals is not None) and nk * 32 <= RECORD_MAX_BYTES:
             rec = torch.zeros((nk, 16), dtype=torch.int16, device=dev)
             r32 = rec.view(torch.int32)
-            r32[:, 0] = self.offsets[:-1]
-            r32[:, 1] = (offsets[1:] - offsets[:-1]).to(torch.int32)
+            cnt = offsets[1:] - offsets[:-1]
             rank = torch.arange(self.n_entries, device=dev) - offsets[ks]
-            sel = rank < (RECORD_IDS if self.vals is None else RECORD_IDS_W)
-            rec.view(-1)[ks[sel] * 16 + 4 + rank[sel]] = self.ids[sel]
-            if self.vals is not None:
+            if self.vals is None:
+                big = cnt > RECORD_IDS
+                h0 = cnt.clamp(max=0xFFFF)
+                rec[:, 0] = (h0 - (h0 >= 0x8000).to(torch.int64) * 0x10000).to(torch.int16)
+                cap = torch.where(big, RECORD_IDS_BIG, RECORD_IDS)
+                sel = rank < cap[ks]
+                rec.view(-1)[ks[sel] * 16 + 1 + rank[sel]] = self.ids[sel]
+                r32[big, 6] = self.offsets[:-1][big]
+                r32[big, 7] = cnt[big].to(torch.int32)
+                del big, h0, cap
+            else:
+                r32[:, 0] = self.offsets[:-1]
+                r32[:, 1] = cnt.to(torch.int32)
+                sel = rank < RECORD_IDS_W
+                rec.view(-1)[ks[sel] * 16 + 4 + rank[sel]] = self.ids[sel]
                 rb = rec.view(torch.uint8)
                 rb.view(-1)[ks[sel] * 32 + 24 + rank[sel]] = self.vals[sel].to(torch.uint8)
             self.records = r32
-            del rec, rank, sel
+            del rec, rank, sel, cnt
         # l-mer records, 64 bytes per l-mer (GkmExplain through the index, which
         # sums over every tile): unweighted with at most LREC_MAX_TILES tiles,
         # whose buckets of one l-mer are consecutive -- the run's start, the
@@ -486,11 +501,12 @@ class DeviceIndex:
             self.lmer_records = r32
             del rec, lmer, rank, sel, cnt
         del ks
-        # at least one entry: the query reads entry 0 where a lane has none
-        if self.ids.numel() == 0:
-            self.ids = torch.zeros(1, dtype=torch.int16, device=dev)
-            if self.vals is not None:
-                self.vals = torch.zeros(1, dtype=self.vals.dtype, device=dev)
+        # INDEX_PAD entries past the last bucket: the query reads ahead of a
+        # bucket's overflow (and reads the first entries where a lane has none)
+        self.ids = torch.cat([self.ids, torch.zeros(INDEX_PAD, dtype=torch.int16, device=dev)])
+        if self.vals is not None:
+            self.vals = torch.cat([self.vals,
+                                   torch.zeros(INDEX_PAD, dtype=self.vals.dtype, device=dev)])
         # record tables 1..q: the bucket records again with the l-mer's digits 0
         # and q swapped, so a pair of masks differing in bit 0 of digit q reads
         # one 128-byte line (with at most 2 tiles an l-mer pair's records of a
@@ -552,7 +568,9 @@ def index_tile(nm: int, pack: int | bool = 1) -> int:
 MAX_PARTIAL_BYTES = 8 << 30  # per-chunk index counts of one launch (rows per launch)
 LREC_IDS = 28                # ids held in a 64-byte l-mer record
 LREC_MAX_TILES = 4           # tiles an l-mer record covers
-RECORD_IDS = 12              # ids held in a bucket record
+RECORD_IDS = 15              # ids an unweighted bucket record holds (buckets up to 15)
+RECORD_IDS_BIG = 11          # ids it holds beside the start and count (longer buckets)
+INDEX_PAD = 16               # entries past the last bucket of ids and vals (GKM_INDEX_PAD)
 RECORD_IDS_W = 8             # (id, weight) pairs held in a record of a weighted index
 RECORD_MAX_BYTES = 4 << 30   # larger record tables (l >= 13 with many tiles) are not built
 RECORD_TABLES_MAX_BYTES = 4 << 30  # all the digit-swapped record tables together

@@ -30,6 +30,11 @@
 
 #include <stdint.h>
 
+/* Entries the l-mer index's ids (and vals) arrays hold past their last
+ * bucket: the index query reads up to this many entries ahead of a
+ * bucket's overflow without branching around the loads. */
+#define GKM_INDEX_PAD 16
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -203,8 +208,11 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * (needed for window weights; NULL: in order); the
  * index is a CSR over keys lmer * n_tiles + tile (offsets [4^l n_tiles + 1]
  * uint32; ids uint16 SV within its tile of `tile` SVs; vals uint8 window
- * weights or NULL) of every SV window's l-mer, forward and RC; records: NULL
- * or [record_tables][4^l n_tiles][8] uint32 bucket records, one per key:
+ * weights or NULL) of every SV window's l-mer, forward and RC; ids and vals
+ * hold GKM_INDEX_PAD entries past the last bucket (read ahead, never
+ * counted), and at least that many in all; records: NULL
+ * or [record_tables][4^l n_tiles][8] uint32 bucket records, one per key
+ * (at most 4 GiB of tables in all):
  * start, count, the bucket's first 12 ids (uint16), or with vals its first 8
  * ids and their uint8 weights -- one 32-byte load per probe; table q (1 <=
  * record_tables <= l) holds them at the key of the l-mer with its digits 0

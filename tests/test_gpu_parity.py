@@ -484,6 +484,38 @@ def test_index_mixed_bucket_sizes(gpu):
             np.testing.assert_array_equal(c.cpu().numpy(), b.cpu().numpy())
 
 
+def test_index_record_bucket_sizes(gpu):
+    """Buckets on either side of the bucket records' limits give the brute-force
+    integers: unweighted records hold up to 15 ids, a longer bucket 11 of them
+    with its start and count (the rest from the ids array: up to 12 taken by
+    the lanes, more walked balanced); weighted records hold 8 (id, weight)
+    pairs.  A motif copied into n SVs makes a bucket of n entries."""
+    from gkmsvm_amd import engine as E
+
+    rng = np.random.default_rng(41)
+    l, L = 11, 260
+    sizes = [1, 8, 9, 14, 15, 16, 17, 20, 21, 22, 23, 24, 25, 40]
+    S = 96
+    sv = rng.integers(0, 4, (S, L)).astype(np.uint8)
+    q = rng.integers(0, 4, (3, L)).astype(np.uint8)
+    for j, n in enumerate(sizes):
+        motif = rng.integers(0, 4, l).astype(np.uint8)
+        pos = 20 + 15 * j  # motifs 15 apart: no two overlap
+        sv[rng.choice(S, n, replace=False), pos:pos + l] = motif
+        q[j % 3, pos:pos + l] = motif
+    for kt, p in ((2, {"L": l, "k": 7, "d": 3}), (4, {"L": l, "k": 7, "d": 3, "M": 50, "H": 50})):
+        m = _model(kt, p, sv, np.full(S, L), rng.normal(size=S), 0.0)
+        s = m._svs()
+        idx = s.index()
+        cnt = (idx.offsets[1:].long() - idx.offsets[:-1].long()).cpu().numpy()
+        assert all((cnt >= n).any() for n in sizes)
+        xs = E.DeviceSeqs(q, np.full(len(q), L), gpu, rc=s.kp.rc)
+        a = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S)
+        b = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 @pytest.mark.parametrize("d", [1, 2, 3, 4, 5])
 def test_index_count_widths(gpu, d):
     """8-bit LDS counts (with the exact 16-bit fallback), 16-bit and 32-bit
@@ -578,7 +610,7 @@ def test_peak_microbenchmarks_run(gpu):
 
 def test_index_without_entries(gpu):
     """An index whose SVs are all shorter than l holds no entries (the query's
-    loads then read its one padding entry): the histograms are zero, equal to
+    loads then read its padding entries): the histograms are zero, equal to
     the pairwise kernel's."""
     from gkmsvm_amd import engine as E
 
@@ -587,7 +619,7 @@ def test_index_without_entries(gpu):
     m = _model(2, {"L": 11, "k": 7, "d": 3}, sv, np.full(40, 9), rng.normal(size=40))
     s = m._svs()
     idx = s.index()
-    assert idx.n_entries == 0 and idx.ids.numel() == 1
+    assert idx.n_entries == 0 and idx.ids.numel() == E.INDEX_PAD
     qc, ql = _rand_codes(rng, 3, 20, 300)
     xs = E.DeviceSeqs(qc, ql, gpu, rc=s.kp.rc)
     got = E.index_hist(s.kp, idx, xs, 0, xs.n, s.S).cpu().numpy()
