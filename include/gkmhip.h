@@ -212,9 +212,12 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * hold GKM_INDEX_PAD entries past the last bucket (read ahead, never
  * counted), and at least that many in all; records: NULL
  * or [record_tables][4^l n_tiles][8] uint32 bucket records, one per key
- * (at most 4 GiB of tables in all):
- * start, count, the bucket's first 12 ids (uint16), or with vals its first 8
- * ids and their uint8 weights -- one 32-byte load per probe; table q (1 <=
+ * (at most 4 GiB of tables in all), 16 uint16 halfwords each -- one
+ * 32-byte load per probe.  Without vals: halfword 0 = min(count, 0xFFFF); a
+ * bucket of at most 15 entries has its ids in halfwords 1..count, a longer
+ * one its first 11 ids in halfwords 1..11, its start (uint32) in word 6 and
+ * its count in word 7.  With vals: word 0 start, word 1 count, the first 8
+ * ids in halfwords 4..11 and their uint8 weights in bytes 24..31.  Table q (1 <=
  * record_tables <= l) holds them at the key of the l-mer with its digits 0
  * and q swapped (table 0: in key order).  masks / levels [n_masks]: the XOR
  * neighbourhood of mismatch counts < nm with non-zero weight; levels[t] =
