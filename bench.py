@@ -357,6 +357,23 @@ def kernel_only(args):
         torch.cuda.synchronize()
         print(json.dumps({"mode": args.mode, "launches": args.launches}), flush=True)
         return
+    if args.mode == "explain":
+        # a serving step's explain rows: 10 x 1 kb hypothetical attributions
+        # through the l-mer index (k_explain_coef, k_index_explain)
+        from gkmsvm_amd import gkmexplain
+
+        model.use_explain_table = False
+        q = rng.integers(0, 4, (10, 1000)).astype(np.uint8)
+        r = gkmexplain(model, q, mode="hypothetical")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.launches):
+            r = gkmexplain(model, q, mode="hypothetical")
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": args.mode, "launches": args.launches,
+                          "ms_per_launch": (time.perf_counter() - t) / args.launches * 1e3,
+                          "checksum": float(r.double().abs().sum().item())}), flush=True)
+        return
     if args.mode == "score":
         # the serving step's persistent launch: the scoring and variant rows of
         # one 96-request burst of the stream (seed 5) scored through the index
@@ -400,11 +417,11 @@ def kernel_only(args):
 
 
 def peaks(args):
-    """The two peaks the rooflines use, measured: HBM read bandwidth (a
-    streaming read of 4 GiB, and torch's device copy of 2 GiB for comparison)
-    and the int32 VALU rate (chains of single XOR / add / popcount
-    instructions), each the best of several launch shapes, HIP events on the
-    launch stream."""
+    """The peaks the rooflines use, measured: HBM read bandwidth (a streaming
+    read of 4 GiB, and torch's device copy of 2 GiB for comparison), random
+    128-byte line reads by footprint (the l-mer index's access pattern) and
+    the int32 VALU rate (chains of single XOR / add / popcount instructions),
+    each the best of several launch shapes, HIP events on the launch stream."""
     import torch
 
     from gkmsvm_amd import _lib
@@ -431,6 +448,16 @@ def peaks(args):
         t = timed(lambda: _lib.call("gkm_peak_read", src.data_ptr(), nbytes, sink.data_ptr(),
                                     blocks, h), 10)
         read[str(blocks)] = nbytes / t / 1e9
+    # random 128-byte lines (two per lane pair in flight, 512 blocks of 1,024
+    # threads) over footprints from one L2's size to 4 GiB: lines per second
+    gather = {}
+    rounds = 256
+    for mb in (4, 32, 256, 4096):
+        fb = mb << 20
+        t = timed(lambda: _lib.call("gkm_peak_gather", src.data_ptr(), fb, sink.data_ptr(), 512,
+                                    rounds, h), 5)
+        lines = 512 * 1024 / 2 * rounds
+        gather[f"{mb}MB"] = {"lines_per_s": lines / t, "GBps_128B": lines * 128 / t / 1e9}
     half = src[: src.numel() // 2]
     dst = torch.empty_like(half)
     t = timed(lambda: dst.copy_(half), 10)
@@ -445,6 +472,7 @@ def peaks(args):
         "mode": "peaks",
         "hbm_read_GBps": max(read.values()), "hbm_read_by_blocks": read,
         "hbm_copy_read_plus_write_GBps": copy_gbs, "hbm_peak_GBps": HBM_PEAK / 1e9,
+        "random_128B_lines_by_footprint": gather,
         "valu_int32_TOPs": max(valu.values()), "valu_by_blocks": valu,
         "valu_peak_used_TOPs": INT_VALU_PEAK / 1e12,
         "valu_derived_TOPs": INT_VALU_DERIVED / 1e12}), flush=True)
@@ -583,13 +611,13 @@ def main():
     ap.add_argument("--no-tables", action="store_true",
                     help="skip the l-mer table runs (profiles of the per-SV kernels)")
     ap.add_argument("--mode", default="serve",
-                    choices=["serve", "cross", "index", "score", "table", "copy", "peaks",
-                             "ism", "configs"],
-                    help="serve: the benchmark; cross/index/score/table/copy: one kernel "
-                         "family on the serving shape, for profiling (score: one step's "
-                         "scoring rows in the persistent index launch); ism: the -t 5 ISM "
-                         "sample, for profiling; configs: bounded timings of the other "
-                         "configurations")
+                    choices=["serve", "cross", "index", "score", "explain", "table", "copy",
+                             "peaks", "ism", "configs"],
+                    help="serve: the benchmark; cross/index/score/explain/table/copy: one "
+                         "kernel family on the serving shape, for profiling (score: one "
+                         "step's scoring rows in the persistent index launch; explain: 10 "
+                         "x 1 kb rows through the index); ism: the -t 5 ISM sample, for "
+                         "profiling; configs: bounded timings of the other configurations")
     ap.add_argument("--c3-queries", type=int, default=100_000)
     ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)

@@ -1,8 +1,9 @@
-// Peak microbenchmarks of the two rooflines the hot kernels are measured
-// against: HBM read bandwidth (a streaming read of a large buffer) and the
-// int32 VALU rate (independent chains of XOR, add and popcount, written as
-// single instructions so that each one is one lane operation).  Measurement
-// only: nothing on the scoring path calls them.
+// Peak microbenchmarks of the rooflines the hot kernels are measured against:
+// HBM read bandwidth (a streaming read of a large buffer), random 128-byte
+// line reads (the l-mer index's access pattern) and the int32 VALU rate
+// (independent chains of XOR, add and popcount, written as single
+// instructions so that each one is one lane operation).  Measurement only:
+// nothing on the scoring path calls them.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -66,6 +67,62 @@ __global__ __launch_bounds__(PEAK_THREADS) void k_peak_valu(int rounds, uint32_t
         x ^= a[k];
     if (x == 0x9E3779B9u)
         sink[0] = x;
+}
+
+// Random 128-byte lines: each pair of lanes reads two 16-byte pieces of one
+// line (32 bytes per lane, the index query's record pattern), `depth` lines
+// per lane pair in flight per round; the line of round r is a hash of the lane
+// pair and r, below n_lines.  Blocks of 1,024 threads.
+constexpr int GATHER_THREADS = 1024;
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+template <int DEPTH>
+__global__ __launch_bounds__(GATHER_THREADS) void k_peak_gather(const u32x4 *__restrict__ src,
+                                                                uint64_t n_lines, int rounds,
+                                                                uint32_t *sink)
+{
+    const uint32_t gid = blockIdx.x * GATHER_THREADS + threadIdx.x;
+    const uint32_t pair = gid >> 1, half = gid & 1u;
+    uint32_t acc = 0;
+    for (int r = 0; r < rounds; r += DEPTH) {
+        u32x4 v[DEPTH][2];
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) {
+            const uint64_t line =
+                (uint64_t)mix32(pair * 7919u + (uint32_t)(r + d) * 104729u) % n_lines;
+            const u32x4 *q = src + line * 8 + half * 4;  // 8 vectors per 128-byte line
+            v[d][0] = q[0];
+            v[d][1] = q[1];
+        }
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d)
+            acc ^= v[d][0].x ^ v[d][0].y ^ v[d][0].z ^ v[d][0].w ^ v[d][1].x ^ v[d][1].y ^
+                   v[d][1].z ^ v[d][1].w;
+    }
+    if (acc == 0x9E3779B9u)
+        sink[0] = acc;
+}
+
+GKM_API int gkm_peak_gather(const void *src, int64_t bytes, void *sink, int blocks, int rounds,
+                            void *stream)
+{
+    if (bytes < 128 || bytes % 128 != 0 || blocks < 1 || rounds < 2 || rounds % 2 != 0)
+        return fail(-1, "gkm_peak_gather: bytes must be a positive multiple of 128, blocks >= 1, "
+                        "rounds even and >= 2");
+    hipLaunchKernelGGL(k_peak_gather<2>, dim3((unsigned)blocks), dim3(GATHER_THREADS), 0,
+                       S_(stream), (const u32x4 *)src, (uint64_t)bytes / 128, rounds,
+                       (uint32_t *)sink);
+    GKM_CHECK_LAUNCH("gkm_peak_gather");
+    return 0;
 }
 
 GKM_API int gkm_peak_read(const void *src, int64_t bytes, void *sink, int blocks, void *stream)

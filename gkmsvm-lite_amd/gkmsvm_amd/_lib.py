@@ -68,6 +68,7 @@ SIGNATURES = {
     "gkm_deltasvm_score": [P, P, I, I, I, I, P, I, P, I, D, P, P],
     "gkm_peak_read": [P, I64, P, I, P],
     "gkm_peak_valu": [I, P, I, P],
+    "gkm_peak_gather": [P, I64, P, I, I, P],
 }
 _RESTYPE = {"gkm_last_error": ctypes.c_char_p, "gkm_scratch_bytes": ctypes.c_int64,
             "gkm_index_partial_bytes": ctypes.c_int64, "gkm_index_psum_bytes": ctypes.c_int64,

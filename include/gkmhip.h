@@ -427,6 +427,11 @@ int gkm_deltasvm_score(const uint8_t *codes, const int32_t *xlen, int B, int lma
  * 256 threads.  sink: 4 device bytes, written only if a sentinel comes up. */
 int gkm_peak_read(const void *src, int64_t bytes, void *sink, int blocks, void *stream);
 int gkm_peak_valu(int rounds, void *sink, int blocks, void *stream);
+/* gkm_peak_gather reads `rounds` random 128-byte lines of the first `bytes`
+ * (a multiple of 128) of src per lane pair, 32 bytes per lane, two lines per
+ * lane pair in flight, with `blocks` blocks of 1,024 threads (rounds even). */
+int gkm_peak_gather(const void *src, int64_t bytes, void *sink, int blocks, int rounds,
+                    void *stream);
 
 #ifdef __cplusplus
 }
