@@ -1299,7 +1299,14 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     // lane t in group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
     // v = w ^ mask[m], whose bucket is [e0, e0 + c).  Returns this lane's
     // number of entries walked (every one of them was handed to count once).
-    auto walk = [&](auto keyf, auto count) -> uint32_t {
+    // plain: count(sid, key, w) is atomicAdd(&cnt[key.off + sid], key.inc)
+    // (std::true_type), so an entry slot a lane does not hold can add 0 to a
+    // word of its own instead of being branched around: the record and
+    // first-round counts then run without exec masks (fewer scalar
+    // instructions per step)
+    const int safe_word = min(lane, ncnt - 1);
+    auto walk = [&](auto keyf, auto count, auto plain) -> uint32_t {
+        constexpr bool PLAIN = decltype(plain)::value;
         using Key = decltype(keyf(0));
         const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
         uint32_t walked = 0;
@@ -1508,10 +1515,16 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p)
 #pragma unroll
-                for (int k = 0; k < record_ids<RK>(); ++k)
-                    if (k < n_in[p])
+                for (int k = 0; k < record_ids<RK>(); ++k) {
+                    if constexpr (PLAIN) {
+                        const bool in = k < n_in[p];
+                        count(in ? (int)rec_id<RK>(rw[p], k) : safe_word,
+                              in ? key[p] : Key{0, 0u}, 1u);
+                    } else if (k < n_in[p]) {
                         count((int)rec_id<RK>(rw[p], k), key[p],
                               PACK > 1 && !W16 ? 1u : aw * (RK == 2 ? rec_val(rw[p], k) : 1u));
+                    }
+                }
             if (__ballot(!small) == 0) {
                 // the rests, from the loads above: entry k of probe p's rest for
                 // k < c - n_in; the round stops at the first k past every
@@ -1529,9 +1542,14 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                     if (__ballot(k < dtop) == 0)
                         break;
 #pragma unroll
-                    for (int p = 0; p < INDEX_PROBES; ++p)
-                        if (k < d[p])
+                    for (int p = 0; p < INDEX_PROBES; ++p) {
+                        if constexpr (PLAIN) {
+                            const bool in = k < d[p];
+                            count(in ? sid0[p][k] : safe_word, in ? key[p] : Key{0, 0u}, 1u);
+                        } else if (k < d[p]) {
                             count(sid0[p][k], key[p], PACK > 1 && !W16 ? 1u : aw * wv0[p][k]);
+                        }
+                    }
                 }
                 return;
             }
@@ -1622,7 +1640,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                      return;
                  adds += add;
                  atomicAdd(&cnt[k.off + sid], add << k.inc);
-             });
+             },
+             std::false_type{});
         __syncthreads();
         unsigned long long d = adds;
         for (int i = threadIdx.x; i < ncnt; i += INDEX_THREADS) {
@@ -1665,7 +1684,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                      if ((DIAG && sid < lo_col) || (unsigned)r >= (unsigned)n)
                          return;
                      atomicAdd(&cnt[k.off + r], add);
-                 });
+                 },
+                 std::false_type{});
             __syncthreads();
             if (fused) {
                 fold(n, s0, per1{});
@@ -1691,7 +1711,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                     ++adds;
                 }
                 atomicAdd(&cnt[k.off + sid], k.inc);
-            });
+            },
+            std::integral_constant<bool, !DIAG>{});
         if constexpr (!DIAG)
             adds = walked;
         __syncthreads();
@@ -1755,7 +1776,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                      if ((DIAG && sid < lo_col) || (unsigned)r >= (unsigned)n)
                          return;
                      atomicAdd(&cnt[k.off + r], k.inc);
-                 });
+                 },
+                 std::false_type{});
             __syncthreads();
             if (fused) {
                 fold(n, s0, per2{});
@@ -1787,7 +1809,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 if (DIAG && sid < lo_col)
                     return;
                 atomicAdd(&cnt[k.off + sid], PACK == 2 ? k.inc : add);
-            });
+            },
+            std::integral_constant<bool, PACK == 2 && !DIAG>{});
         __syncthreads();
         if (fused) {
             fold(ns, 0, std::integral_constant<int, PACK == 2 ? 2 : 1>{});
