@@ -1019,12 +1019,17 @@ constexpr int INDEX_PASSES = 4;      // 64-entry passes of a balanced walk issue
 constexpr int RECORD_IDS = 15;       // ids a 32-byte bucket record holds (unweighted, short)
 constexpr int RECORD_IDS_BIG = 11;   // ids it holds beside the start and count (long)
 constexpr int RECORD_IDS_W = 8;      // (id, weight) pairs held in one (weighted index)
+constexpr int RECORD_SINKS = WAVE;   // sink words past an index tile's LDS counts (one a lane)
 
 // Bucket records, 8 words (16 halfwords), one per key.
 // RK 1 (unweighted): halfword 0 = min(count, 0xFFFF); a bucket of at most
 // RECORD_IDS entries has its ids in halfwords 1..count (no start: nothing past
-// them); a longer one its first RECORD_IDS_BIG ids in halfwords 1..11, its
-// start in word 6 and its count in word 7 (the rest at ids[start + 11 ..]).
+// them) and 0xFFFF in halfwords count + 1..15 (the sink mark: a lane adds
+// every slot unconditionally, at min(slot, ns + lane) -- ns the tile's SV count,
+// the lane's own LDS word past the tile's counts, so the padding adds of a
+// wave meet on no word and no bank); a longer one its first RECORD_IDS_BIG ids
+// in halfwords 1..11, its start in word 6 and its count in word 7 (the rest at
+// ids[start + 11 ..]).
 // RK 2 (weighted): word 0 start, word 1 count, RECORD_IDS_W uint16 ids in
 // halfwords 4..11 and their uint8 weights in bytes 24..31.
 // RK 3: one 64-byte record per l-mer for unweighted indexes of at most 4 SV
@@ -1299,14 +1304,21 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     // lane t in group g takes mask m = (INDEX_PROBES g + p) 64 + t, neighbour
     // v = w ^ mask[m], whose bucket is [e0, e0 + c).  Returns this lane's
     // number of entries walked (every one of them was handed to count once).
-    // plain: count(sid, key, w) is atomicAdd(&cnt[key.off + sid], key.inc)
-    // (std::true_type), so an entry slot a lane does not hold can add 0 to a
-    // word of its own instead of being branched around: the record and
-    // first-round counts then run without exec masks (fewer scalar
-    // instructions per step)
+    // plain (1): count(sid, key, w) is atomicAdd(&cnt[key.off + sid], key.inc),
+    // so an entry slot a lane does not hold can add 0 to a word of its own
+    // instead of being branched around: the record and first-round counts
+    // then run without exec masks (fewer scalar instructions per step).
+    // sink (2, one count word per SV, key.off 0): every record slot is added,
+    // with the probe's increment (0 for a probe past the masks), at
+    // min(slot, ns + lane) -- the slots past a short bucket's ids hold
+    // 0xFFFF, so they land on the lane's own word past the counts (which
+    // nothing reads), and a long bucket's start and count slots are forced
+    // there: two instructions per slot, no compare
     const int safe_word = min(lane, ncnt - 1);
+    const uint32_t sink_lane = (uint32_t)(ns + lane);
     auto walk = [&](auto keyf, auto count, auto plain) -> uint32_t {
-        constexpr bool PLAIN = decltype(plain)::value;
+        constexpr int PMODE = (int)decltype(plain)::value;
+        constexpr bool PLAIN = PMODE == 1;
         using Key = decltype(keyf(0));
         const int n_groups = (a.n_masks + INDEX_PROBES * WAVE - 1) / (INDEX_PROBES * WAVE);
         uint32_t walked = 0;
@@ -1446,6 +1458,7 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             uint32_t rw[INDEX_PROBES][8], rb0[INDEX_PROBES];
             int c[INDEX_PROBES], lv[INDEX_PROBES], n_in[INDEX_PROBES];
             Key key[INDEX_PROBES];
+            uint32_t sinc[INDEX_PROBES];  // sink mode: the probe's increment
             bool small = true;
 #pragma unroll
             for (int p = 0; p < INDEX_PROBES; ++p) {
@@ -1467,6 +1480,8 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
                 rb0[p] = bk.e0 + (uint32_t)n_in[p];    // the rest of the bucket from here
                 lv[p] = max(nlvb[p], 0);  // (nlvb is the next fetch's once fetched)
                 key[p] = keyf(lv[p]);
+                if constexpr (PMODE == 2)
+                    sinc[p] = nlvb[p] < 0 ? 0u : key[p].inc;
                 walked += (uint32_t)c[p];
                 // a group whose buckets' rests all fit INDEX_WALK entries takes
                 // them from the loads below; a longer rest (repeats) is walked
@@ -1516,7 +1531,13 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
             for (int p = 0; p < INDEX_PROBES; ++p)
 #pragma unroll
                 for (int k = 0; k < record_ids<RK>(); ++k) {
-                    if constexpr (PLAIN) {
+                    if constexpr (PMODE == 2) {
+                        static_assert(RK == 1, "sink slots are a property of RK 1 records");
+                        uint32_t id = rec_id<RK>(rw[p], k);
+                        if (k >= RECORD_IDS_BIG)
+                            id |= c[p] > RECORD_IDS ? 0xFFFFu : 0u;
+                        count((int)min(id, sink_lane), Key{0, sinc[p]}, 1u);
+                    } else if constexpr (PLAIN) {
                         const bool in = k < n_in[p];
                         count(in ? (int)rec_id<RK>(rw[p], k) : safe_word,
                               in ? key[p] : Key{0, 0u}, 1u);
@@ -1702,17 +1723,25 @@ __device__ __forceinline__ void index_block(const IndexArgs &a, int b, int chunk
     } else if constexpr (PACK == 4) {
         // every entry walked is one add, except (DIAG) those skipped
         uint32_t adds = 0;
-        const uint32_t walked = walk(
-            [&](int lv) { return CntKey{(lv >> 2) * ns, 1u << (8 * (lv & 3))}; },
-            [&](int sid, CntKey k, uint32_t) {
-                if constexpr (DIAG) {
-                    if (sid < lo_col)
-                        return;
-                    ++adds;
-                }
-                atomicAdd(&cnt[k.off + sid], k.inc);
-            },
-            std::integral_constant<bool, !DIAG>{});
+        auto key4 = [&](int lv) { return CntKey{(lv >> 2) * ns, 1u << (8 * (lv & 3))}; };
+        auto count4 = [&](int sid, CntKey k, uint32_t) {
+            if constexpr (DIAG) {
+                if (sid < lo_col)
+                    return;
+                ++adds;
+            }
+            atomicAdd(&cnt[k.off + sid], k.inc);
+        };
+        uint32_t walked;
+        if constexpr (RK == 1 && !DIAG) {
+            // up to 4 levels: one word per SV, the records' sink slots apply
+            if (wps == 1)
+                walked = walk(key4, count4, std::integral_constant<int, 2>{});
+            else
+                walked = walk(key4, count4, std::true_type{});
+        } else {
+            walked = walk(key4, count4, std::integral_constant<bool, !DIAG>{});
+        }
         if constexpr (!DIAG)
             adds = walked;
         __syncthreads();
@@ -2382,7 +2411,7 @@ static int launch_index_tile(const char *who, const uint32_t *xwin, const int32_
     if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
         return 0;
     const size_t words = (size_t)tile * ((nm + pack - 1) / pack);
-    size_t lds = words * sizeof(uint32_t);
+    size_t lds = (words + RECORD_SINKS) * sizeof(uint32_t);  // + the records' sink words
     if (lds > 159 * 1024)
         return fail(-2, "%s: a tile of %d SVs x %d levels exceeds LDS", who, tile, nm);
     IndexArgs a;

@@ -468,6 +468,7 @@ class DeviceIndex:
                 big = cnt > RECORD_IDS
                 h0 = cnt.clamp(max=0xFFFF)
                 rec[:, 0] = (h0 - (h0 >= 0x8000).to(torch.int64) * 0x10000).to(torch.int16)
+                rec[:, 1:] = -1  # 0xFFFF past the ids: the sink mark
                 cap = torch.where(big, RECORD_IDS_BIG, RECORD_IDS)
                 sel = rank < cap[ks]
                 rec.view(-1)[ks[sel] * 16 + 1 + rank[sel]] = self.ids[sel]

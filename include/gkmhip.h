@@ -214,9 +214,10 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
  * or [record_tables][4^l n_tiles][8] uint32 bucket records, one per key
  * (at most 4 GiB of tables in all), 16 uint16 halfwords each -- one
  * 32-byte load per probe.  Without vals: halfword 0 = min(count, 0xFFFF); a
- * bucket of at most 15 entries has its ids in halfwords 1..count, a longer
- * one its first 11 ids in halfwords 1..11, its start (uint32) in word 6 and
- * its count in word 7.  With vals: word 0 start, word 1 count, the first 8
+ * bucket of at most 15 entries has its ids in halfwords 1..count and 0xFFFF
+ * in halfwords count + 1..15 (the kernel adds those slots to LDS words past
+ * the tile's counts, which nothing reads), a longer one its first 11 ids in halfwords 1..11, its start (uint32) in word 6
+ * and its count in word 7.  With vals: word 0 start, word 1 count, the first 8
  * ids in halfwords 4..11 and their uint8 weights in bytes 24..31.  Table q (1 <=
  * record_tables <= l) holds them at the key of the l-mer with its digits 0
  * and q swapped (table 0: in key order).  masks / levels [n_masks]: the XOR
