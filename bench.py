@@ -222,6 +222,7 @@ HBM_KINDS = {
     "k_index_tile_queue": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 "
                           "RMW); the persistent launch",
     "k_index_explain": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
+    "k_lmer_explain": "13 B per (window, mask) probe: the mask, its level, V[v] (FP64)",
     "k_table_score": "4 B packed window + 8 B table entry per query window",
     "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",
 }
@@ -359,9 +360,13 @@ def kernel_only(args):
         return
     if args.mode == "explain":
         # a serving step's explain rows: 10 x 1 kb hypothetical attributions
-        # through the l-mer index (k_explain_coef, k_index_explain)
+        # through the SVs' l-mer weights (k_index_explain in its V form) or,
+        # with --explain-kernel index, the l-mer index (k_explain_coef,
+        # k_index_explain)
+        from gkmsvm_amd import explain as X
         from gkmsvm_amd import gkmexplain
 
+        X.EXPLAIN_KERNEL = args.explain_kernel
         model.use_explain_table = False
         q = rng.integers(0, 4, (10, 1000)).astype(np.uint8)
         r = gkmexplain(model, q, mode="hypothetical")
@@ -371,6 +376,7 @@ def kernel_only(args):
             r = gkmexplain(model, q, mode="hypothetical")
         torch.cuda.synchronize()
         print(json.dumps({"mode": args.mode, "launches": args.launches,
+                          "explain_kernel": args.explain_kernel,
                           "ms_per_launch": (time.perf_counter() - t) / args.launches * 1e3,
                           "checksum": float(r.double().abs().sum().item())}), flush=True)
         return
@@ -616,11 +622,13 @@ def main():
                     help="serve: the benchmark; cross/index/score/explain/table/copy: one "
                          "kernel family on the serving shape, for profiling (score: one "
                          "step's scoring rows in the persistent index launch; explain: 10 "
-                         "x 1 kb rows through the index); ism: the -t 5 ISM sample, for "
+                         "x 1 kb explain rows); ism: the -t 5 ISM sample, for "
                          "profiling; configs: bounded timings of the other configurations")
     ap.add_argument("--c3-queries", type=int, default=100_000)
     ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)
+    ap.add_argument("--explain-kernel", default="auto", choices=["auto", "lmer", "index", "sweep"],
+                    help="explain mode: the explain kernel (auto: the library's choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine's multi-GPU sharding is a design "

@@ -1949,8 +1949,10 @@ struct IndexExplainArgs {
     const uint8_t *wtab;     // query window weight by distance, or NULL
     const double *ta, *tb;   // explain tables [l + 2]
     const double *ec;        // [B][S] effective coefficients (SV order of the index)
+    const double *lw;        // RK 4: [4^l] the SVs' l-mer weights V, in place of the index
+    const double *dx;        // RK 4: [B] K_raw(x, x) (normalize), or NULL
     double *out;             // [B][4][lmax], added with FP64 atomics
-    int B, Wx, lmax, n_masks, l, S, tile, n_tiles, chunk_windows, pert;
+    int B, Wx, lmax, n_masks, l, S, tile, n_tiles, chunk_windows, pert, normalize;
 };
 
 // Block = (chunk of query windows, query row); a wave takes a window u and its
@@ -1967,6 +1969,10 @@ struct IndexExplainArgs {
 // records (RK: start, count, the first ids and weights) or bounds -- and gathers
 // the first round of coefficients of all of them with independent loads; a
 // bucket's entries past the first round follow IXEXP_ROUND at a time.
+// RK 4 (kernels without RBF): s is V[v], the SVs' l-mer weights -- the sum
+// over SVs of coef / sqrt(K(sv, sv)) times the windows equal to v (with their
+// window weights) -- one load per probe, and the row's 1 / sqrt(K(x, x))
+// applied at the end (explain/cpu.py:89-143, explain/lmer.py:31-254).
 template <int LT, int RK>
 __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArgs a)
 {
@@ -2020,7 +2026,9 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                 const uint32_t mask = a.masks[t];
                 const uint32_t v = u ^ mask;
                 double sv = 0.0;
-                bool walked = false;
+                bool walked = RK == 4;
+                if (RK == 4)
+                    sv = a.lw[v];
                 if (RK == 3) {
                     // the l-mer's record: its run over all tiles, the first
                     // LREC_IDS ids inline; run entry k is in the tile whose
@@ -2066,7 +2074,7 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
                     }
                 }
                 // per tile: 32-byte records (RK 1, 2) or bounds (RK 0; RK 3 saturated)
-                constexpr int RT = RK == 3 ? 0 : RK;
+                constexpr int RT = RK >= 3 ? 0 : RK;
                 constexpr int R0 = RT ? record_ids<RT>() : IXEXP_ROUND;  // first round
                 constexpr int XT = RT ? IXEXP_TILES_REC : IXEXP_TILES;
                 for (int tl0 = 0; tl0 < (walked ? 0 : nt); tl0 += XT) {
@@ -2155,7 +2163,9 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
         }
         // wave totals (every lane ends with them), then lane j < 4 l places
         // accumulator j: position k = j / 4, slot j % 4 (0: matches, 1..3: delta)
-        const double aw = a.wtab ? (double)window_weight(a.wtab, wx, i) : 1.0;
+        double aw = a.wtab ? (double)window_weight(a.wtab, wx, i) : 1.0;
+        if (RK == 4 && a.normalize)
+            aw *= 1.0 / sqrt(fmax(a.dx[b], 1e-20));
 #pragma unroll
         for (int k = 0; k < LM; k++) {
             for (int off = WAVE / 2; off > 0; off >>= 1) {
@@ -3524,6 +3534,9 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
     a.ta = ta;
     a.tb = tb;
     a.ec = ec;
+    a.lw = nullptr;
+    a.dx = nullptr;
+    a.normalize = 0;
     a.out = out;
     a.B = B;
     a.Wx = Wx;
@@ -3554,6 +3567,57 @@ GKM_API int gkm_index_explain(int perturbation, const uint32_t *xwin, const int3
 #undef GKM_IXEXP
     hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_index_explain");
+    return 0;
+}
+
+GKM_API int gkm_lmer_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B,
+                             int Wx, int lmax, const double *lw, const double *dx, int normalize,
+                             const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                             const uint8_t *wtab, int chunk_windows, const double *ta,
+                             const double *tb, double *out, void *stream)
+{
+    if (l < 1 || l > 13)
+        return fail(-1, "gkm_lmer_explain: the l-mer weights need 1 <= l <= 13, got %d", l);
+    if (B < 0 || B > 65535 || Wx < 0 || lmax < Wx + l - 1 || chunk_windows < 1)
+        return fail(-1, "gkm_lmer_explain: need 0 <= B <= 65535, lmax >= Wx + l - 1 and "
+                        "chunk_windows >= 1");
+    if (lw == nullptr || (normalize && dx == nullptr))
+        return fail(-1, "gkm_lmer_explain: needs the l-mer weights (and dx when normalising)");
+    if (B == 0 || Wx == 0 || n_masks <= 0)
+        return 0;
+    IndexExplainArgs a = {};
+    a.xwin = xwin;
+    a.xlen = xlen;
+    a.masks = masks;
+    a.levels = levels;
+    a.wtab = wtab;
+    a.ta = ta;
+    a.tb = tb;
+    a.lw = lw;
+    a.dx = dx;
+    a.normalize = normalize != 0;
+    a.out = out;
+    a.B = B;
+    a.Wx = Wx;
+    a.lmax = lmax;
+    a.n_masks = n_masks;
+    a.l = l;
+    a.chunk_windows = chunk_windows;
+    a.pert = perturbation != 0;
+    const int lm = (l == 10 || l == 11 || l == 12) ? l : 15;
+    size_t lds = (4 * ((size_t)chunk_windows + lm - 1) + 32) * sizeof(double);
+    if (lds > 64 * 1024)
+        return fail(-2, "gkm_lmer_explain: chunk_windows %d too large", chunk_windows);
+    const int64_t n_pairs = (int64_t)((Wx + chunk_windows - 1) / chunk_windows) * B;
+    if (n_pairs > ((int64_t)1 << 30))
+        return fail(-1, "gkm_lmer_explain: %lld blocks exceed the launch limit",
+                    (long long)n_pairs);
+    dim3 grid((unsigned)(8 * ((n_pairs + 7) / 8)));
+    typedef void (*ixexp_fn)(IndexExplainArgs);
+    const ixexp_fn fn = l == 10 ? k_index_explain<10, 4> : l == 11 ? k_index_explain<11, 4>
+                      : l == 12 ? k_index_explain<12, 4> : k_index_explain<0, 4>;
+    hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), lds, S_(stream), a);
+    GKM_CHECK_LAUNCH("gkm_lmer_explain");
     return 0;
 }
 

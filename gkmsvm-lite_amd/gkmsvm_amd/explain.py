@@ -61,11 +61,24 @@ SWEEP_COMPARISONS_PER_S = 5e12
 INDEX_EXPLAIN_PROBES_PER_S = 3.5e10   # (window, mask, SV tile) records read (l-mer records: per probe)
 INDEX_EXPLAIN_ENTRIES_PER_S = 2.0e11
 
-# Per-SV explain kernel: "auto" (the l-mer index when the cost model favours
-# it and the model allows it, else the sweep), "index" (gkm_index_explain:
-# every query window visits its neighbourhood in the SVs' l-mer index) or
-# "sweep" (gkm_explain: lanes = 64 diagonals of one query x SV pair).  Both
-# compute the same attributions.
+# gkm_lmer_explain (the SVs' l-mer weights V): (window, mask) probes per second
+LMER_EXPLAIN_PROBES_PER_S = 1.5e11
+LMER_PROBE_BYTES = 13.0  # its algorithmic bytes: the mask, its level and V[v]
+# The l-mer path's limits (kernels/lmer_index.py:48-58, explain/lmer.py:21-29):
+# V is taken up to l = 12 and, past LMER_MAX_NEIGHBORS neighbours a window,
+# only where the SVs hold LMER_SV_WINDOWS_PER_NEIGHBOR windows per neighbour
+LMER_MAX_L = 12
+LMER_MAX_NEIGHBORS = 8192
+LMER_SV_WINDOWS_PER_NEIGHBOR = 6
+
+# Explain kernel: "auto" -- for a kernel without RBF, the SVs' l-mer weights
+# where the l-mer path applies (explain/lmer.py:31-150: V, summed over the SVs
+# once and cached), else the l-mer index when the cost model favours it and
+# the model allows it, else the sweep --, "lmer" (gkm_lmer_explain: every
+# query window visits its neighbourhood in V; no RBF), "index"
+# (gkm_index_explain: the same neighbourhood in the SVs' l-mer index, per-SV
+# coefficients) or "sweep" (gkm_explain: lanes = 64 diagonals of one query x
+# SV pair).  All compute the same attributions.
 EXPLAIN_KERNEL = "auto"
 
 
@@ -133,20 +146,39 @@ def _explain_rows(model, codes, lens, resolved, out):
     st = E.stream_handle()
     dx = E.self_kraw(kp, xs) if kp.normalize else torch.ones(B, dtype=torch.float64,
                                                             device=kp.device)
+    ta_d = torch.from_numpy(ta).to(kp.device)
+    tb_d = torch.from_numpy(tb).to(kp.device)
+    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
+    kernel = _explain_kernel(model, sv, kp, xs, ta, tb, cmax)
+    if kernel == "lmer":
+        if "V" not in sv.tables:
+            from . import tables as T
+
+            sv.tables["V"] = T.lmer_weights(sv)
+        masks, levels = _explain_masks(sv, ta, tb, cmax, pert)
+        fwd, _ = xs.windows(kp.l)
+        Wx = fwd.shape[1]
+        wq, _ = kp.wtab(xs.lmax)
+        chunk = max(4, min(64, -(-Wx * B // 1024)))
+        probes = float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks)
+        E._timed("k_lmer_explain", lambda: probes * LMER_PROBE_BYTES, None,
+                 lambda: _lib.call(
+                     "gkm_lmer_explain", int(pert), E._p(fwd), E._p(xs.len), B, Wx, xs.lmax,
+                     E._p(sv.tables["V"]), E._p(dx) if kp.normalize else None,
+                     int(kp.normalize), E._p(masks), E._p(levels), len(masks), kp.l, E._p(wq),
+                     chunk, E._p(ta_d), E._p(tb_d), E._p(buf), st))
+        out += buf
+        return
     hist = None
     if kp.rbf:
         hist = E.sv_hist(sv, xs, 0, B, use_index=model.use_lmer_index)
     ec = torch.empty((B, sv.S), dtype=torch.float64, device=kp.device)
     _lib.call("gkm_explain_coef", E._p(hist), B, sv.S, kp.nm, E._p(kp.tab), E._p(dx),
               E._p(sv.ds), E._p(sv.coef), int(kp.normalize), int(kp.rbf), kp.gamma, E._p(ec), st)
-    ta_d = torch.from_numpy(ta).to(kp.device)
-    tb_d = torch.from_numpy(tb).to(kp.device)
     nm = cmax + 1
     wt, wn = kp.wtab(max(xs.lmax, sv.seqs.lmax))
-    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
     norient = 2 if kp.rc else 1
     units = lambda: E.comparisons(kp, xs.lengths, sv.seqs.lengths, norient)  # noqa: E731
-    kernel = _explain_kernel(model, sv, kp, xs, ta, tb, cmax)
     if kernel == "index":
         idx = sv.index()
         masks, levels = _explain_masks(sv, ta, tb, cmax, pert)
@@ -217,12 +249,37 @@ def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
     return index_s, pair_s
 
 
+def _lmer_applies(model, sv, kp, ta, tb, cmax) -> bool:
+    """Whether the l-mer weights path takes these rows (explain/lmer.py:61-79):
+    no RBF, the l-mer index allowed, l <= LMER_MAX_L, and a neighbourhood of at
+    most LMER_MAX_NEIGHBORS l-mers or SVs with enough windows per neighbour."""
+    if kp.rbf or model.use_lmer_index is False or kp.l > LMER_MAX_L or cmax < 0:
+        return False
+    n = _n_masks(kp, ta, tb, cmax)
+    if n <= LMER_MAX_NEIGHBORS:
+        return True
+    norient = 2 if kp.rc else 1
+    sv_windows = norient * float(np.clip(sv.seqs.lengths - kp.l + 1, 0, None).sum())
+    return n * LMER_SV_WINDOWS_PER_NEIGHBOR <= sv_windows
+
+
+def _n_masks(kp, ta, tb, cmax) -> int:
+    from math import comb
+
+    weight = np.abs(np.asarray(ta)) + np.abs(np.asarray(tb))
+    return sum(comb(kp.l, m) * 3**m for m in range(cmax + 1) if weight[m] != 0)
+
+
 def per_sv_seconds(model, x_lengths, perturbation: bool) -> float:
-    """Estimated GPU seconds of the cheaper per-SV explain kernel for rows of
-    these lengths (what an explain table has to beat)."""
+    """Estimated GPU seconds of the cheapest explain kernel without the
+    explain table for rows of these lengths (what that table has to beat)."""
     kp = model._params()
+    sv = model._svs()
     ta, tb, cmax = explain_tables(kp.base, perturbation)
-    index_s, pair_s = _path_seconds(model._svs(), kp, x_lengths, ta, tb, cmax)
+    if _lmer_applies(model, sv, kp, ta, tb, cmax):
+        wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
+        return wx * _n_masks(kp, ta, tb, cmax) / LMER_EXPLAIN_PROBES_PER_S
+    index_s, pair_s = _path_seconds(sv, kp, x_lengths, ta, tb, cmax)
     if index_s is None or model.use_lmer_index is False:
         return pair_s
     return index_s if model.use_lmer_index is True else min(index_s, pair_s)
@@ -234,6 +291,8 @@ def _explain_kernel(model, sv, kp, xs, ta, tb, cmax) -> str:
     model.use_lmer_index = False keeps the sweep, True the index)."""
     if EXPLAIN_KERNEL != "auto":
         return EXPLAIN_KERNEL
+    if _lmer_applies(model, sv, kp, ta, tb, cmax):
+        return "lmer"
     use = model.use_lmer_index
     if use is False or kp.l > E.INDEX_MAX_L or cmax < 0:
         return "sweep"

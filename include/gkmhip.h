@@ -202,6 +202,21 @@ int gkm_index_explain(int perturbation, const uint32_t *xwin, const int32_t *xle
                       const double *ta, const double *tb, const double *ec, double *out,
                       void *stream);
 
+/* GkmExplain of a kernel without RBF through the SVs' l-mer weights, ADDED
+ * into out [B][4][lmax]: lw [4^l] = V, V[v] = sum over SVs of
+ * coef / sqrt(K_raw(sv, sv)) times the SV windows (forward and RC) equal to
+ * v, each with its window weight (gkm_lmer_weights); every query window u
+ * visits v = u ^ masks[t] as gkm_index_explain does, with s = V[v], and each
+ * row is scaled by 1 / sqrt(max(dx[b], 1e-20)) when normalize (dx [B]
+ * K_raw(x, x)).  l <= 13, B <= 65535.  _explain_lmer_cpu, _distribute
+ * (explain/cpu.py:68-143); _explain_lmer, _explain_V (explain/lmer.py:31-150,
+ * explain/common.py:87-98). */
+int gkm_lmer_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen, int B, int Wx,
+                     int lmax, const double *lw, const double *dx, int normalize,
+                     const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                     const uint8_t *wtab, int chunk_windows, const double *ta, const double *tb,
+                     double *out, void *stream);
+
 /* The same histograms through an l-mer index of the SV windows, ADDED into
  * hist [B][S][nm]: xwin [B][Wx] packed query windows (gkm_pack_windows), in any
  * order within a row (PAD last), with xpos [B][Wx] their positions in the row

@@ -235,10 +235,11 @@ def test_gkmexplain_fixtures(gpu, name, mode, lm):
 
 @pytest.mark.parametrize("seed", range(6))
 def test_explain_kernels_agree_with_oracle(gpu, seed):
-    """Both per-SV explain kernels (the l-mer index; lanes = diagonals of one
-    pair) against the oracle's per-SV loop on random models: l from 3 to 13
-    (the index to 12), all non-RBF and RBF types, weights, RC on/off, ragged
-    SVs and queries (some shorter than l), long queries."""
+    """The explain kernels (the SVs' l-mer weights, kernels without RBF; the
+    l-mer index; lanes = diagonals of one pair) against the oracle's per-SV
+    loop on random models: l from 3 to 13 (the l-mer paths to 12), all
+    non-RBF and RBF types, weights, RC on/off, ragged SVs and queries (some
+    shorter than l), long queries."""
     from gkmsvm_amd import explain as X
     from gkmsvm_amd import gkmexplain
 
@@ -255,8 +256,10 @@ def test_explain_kernels_agree_with_oracle(gpu, seed):
     om = _oracle_model(m)
     for mode in ("hypothetical", "perturbation"):
         want = O.explain(om, xc, xl, mode)
-        for kernel in ("index", "sweep"):
-            if kernel == "index" and l > 12:
+        for kernel in ("lmer", "index", "sweep"):
+            if kernel in ("lmer", "index") and l > 12:
+                continue
+            if kernel == "lmer" and m._params().rbf:
                 continue
             X.EXPLAIN_KERNEL = kernel
             try:
@@ -268,10 +271,11 @@ def test_explain_kernels_agree_with_oracle(gpu, seed):
 
 @pytest.mark.parametrize("kt", [2, 3, 5])
 def test_index_explain_with_repeats(gpu, kt):
-    """GkmExplain through the index on SVs with poly-A and dinucleotide runs
-    (l-mer buckets of thousands of entries: saturated tile counts in the
-    l-mer records, long runs past the record; weighted: the per-key records)
-    equals the oracle's per-SV loop, hypothetical and perturbation."""
+    """GkmExplain through the index (and, without RBF, the SVs' l-mer
+    weights) on SVs with poly-A and dinucleotide runs (l-mer buckets of
+    thousands of entries: saturated tile counts in the l-mer records, long
+    runs past the record; weighted: the per-key records) equals the oracle's
+    per-SV loop, hypothetical and perturbation."""
     from gkmsvm_amd import explain as X
     from gkmsvm_amd import gkmexplain
 
@@ -293,14 +297,16 @@ def test_index_explain_with_repeats(gpu, kt):
     xc[1, :] = np.tile([0, 2], 200)
     xl = np.array([400, 400, 350])
     om = _oracle_model(m)
+    kernels = ("index",) if m._params().rbf else ("index", "lmer")
     for mode in ("hypothetical", "perturbation"):
         want = O.explain(om, xc, xl, mode)
-        X.EXPLAIN_KERNEL = "index"
-        try:
-            got = gkmexplain(m, xc, mode=mode, lengths=xl).cpu().numpy()
-        finally:
-            X.EXPLAIN_KERNEL = "auto"
-        H.assert_close(got, want, 1e-10, f"index {mode} t={kt}")
+        for kernel in kernels:
+            X.EXPLAIN_KERNEL = kernel
+            try:
+                got = gkmexplain(m, xc, mode=mode, lengths=xl).cpu().numpy()
+            finally:
+                X.EXPLAIN_KERNEL = "auto"
+            H.assert_close(got, want, 1e-10, f"{kernel} {mode} t={kt}")
 
 
 @pytest.mark.parametrize("name", sorted(k for k in H.EXPLAIN_SETS if not k.endswith("_51")))
