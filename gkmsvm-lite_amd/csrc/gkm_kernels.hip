@@ -1974,7 +1974,7 @@ struct IndexExplainArgs {
 // window weights) -- one load per probe, and the row's 1 / sqrt(K(x, x))
 // applied at the end (explain/cpu.py:89-143, explain/lmer.py:31-254).
 template <int LT, int RK>
-__global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArgs a)
+__device__ __forceinline__ void index_explain_block(const IndexExplainArgs &a)
 {
     constexpr int LM = LT > 0 ? LT : 15;
     extern __shared__ __align__(16) uint8_t smem[];
@@ -2210,6 +2210,19 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArg
         if (r != 0.0)
             atomicAdd(&ob[(int64_t)v * a.lmax + p], r);
     }
+}
+
+template <int LT, int RK>
+__global__ __launch_bounds__(IXEXP_THREADS) void k_index_explain(IndexExplainArgs a)
+{
+    index_explain_block<LT, RK>(a);
+}
+
+// the l-mer weights form (RK 4), under its own name
+template <int LT>
+__global__ __launch_bounds__(IXEXP_THREADS) void k_lmer_explain(IndexExplainArgs a)
+{
+    index_explain_block<LT, 4>(a);
 }
 
 // hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m], over the
@@ -3614,8 +3627,8 @@ GKM_API int gkm_lmer_explain(int perturbation, const uint32_t *xwin, const int32
                     (long long)n_pairs);
     dim3 grid((unsigned)(8 * ((n_pairs + 7) / 8)));
     typedef void (*ixexp_fn)(IndexExplainArgs);
-    const ixexp_fn fn = l == 10 ? k_index_explain<10, 4> : l == 11 ? k_index_explain<11, 4>
-                      : l == 12 ? k_index_explain<12, 4> : k_index_explain<0, 4>;
+    const ixexp_fn fn = l == 10 ? k_lmer_explain<10> : l == 11 ? k_lmer_explain<11>
+                      : l == 12 ? k_lmer_explain<12> : k_lmer_explain<0>;
     hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_lmer_explain");
     return 0;
