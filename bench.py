@@ -13,8 +13,9 @@ query x SV kernel evaluations per second over the timed steps (a scoring row
 counts S evaluations, an SNV pair 2 S, an explain call S) with the model's
 l-mer tables off: scoring and variant rows go through the pairwise kernel or
 the l-mer index (whichever the cost model picks; both give the same integer
-histograms), explain rows through the explain sweep -- every query x SV pair
-is evaluated.  p50/p99 are request latencies from arrival to result.  Beside
+histograms) -- every query x SV pair is evaluated --, explain rows through
+the SVs' l-mer weights V (the sum over SVs that the reference's explain takes
+for a kernel without RBF, explain/lmer.py:31-150, built once per model).  p50/p99 are request latencies from arrival to result.  Beside
 it: the plain path (the same requests, one API call each: model(x),
 model.score_variants, gkmexplain) on the same kernels, and the same stream
 through the exact l-mer tables the server builds at start for a non-RBF
@@ -686,9 +687,10 @@ def main():
         step_ms = {"min": float(ms.min()), "p50": float(np.median(ms)), "max": float(ms.max())}
         return el, timer.summary(), lat_sat, step_ms
 
-    # 1. The headline: the stream served through the per-SV kernels (pairwise
-    # kernel / l-mer index for scoring, the explain sweep), tables off -- the
-    # path that evaluates every query x SV pair.
+    # 1. The headline: the stream served with the l-mer tables off -- scoring
+    # and variant rows through the per-SV kernels (pairwise kernel / l-mer
+    # index: every query x SV pair evaluated), explain rows through the SVs'
+    # l-mer weights V.
     model_k = make_model()[0]
     model_k.use_lmer_table = False
     model_k.use_explain_table = False
@@ -773,7 +775,8 @@ def main():
         "config": {
             "workload": "serving: 72,000-SV -t 2 l=11 k=7 d=3 model of 500 bp SVs resident in "
                         "HBM; Poisson stream mixing scoring (50-2000 bp), SNV pairs (300 bp) "
-                        "and gkmexplain (1 kb); served through the per-SV kernels",
+                        "and gkmexplain (1 kb); scoring and variant rows served through the "
+                        "per-SV kernels, explain rows through the SVs' l-mer weights",
             "requests_per_step": args.requests,
             "arrival_rate_per_s": args.rate,
             "step_wall_ms": step_ms,
