@@ -206,8 +206,8 @@ def test_server_first_requests_see_finished_tables(gpu, tables):
     """Requests submitted right after GkmServer(...).start(), with no sync of
     the caller's stream in between, read the tables and caches the
     constructor built on that stream (l-mer and explain tables, or the SVs'
-    l-mer index and explain neighbourhoods) whole: perturbation explain and
-    scoring equal the same calls made afterwards."""
+    l-mer weights, l-mer index and explain neighbourhoods) whole:
+    perturbation explain and scoring equal the same calls made afterwards."""
     from gkmsvm_amd import gkmexplain
     from gkmsvm_amd.serving import GkmServer
 
@@ -218,6 +218,8 @@ def test_server_first_requests_see_finished_tables(gpu, tables):
     qs = _queries(8, 200, 900, 21)
     ex = _queries(4, 300, 600, 22)
     srv = GkmServer(m, hbm_budget=2 << 30, tables=tables).start()
+    if not tables:
+        assert "V" in m._svs().tables  # built by the constructor, not by a step
     f_ex = [srv.submit("explain", [q], mode="perturbation") for q in ex]
     f_sc = [srv.submit("score", [q]) for q in qs]
     got_ex = [f.result(timeout=120) for f in f_ex]
