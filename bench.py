@@ -597,6 +597,70 @@ def config_runs(args):
           flush=True)
 
 
+# The reference's published prediction throughput at 72,145 SVs, 1,000 x 300 bp
+# queries, seq/s (benchmarks/results/throughput.tsv:50,54 on an NVIDIA L40S,
+# benchmarks/results/a100/throughput.tsv on an A100-SXM4-80GB, and the 16-thread
+# EPYC 9384X rows of the first file)
+REF_THROUGHPUT = {
+    "t2": {"path": "l-mer table", "L40S": 45682.5, "A100": 35160.3, "EPYC 16 threads": 55960.3},
+    "t3": {"path": "l-mer index", "L40S": 2202.5, "A100": 2260.9, "EPYC 16 threads": 114.7},
+}
+
+
+def ref_benchmark(args):
+    """The reference's own throughput benchmark (benchmarks/throughput_sweep.py)
+    at its largest size: the ENCODE ENCFF579AOX model's 72,145 SVs (l=11 k=7
+    d=3; tests/golden/lsgkm, the reference's fixture), scored as -t 2
+    (gkm_esttrunc, the file's kernel) and -t 3 (gkmrbf, gamma 1, the same SVs
+    and coefficients), 1,000 queries x 300 bp; model(x) timed on its second
+    call after a warm call, as the reference times it.  The reference's
+    queries are GM12878 DHS sequences, which this repository does not hold:
+    here uniform iid ACGT, seeded.  -t 2 is timed through the l-mer table (the
+    reference's default for it) and through the per-SV index path."""
+    import torch
+
+    from gkmsvm_amd import GkmSVM, load_lsgkm_model
+
+    path = os.path.join(ROOT, "tests", "golden", "lsgkm", "encode_ENCFF579AOX.model.txt.gz")
+    full = load_lsgkm_model(path)
+    q = np.random.default_rng(11).integers(0, 4, (1000, 300)).astype(np.uint8)
+    S = len(full.coefficients)
+
+    def run(m, label, kernel, path_name):
+        t0 = time.perf_counter()
+        m(q).cpu()
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            m(q).cpu()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        steady = float(np.median(times))
+        ref = REF_THROUGHPUT[kernel]
+        print(json.dumps({"config": f"reference throughput benchmark: ENCFF579AOX {S:,} SVs, "
+                                    f"-{kernel[0]} {kernel[1]} ({label}), 1,000 x 300 bp queries",
+                          "path": path_name, "first_call_s": first, "steady_s": steady,
+                          "seq_per_s": 1000 / steady, "pairs_per_s": 1000 * S / steady,
+                          "reference_seq_per_s": {k: v for k, v in ref.items() if k != "path"},
+                          "reference_path": ref["path"],
+                          "x_L40S": 1000 / steady / ref["L40S"]}), flush=True)
+
+    t2 = full
+    t2.rocm()
+    t2.use_lmer_table = True
+    run(t2, "gkm_esttrunc", "t2", "l-mer table")
+    t2.use_lmer_table = False
+    run(t2, "gkm_esttrunc", "t2", "per-SV (l-mer index or pairwise, by the cost model)")
+    del t2
+    torch.cuda.empty_cache()
+    t3 = GkmSVM(full.support_sequences, full.coefficients, full.bias, "gkmrbf",
+                {**full.kernel_params, "gamma": 1.0}, sv_lengths=full.sv_lengths)
+    t3.rocm()
+    run(t3, "gkmrbf gamma 1", "t3", "per-SV (l-mer index or pairwise, by the cost model)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -619,12 +683,13 @@ def main():
                     help="skip the l-mer table runs (profiles of the per-SV kernels)")
     ap.add_argument("--mode", default="serve",
                     choices=["serve", "cross", "index", "score", "explain", "table", "copy",
-                             "peaks", "ism", "configs"],
+                             "peaks", "ism", "configs", "refbench"],
                     help="serve: the benchmark; cross/index/score/explain/table/copy: one "
                          "kernel family on the serving shape, for profiling (score: one "
                          "step's scoring rows in the persistent index launch; explain: 10 "
                          "x 1 kb explain rows); ism: the -t 5 ISM sample, for "
-                         "profiling; configs: bounded timings of the other configurations")
+                         "profiling; configs: bounded timings of the other configurations; "
+                         "refbench: the reference's own throughput benchmark shape")
     ap.add_argument("--c3-queries", type=int, default=100_000)
     ap.add_argument("--c4-queries", type=int, default=8)
     ap.add_argument("--launches", type=int, default=5)
@@ -638,6 +703,8 @@ def main():
 
     if args.mode == "configs":
         return config_runs(args)
+    if args.mode == "refbench":
+        return ref_benchmark(args)
     if args.mode != "serve":
         return kernel_only(args)
 
