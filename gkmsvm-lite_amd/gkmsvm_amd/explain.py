@@ -146,8 +146,7 @@ def _explain_rows(model, codes, lens, resolved, out):
     st = E.stream_handle()
     dx = E.self_kraw(kp, xs) if kp.normalize else torch.ones(B, dtype=torch.float64,
                                                             device=kp.device)
-    ta_d = torch.from_numpy(ta).to(kp.device)
-    tb_d = torch.from_numpy(tb).to(kp.device)
+    ta_d, tb_d = _explain_ab(sv, ta, tb, pert)
     buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
     kernel = _explain_kernel(model, sv, kp, xs, ta, tb, cmax)
     if kernel == "lmer":
@@ -202,6 +201,16 @@ def _explain_rows(model, codes, lens, resolved, out):
             E._p(sv.seqs.len), sv.S, sv.seqs.nw, kp.l, nm, cmax, E._p(wt), wn,
             _sv_chunk(B, sv.S), E._p(ta_d), E._p(tb_d), E._p(ec), E._p(buf), st))
     out += buf
+
+
+def _explain_ab(sv, ta, tb, pert):
+    """The explain tables ta, tb of this mode on the device (cached: a copy
+    from pageable host memory would wait for the work queued on the stream)."""
+    key = ("explain_ab", bool(pert))
+    if key not in sv.tables:
+        sv.tables[key] = (torch.from_numpy(np.ascontiguousarray(ta)).to(sv.kp.device),
+                          torch.from_numpy(np.ascontiguousarray(tb)).to(sv.kp.device))
+    return sv.tables[key]
 
 
 def _explain_masks(sv, ta, tb, cmax, pert):

@@ -229,13 +229,19 @@ class GkmServer:
     def _prepare(self, stage_len: int):
         """Build the device state that steps would otherwise build lazily on
         a server stream, where another stream could read it half-written: the
-        window-weight table, the SVs' l-mer weights V (explain without RBF),
-        the SVs' l-mer index and the explain neighbourhoods."""
-        from .explain import _explain_masks, _lmer_applies, explain_tables
+        window-weight table, the explain weight tables, the SVs' l-mer
+        weights V (explain without RBF), the SVs' l-mer index and the explain
+        neighbourhoods."""
+        from .explain import _explain_ab, _explain_masks, _lmer_applies, explain_tables
 
         kp, sv = self.kp, self.sv
         kp.wtab(stage_len)
-        if self.table is not None or self.model.use_lmer_index is False:
+        if self.table is not None:
+            return
+        for pert in (False, True):
+            ta, tb, cmax = explain_tables(kp.base, pert)
+            _explain_ab(sv, ta, tb, pert)
+        if self.model.use_lmer_index is False:
             return
         for pert in (False, True):
             ta, tb, cmax = explain_tables(kp.base, pert)
