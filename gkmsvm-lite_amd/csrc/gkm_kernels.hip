@@ -2738,6 +2738,43 @@ GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, con
     return 0;
 }
 
+GKM_API int gkm_index_hist_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
+                                 int B, int Wx, const uint32_t *offsets, const uint16_t *ids,
+                                 const uint8_t *vals, const uint32_t *records, int record_tables,
+                                 const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                                 int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                                 int chunk_windows, int pack_counts, int n_pairs, int n_blocks,
+                                 void *workspace, void *partial, void *hist, void *stream)
+{
+    if (workspace == nullptr || partial == nullptr)
+        return fail(-1, "gkm_index_hist_queue: workspace and partial are required");
+    if ((uintptr_t)workspace % 256)
+        return fail(-1, "gkm_index_hist_queue: workspace must be 256-byte aligned");
+    if (n_blocks < 1 || n_blocks > 65536)
+        return fail(-1, "gkm_index_hist_queue: n_blocks must be 1..65536, got %d", n_blocks);
+    if (chunk_windows < 1)
+        return fail(-1, "gkm_index_hist_queue: chunk_windows must be >= 1");
+    if (B <= 0 || Wx <= 0 || S <= 0 || n_masks <= 0)
+        return 0;
+    // workspace: 8 queue counters | first[B] (the level-sum part is not used)
+    int *queue = (int *)workspace;
+    int32_t *pair_first = (int32_t *)((uint8_t *)workspace + 256);
+    const int nchunks = (Wx + chunk_windows - 1) / chunk_windows;
+    hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), xlen, B, l,
+                       chunk_windows, nchunks, queue, pair_first);
+    GKM_CHECK_LAUNCH("gkm_index_hist_queue");
+    int count_bytes = 4;
+    int rc = launch_index_tile("gkm_index_hist_queue", xwin, xpos, xlen, B, Wx, offsets, ids,
+                               vals, records, record_tables, masks, levels, n_masks, l, nm, wtab,
+                               S, tile, n_tiles, chunk_windows, pack_counts, -1, 0, partial,
+                               nullptr, &count_bytes, stream, pair_first, n_pairs, queue,
+                               n_blocks);
+    if (rc || hist == nullptr)
+        return rc;
+    return gkm_index_reduce(partial, count_bytes, xlen, B, Wx, l, nm, S, tile, chunk_windows,
+                            hist, stream);
+}
+
 // ---------------------------------------------------------------------------
 // ISM through the l-mer index
 // ---------------------------------------------------------------------------

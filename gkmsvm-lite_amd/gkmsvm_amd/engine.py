@@ -593,10 +593,14 @@ def index_hist(kp: KernelParams, idx: DeviceIndex, xs: DeviceSeqs, x0: int, x1: 
     min_cw = 16 if not idx.pack16 else max(1, min(16, 65535 // max(1, idx.max_sv_windows)))
     per_row = -(-Wx // min_cw) * S * kp.nm * (2 if idx.pack16 else 4)
     step = max(1, min(B, MAX_PARTIAL_BYTES // max(1, per_row), 65535))
+    queue = col_min < 0 and target_blocks is None
     for r0 in range(0, B, step):
         r1 = min(B, r0 + step)
-        _index_launch(kp, idx, xs, x0 + r0, x0 + r1, S, col_min, out[r0:r1], stream,
-                      target_blocks)
+        if queue:  # one persistent launch per row group (the Gram's triangle: a grid)
+            _index_launch_queue(kp, idx, xs, x0 + r0, x0 + r1, S, out[r0:r1], stream)
+        else:
+            _index_launch(kp, idx, xs, x0 + r0, x0 + r1, S, col_min, out[r0:r1], stream,
+                          target_blocks)
     return out
 
 
@@ -684,6 +688,24 @@ def _index_launch(kp, idx, xs, x0, x1, S, col_min, out, stream, target_blocks):
                              stream_handle(stream)))
     _lib.call("gkm_index_reduce", _p(partial), L.count_bytes, _p(xs.len[x0:x1]),
               L.B, L.Wx, kp.l, kp.nm, S, idx.tile, L.cw, _p(out), stream_handle(stream))
+    return out
+
+
+def _index_launch_queue(kp, idx, xs, x0, x1, S, out, stream):
+    """Rows x0..x1's histograms added into out through one persistent launch
+    (gkm_index_hist_queue): chunks of INDEX_CHUNK_UNIFORM windows for rows of
+    one length, INDEX_CHUNK_RAGGED otherwise."""
+    wx = np.clip(xs.lengths[x0:x1] - kp.l + 1, 0, None)
+    ragged = int(wx.max(initial=0)) > 1.5 * max(1.0, float(wx.mean()) if len(wx) else 1.0)
+    chunk = INDEX_CHUNK_RAGGED if ragged else INDEX_CHUNK_UNIFORM
+    L = IndexLaunch(kp, idx, xs, x0, x1, S, -1, stream, None, chunk=chunk)
+    n_pairs = int(np.minimum(-(-wx // L.cw), L.nchunks).sum())
+    ws = torch.empty(_lib.load().gkm_index_queue_bytes(L.B, 0, idx.n_tiles, kp.nm),
+                     dtype=torch.uint8, device=kp.device)
+    partial = torch.empty(L.partial_bytes(kp, idx, S), device=kp.device, dtype=torch.uint8)
+    _timed("k_index_tile_queue", lambda: index_bytes(kp, idx, xs.lengths[x0:x1]), stream,
+           lambda: _lib.call("gkm_index_hist_queue", *L.args[:-2], n_pairs, n_cus(kp.device),
+                             _p(ws), _p(partial), _p(out), stream_handle(stream)))
     return out
 
 

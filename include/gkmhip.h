@@ -331,6 +331,21 @@ int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32
                           const double *tab, const double *dx, int shift, int normalize,
                           double bias, double *out, void *stream);
 
+/* gkm_index_hist as one persistent launch (the work list and queues of
+ * gkm_index_score_queue): the same per-chunk partial counts in `partial`
+ * (layout and size of gkm_index_hist's), then, hist non-NULL, ADDED into hist
+ * [B][S][nm] -- the same integers.  workspace: gkm_index_queue_bytes(B, 0,
+ * n_tiles, nm) bytes, 256-byte aligned.  For what the per-SV histograms feed
+ * (RBF kernels, the explain coefficients); kernels/lmer_index.py:239-268
+ * (LmerIndex.pairwise). */
+int gkm_index_hist_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen, int B,
+                         int Wx, const uint32_t *offsets, const uint16_t *ids, const uint8_t *vals,
+                         const uint32_t *records, int record_tables, const uint32_t *masks,
+                         const uint8_t *levels, int n_masks, int l, int nm, const uint8_t *wtab,
+                         int S, int tile, int n_tiles, int chunk_windows, int pack_counts,
+                         int n_pairs, int n_blocks, void *workspace, void *partial, void *hist,
+                         void *stream);
+
 /* ---- No-allocation variant --------------------------------------------------
  * No counterpart in the reference: for callers that cannot allocate device
  * memory or pass a workspace (code under HIP graph capture, tight serving
