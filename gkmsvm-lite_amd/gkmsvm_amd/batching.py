@@ -220,7 +220,7 @@ class BatchScorer:
     # -- the persistent launch -------------------------------------------------------
 
     QUEUE_ROWS = 1 << 17   # rows per persistent launch (device memory of one launch)
-    QUEUE_CHUNKS = (16, 32, 64)
+    QUEUE_CHUNKS = (16, 24, 32, 64)
     QUEUE_TUNE_ROWS = 1024
 
     def _use_queue(self, lens) -> bool:
