@@ -8,8 +8,14 @@ score_variants) and GkmExplain calls (1 kb, hypothetical), admitted by an
 HBM budget.
 
 A step is one burst of --requests requests served through the continuous-
-batching server (gkmsvm_amd.serving.GkmServer) as they arrive.  value is
-query x SV kernel evaluations per second over the timed steps (a scoring row
+batching server (gkmsvm_amd.serving.GkmServer) as they arrive; up to two
+bursts are outstanding (a burst arrives once the one before has been admitted
+into a step and the one before that has all its results), so the server
+admits and launches a step while the one before it is on the GPU: the server
+at saturation, one burst per step.  config.closed_loop times
+the same steps one burst at a time (each arriving after the last result of
+the one before), where the host's admission and delivery do not overlap the
+GPU.  value is query x SV kernel evaluations per second over the timed steps (a scoring row
 counts S evaluations, an SNV pair 2 S, an explain call S) with the model's
 l-mer tables off: scoring and variant rows go through the pairwise kernel or
 the l-mer index (whichever the cost model picks; both give the same integer
@@ -24,7 +30,9 @@ evaluated, and evaluations are counted as the per-SV path's only to compare
 rates), served and plain.
 
 roofline: every launch of the hot kernels is timed with HIP events on the
-stream it runs on during the timed steps; the object describes the kernel
+stream it runs on during the closed-loop pass over the timed steps (one step
+on the GPU at a time, so a launch's events hold only its own kernels); the
+object describes the kernel
 with the most GPU time in the value run (the others under "others"; the
 table run's under config.served_lmer_tables.roofline).  achieved and frac
 are measured work per launch over the live mean launch time: HBM bytes
@@ -124,14 +132,12 @@ def run_plain(model, reqs):
     torch.cuda.synchronize()
 
 
-def run_served(server, reqs, rate, rng):
+def submit_burst(server, reqs, rate, rng):
     """Submit reqs at Poisson arrival times (rate per second) from this (the
-    client's) thread while the server's worker runs; returns the wall time
-    from the first arrival to the last result."""
+    client's) thread while the server's threads run; returns their futures."""
     gaps = rng.exponential(1.0 / rate, len(reqs))
     futs = []
-    t0 = time.perf_counter()
-    due = t0
+    due = time.perf_counter()
     for (kind, payload), g in zip(reqs, gaps):
         due += g
         while True:
@@ -140,9 +146,46 @@ def run_served(server, reqs, rate, rng):
                 break
             time.sleep(min(due - now, 5e-4))
         futs.append(server.submit(kind, payload))
-    for f in futs:
+    return futs
+
+
+def run_served(server, reqs, rate, rng):
+    """One burst, closed loop: submit reqs and wait for every result; returns
+    the wall time from the first arrival to the last result."""
+    t0 = time.perf_counter()
+    for f in submit_burst(server, reqs, rate, rng):
         f.result()
     return time.perf_counter() - t0
+
+
+def run_served_pipelined(server, bursts, rate, rng, depth=2):
+    """The bursts with up to `depth` of them outstanding, each arriving once
+    the one before has been admitted into a step (so a step is one burst)
+    and, past `depth`, once the oldest outstanding one has all its results:
+    the next step is admitted and launched (on the server's other stream)
+    while one is on the GPU -- the server at saturation.  Returns (wall time
+    from the first arrival to the last result, the intervals between
+    consecutive bursts' last results)."""
+    from collections import deque
+
+    outstanding = deque()
+    done = []
+    t0 = time.perf_counter()
+    for reqs in bursts:
+        if len(outstanding) >= depth:
+            for f in outstanding.popleft():
+                f.result()
+            done.append(time.perf_counter())
+        futs = submit_burst(server, reqs, rate, rng)
+        while not (futs[-1].t_admit or futs[-1].done()):
+            time.sleep(2e-5)  # until the launcher has taken the burst
+        outstanding.append(futs)
+    while outstanding:
+        for f in outstanding.popleft():
+            f.result()
+        done.append(time.perf_counter())
+    marks = np.array([t0] + done)
+    return marks[-1] - t0, np.diff(marks)
 
 
 def host_cpu() -> dict:
@@ -605,6 +648,21 @@ REF_THROUGHPUT = {
     "t2": {"path": "l-mer table", "L40S": 45682.5, "A100": 35160.3, "EPYC 16 threads": 55960.3},
     "t3": {"path": "l-mer index", "L40S": 2202.5, "A100": 2260.9, "EPYC 16 threads": 114.7},
 }
+# The reference's GkmExplain / ISM throughput on the same model, 300 bp
+# sequences (benchmarks/results/explain_throughput.tsv:25,29,33,34,50,53,55,56
+# and a100/explain_throughput.tsv:50,53,55,56): 1,000 sequences, ISM also 100;
+# the GPU rows ran without the explain table (its "auto" builds it from 10,000
+# sequences on), the 16-thread CPU rows of the explain modes with it.
+REF_EXPLAIN = {
+    ("hypothetical", 1000): {"L40S": 2031.6, "A100": 2375.5, "EPYC 16 threads": 44100.2},
+    ("perturbation", 1000): {"L40S": 632.4, "A100": 1149.0, "EPYC 16 threads": 39170.2},
+    ("ISM", 1000): {"L40S": 2337.3, "A100": 1747.4, "EPYC 16 threads": 547.0},
+    ("ISM", 100): {"L40S": 1596.0, "A100": 1185.4, "EPYC 16 threads": 503.2},
+}
+# The reference's ISM above goes through its l-mer table (its planner builds it
+# for 100 sequences and up, svm.py:508-536); its per-SV (l-mer index) ISM is
+# published only for a -t 3 dsQTL model of 32,715 SVs (l=10) that this
+# repository does not hold: 8.3 seq/s on an L40S (explain_throughput.tsv:64).
 
 
 def ref_benchmark(args):
@@ -616,7 +674,12 @@ def ref_benchmark(args):
     call after a warm call, as the reference times it.  The reference's
     queries are GM12878 DHS sequences, which this repository does not hold:
     here uniform iid ACGT, seeded.  -t 2 is timed through the l-mer table (the
-    reference's default for it) and through the per-SV index path."""
+    reference's default for it) and through the per-SV index path.  Then the
+    reference's explain benchmark (benchmarks/explain_throughput.py) on the -t 2
+    model: hypothetical and perturbation GkmExplain of the 1,000 queries without
+    the explain table, ISM of them and of the first 100 through the l-mer table
+    (built by the scoring run before), and ISM of the first 100 through the
+    per-SV path (the l-mer index), which has no published counterpart here."""
     import torch
 
     from gkmsvm_amd import GkmSVM, load_lsgkm_model
@@ -626,18 +689,21 @@ def ref_benchmark(args):
     q = np.random.default_rng(11).integers(0, 4, (1000, 300)).astype(np.uint8)
     S = len(full.coefficients)
 
-    def run(m, label, kernel, path_name):
+    def timed_calls(fn):
         t0 = time.perf_counter()
-        m(q).cpu()
+        fn().cpu()
         torch.cuda.synchronize()
         first = time.perf_counter() - t0
         times = []
         for _ in range(3):
             t0 = time.perf_counter()
-            m(q).cpu()
+            fn().cpu()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        steady = float(np.median(times))
+        return first, float(np.median(times))
+
+    def run(m, label, kernel, path_name):
+        first, steady = timed_calls(lambda: m(q))
         ref = REF_THROUGHPUT[kernel]
         print(json.dumps({"config": f"reference throughput benchmark: ENCFF579AOX {S:,} SVs, "
                                     f"-{kernel[0]} {kernel[1]} ({label}), 1,000 x 300 bp queries",
@@ -647,12 +713,46 @@ def ref_benchmark(args):
                           "reference_path": ref["path"],
                           "x_L40S": 1000 / steady / ref["L40S"]}), flush=True)
 
+    def run_explain(m, op, n, per_sv=False):
+        """GkmExplain / ISM of the first n queries on the -t 2 model, with the
+        explain table off (as in the reference's GPU rows at this size)."""
+        from gkmsvm_amd import engine as E
+        from gkmsvm_amd import gkmexplain, ism
+
+        if op == "ISM":
+            fn = lambda: ism(m, q[:n])  # noqa: E731
+        else:
+            fn = lambda: gkmexplain(m, q[:n], mode=op)  # noqa: E731
+        first, steady = timed_calls(fn)
+        E.TIMER = E.KernelTimer()  # one more call: which hot kernels ran, and their time
+        try:
+            fn().cpu()
+            kernels = {k: round(v["total_ms"], 3) for k, v in E.TIMER.summary().items()}
+        finally:
+            E.TIMER = None
+        line = {"config": f"reference explain benchmark: ENCFF579AOX {S:,} SVs, -t 2, "
+                          f"{op}{' per-SV' if per_sv else ''}, {n:,} x 300 bp sequences",
+                "kernels_ms": kernels, "first_call_s": first, "steady_s": steady,
+                "seq_per_s": n / steady}
+        if not per_sv:
+            ref = REF_EXPLAIN[(op, n)]
+            line.update(reference_seq_per_s=ref, x_L40S=n / steady / ref["L40S"],
+                        x_A100=n / steady / ref["A100"])
+        print(json.dumps(line), flush=True)
+
     t2 = full
     t2.rocm()
     t2.use_lmer_table = True
     run(t2, "gkm_esttrunc", "t2", "l-mer table")
     t2.use_lmer_table = False
     run(t2, "gkm_esttrunc", "t2", "per-SV (l-mer index or pairwise, by the cost model)")
+    t2.use_lmer_table = "auto"
+    t2.use_explain_table = False
+    for op in ("hypothetical", "perturbation", "ISM"):
+        run_explain(t2, op, 1000)
+    run_explain(t2, "ISM", 100)
+    t2.use_lmer_table = False
+    run_explain(t2, "ISM", 100, per_sv=True)
     del t2
     torch.cuda.empty_cache()
     t3 = GkmSVM(full.support_sequences, full.coefficients, full.bias, "gkmrbf",
@@ -722,37 +822,48 @@ def main():
     steps = [make_requests(args.requests, 100 + i) for i in range(n_steps)]
     rng = np.random.default_rng(5)
 
-    def served_run(server, timed_steps, events_in_timed=True):
+    def wall_ms(seconds):
+        ms = np.asarray(seconds) * 1e3
+        return {"min": float(ms.min()), "p50": float(np.median(ms)), "max": float(ms.max())}
+
+    def served_run(server, timed_steps, events_in_closed=True):
         """(seconds over the timed steps, kernel summary, latency at saturation,
-        per-step wall ms).  events_in_timed: the kernels' HIP events are
-        recorded during the timed steps; else (the host-bound table path,
-        where they would add ~3 % to a step) during a second pass over the
-        same steps after them."""
-        for i in range(args.warmup):
-            run_served(server, steps[i], args.rate, rng)
+        per-step wall ms, the closed-loop figures).  The timed steps run with
+        two bursts outstanding (run_served_pipelined); then the same steps
+        closed loop, one burst at a time, with the kernels' HIP events
+        recorded there: one step on the GPU at a time, so a launch's events
+        hold only its own kernels.  events_in_closed False (the host-bound
+        table path, where events would add ~3 % to a step): the events come
+        from a second closed-loop pass."""
+        run_served_pipelined(server, steps[:args.warmup], args.rate, rng)
         server.completed.clear()
-        timer = E.KernelTimer()
-        E.TIMER = timer if events_in_timed else None
         torch.cuda.synchronize()
-        per_step = []
         t0 = time.perf_counter()
-        for s_ in timed_steps:
-            per_step.append(run_served(server, s_, args.rate, rng))
+        _, per_step = run_served_pipelined(server, timed_steps, args.rate, rng)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        E.TIMER = None
         lat_sat = latency_stats(server.completed)
         server.completed.clear()
-        if not events_in_timed:
+        timer = E.KernelTimer()
+        E.TIMER = timer if events_in_closed else None
+        closed = []
+        t0 = time.perf_counter()
+        for s_ in timed_steps:
+            closed.append(run_served(server, s_, args.rate, rng))
+        torch.cuda.synchronize()
+        el_closed = time.perf_counter() - t0
+        E.TIMER = None
+        closed = {"ms_per_step": el_closed / len(timed_steps) * 1e3, "step_wall_ms": wall_ms(closed),
+                  "latency_ms": latency_stats(server.completed)}
+        server.completed.clear()
+        if not events_in_closed:
             E.TIMER = timer
             for s_ in timed_steps:
                 run_served(server, s_, args.rate, rng)
             torch.cuda.synchronize()
             E.TIMER = None
             server.completed.clear()
-        ms = np.array(per_step) * 1e3
-        step_ms = {"min": float(ms.min()), "p50": float(np.median(ms)), "max": float(ms.max())}
-        return el, timer.summary(), lat_sat, step_ms
+        return el, timer.summary(), lat_sat, wall_ms(per_step), closed
 
     # 1. The headline: the stream served with the l-mer tables off -- scoring
     # and variant rows through the per-SV kernels (pairwise kernel / l-mer
@@ -766,8 +877,9 @@ def main():
     ktimed = steps[args.warmup:args.warmup + args.steps]
     k_evals = sum(evals(s_) for s_ in ktimed)
     server_k = GkmServer(model_k, hbm_budget=16 << 30, tables=False).start()
-    served_s, summary, lat_sat, step_ms = served_run(server_k, ktimed)
+    served_s, summary, lat_sat, step_ms, closed = served_run(server_k, ktimed)
     served_rate = k_evals / served_s
+    closed["value"] = k_evals / (closed["ms_per_step"] * 1e-3 * len(ktimed))
     # latency at a nominal load: arrivals at a fraction of the measured capacity
     cap = args.requests * len(ktimed) / served_s
     load_rate = args.load * cap
@@ -788,8 +900,9 @@ def main():
         server.start()
         ttimed = steps[args.warmup:args.warmup + args.table_steps]
         t_evals = sum(evals(s_) for s_ in ttimed)
-        served_t_s, summary_t, lat_t_sat, step_ms_t = served_run(server, ttimed,
-                                                                 events_in_timed=False)
+        served_t_s, summary_t, lat_t_sat, step_ms_t, closed_t = served_run(
+            server, ttimed, events_in_closed=False)
+        closed_t["value"] = t_evals / (closed_t["ms_per_step"] * 1e-3 * len(ttimed))
         cap_t = args.requests * len(ttimed) / served_t_s
         server.completed.clear()
         for s_ in steps[args.warmup:args.warmup + args.lat_steps]:
@@ -798,7 +911,7 @@ def main():
         server.stop()
         served_t = {"value": t_evals / served_t_s, "ms_per_step": served_t_s / len(ttimed) * 1e3,
                     "steps": len(ttimed), "step_wall_ms": step_ms_t, "table_build_s": build_s,
-                    "latency_ms_at_saturation": lat_t_sat,
+                    "latency_ms_at_saturation": lat_t_sat, "closed_loop": closed_t,
                     "latency_load_rate_per_s": args.load * cap_t, "latency_ms": lat_t,
                     "roofline": roofline_of(summary_t)}
 
@@ -848,6 +961,7 @@ def main():
             "arrival_rate_per_s": args.rate,
             "step_wall_ms": step_ms,
             "latency_ms_at_saturation": lat_sat,
+            "closed_loop": closed,
             "latency_load_rate_per_s": load_rate,
             "latency_ms": lat,
             "plain": plain,

@@ -8,12 +8,22 @@ requests into GPU steps while earlier steps run: each step packs every
 admitted scoring/variant row into one batch (rows of any length; their bytes
 go over in one pinned copy and are decoded on the device, where rows with a
 letter other than ACGTN are flagged to fail alone) and launches the index or pairwise
-kernel (or the l-mer table lookup) once for all of them on its own HIP
-stream; the step's explain requests follow as one batch per mode.  A
-harvester thread waits on each step's completion event and hands out its
-results.  Up to ``streams`` steps are in flight, so a request that arrives
-while others are on the GPU is admitted as soon as a stream frees up and its
-work overlaps theirs.  Admission is by memory: a step takes requests while the bytes it
+kernel (or the l-mer table lookup) once for all of them; the step's explain
+requests follow as one batch per mode.  A harvester thread waits on each
+step's completion event and hands out its results.  Up to ``streams`` steps
+are in flight, each with its own pinned buffers, so a request that arrives
+while others are on the GPU is admitted as soon as a slot frees up: the host
+decodes, stages and launches its step while the one before runs.  The steps
+share one HIP stream and so run on the device in admission order: the GPU
+goes from one step to the next without waiting on the host, and a step's
+short kernels after its index launch (explain rows, the reduction, the copies
+back) never queue behind the next step's persistent launch, which holds every
+CU.  (Steps on streams of their own ran side by side and finished together;
+ordering them with hipStreamWaitEvent blocked the launching thread until the
+event had completed.)  While a step is in flight the launcher lets arrivals
+join the next step for up to ``coalesce_s`` past the oldest one's (that step
+would wait behind the one in flight anyway), so a burst that arrives over a
+fraction of a millisecond becomes one step rather than several.  Admission is by memory: a step takes requests while the bytes it
 would hold (histograms, planes, outputs) stay within ``hbm_budget``; a
 request larger than the whole budget is rejected with MemoryError.
 
@@ -134,10 +144,20 @@ class Request:
         fn(self)
 
 
+class _Slot:
+    """A step in flight: the server's stream and this slot's pinned buffers."""
+
+    __slots__ = ("stream", "pool")
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.pool = {}
+
+
 @dataclass
 class _Step:
     reqs: list
-    stream: torch.cuda.Stream
+    slot: _Slot
     done: torch.cuda.Event
     score_host: torch.Tensor | None
     score_short: np.ndarray | None  # scoring rows shorter than l (scored NaN)
@@ -154,10 +174,12 @@ class GkmServer:
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
                  streams: int = 2, max_explain: int = 64, tables: bool = True,
-                 history: int = 1 << 16, stage_len: int = 2048, freeze_gc: bool = True):
+                 history: int = 1 << 16, stage_len: int = 2048, freeze_gc: bool = True,
+                 coalesce_s: float | None = None):
         """hbm_budget: device bytes a step may hold; max_rows / max_explain:
         scoring+variant rows and explain requests per step; streams: steps in
-        flight; tables: build the exact l-mer tables at start where the model
+        flight (slots with their own pinned buffers, on one HIP stream);
+        tables: build the exact l-mer tables at start where the model
         allows them; history: completed requests kept for latency_stats;
         stage_len: pinned host buffers are sized at start for steps of rows up
         to this many bases (longer rows grow them); freeze_gc: while the
@@ -165,7 +187,11 @@ class GkmServer:
         cyclic collector's generations (gc.freeze, process-wide), so a full
         collection scans only what was allocated since -- without it, a
         collection over the interpreter's whole heap (tens of ms with torch
-        loaded) lands inside some step."""
+        loaded) lands inside some step; coalesce_s: while a step is in flight,
+        how long past the oldest pending request's arrival the launcher lets
+        later arrivals join its step (the step would queue behind the one in
+        flight on the device anyway); default 0.5 ms on the per-SV kernels,
+        0 with the tables, whose steps the host bounds."""
         model.rocm()
         self.model = model
         self.sv = model._svs()
@@ -186,17 +212,19 @@ class GkmServer:
             self.explain_tables = True
         self.hbm_budget = int(hbm_budget)
         self.max_rows = int(max_rows)
+        self.coalesce_s = (5e-4 if self.table is None else 0.0) if coalesce_s is None \
+            else float(coalesce_s)
         self.max_explain = int(max_explain)
-        self._streams = [torch.cuda.Stream(device=self.kp.device) for _ in range(streams)]
-        self._free = list(self._streams)
-        # pinned staging and result buffers, one set per stream: a stream's
-        # step is delivered (its results copied out) before the stream takes a
-        # new step; sized here so no step allocates pinned memory
-        self._pinned = {id(st): {} for st in self._streams}
+        stream = torch.cuda.Stream(device=self.kp.device)
+        self._slots = [_Slot(stream) for _ in range(max(1, int(streams)))]
+        self._free = list(self._slots)
+        # pinned staging and result buffers, one set per slot: a slot's step
+        # is delivered (its results copied out) before the slot takes a new
+        # step; sized here so no step allocates pinned memory
         self._presize(int(stage_len))
         self._prepare(int(stage_len))
         # the tables and caches above were written on the caller's current
-        # stream, which the server's streams do not wait on: finish them now
+        # stream, which the server's stream does not wait on: finish them now
         torch.cuda.current_stream(self.kp.device).synchronize()
         self.freeze_gc = bool(freeze_gc)
         self._froze = False
@@ -214,10 +242,10 @@ class GkmServer:
         self.completed: deque = deque(maxlen=int(history))
 
     def _presize(self, stage_len: int):
-        """Allocate each stream's pinned buffers for a full step of rows up to
+        """Allocate each slot's pinned buffers for a full step of rows up to
         stage_len bases: staged string bytes, scores, explain outputs, flags."""
         rows = self.max_rows + self.max_explain
-        for st in self._streams:
+        for st in self._slots:
             self._host_buffer(st, "stage", (rows * stage_len + 12 * rows + 8,), torch.uint8)
             self._host_buffer(st, "out", (self.max_rows + self.max_explain * 4 * stage_len,))
             self._host_buffer(st, "bad", (rows,), torch.int32)
@@ -228,7 +256,7 @@ class GkmServer:
 
     def _prepare(self, stage_len: int):
         """Build the device state that steps would otherwise build lazily on
-        a server stream, where another stream could read it half-written: the
+        the server's stream, where another stream could read it half-written: the
         window-weight table, the explain weight tables, the SVs' l-mer
         weights V (explain without RBF), the SVs' l-mer index and the explain
         neighbourhoods."""
@@ -374,27 +402,46 @@ class GkmServer:
                 rows += req.rows()
         return batch
 
+    def _hold(self) -> float:
+        """Seconds to hold the pending requests back (0: admit now).  While a
+        step is in flight the next one only queues behind it on the device,
+        so arrivals get until coalesce_s past the oldest pending request's to
+        join it; no hold when the GPU is idle, when stopping, or once the
+        pending rows fill a step."""
+        if not self._inflight or self._stop or self.coalesce_s <= 0:
+            return 0.0
+        left = self._pending[0].t_submit + self.coalesce_s - time.perf_counter()
+        if left <= 0 or sum(r.n_rows for r in self._pending) >= self.max_rows:
+            return 0.0
+        return left
+
     # -- the worker ---------------------------------------------------------------------
 
     def _loop(self):
-        """The launcher: whenever requests are pending and a stream is free, it
-        admits the next step and launches it (results are the harvester's)."""
+        """The launcher: whenever requests are pending and a slot is free, it
+        admits the next step (after the hold of _hold) and launches it
+        (results are the harvester's)."""
         torch.cuda.set_device(self.kp.device)
         while True:
             with self._lock:
-                while not (self._pending and self._free):
-                    if self._stop and not self._pending:
-                        self._launcher_done = True
-                        self._lock.notify_all()
-                        return
-                    self._lock.wait()
+                while True:
+                    while not (self._pending and self._free):
+                        if self._stop and not self._pending:
+                            self._launcher_done = True
+                            self._lock.notify_all()
+                            return
+                        self._lock.wait()
+                    hold = self._hold()
+                    if hold <= 0:
+                        break
+                    self._lock.wait(hold)
                 batch = self._admit()
-                stream = self._free.pop()
+                slot = self._free.pop()
             try:
-                st = self._launch(batch, stream)
+                st = self._launch(batch, slot)
             except Exception as exc:  # surface launch errors on the requests
                 with self._lock:
-                    self._free.append(stream)
+                    self._free.append(slot)
                     self._lock.notify_all()
                 self._finish([(r, None, exc) for r in batch])
                 continue
@@ -405,7 +452,7 @@ class GkmServer:
     def _harvest_loop(self):
         """The harvester: waits on each step's completion event (oldest first,
         blocking in the runtime rather than polling), hands out its results and
-        frees its stream."""
+        frees its slot."""
         torch.cuda.set_device(self.kp.device)
         while True:
             with self._lock:
@@ -421,13 +468,13 @@ class GkmServer:
                 self._finish([(r, None, exc) for r in st.reqs])  # step's requests, not the server
             with self._lock:
                 self._inflight.pop(0)
-                self._free.append(st.stream)
+                self._free.append(st.slot)
                 self._lock.notify_all()
 
-    def _host_buffer(self, stream, key: str, shape, dtype=torch.float64) -> torch.Tensor:
-        """A view of `shape` on the stream's pinned buffer `key` (grown when
+    def _host_buffer(self, slot, key: str, shape, dtype=torch.float64) -> torch.Tensor:
+        """A view of `shape` on the slot's pinned buffer `key` (grown when
         too small)."""
-        pool = self._pinned[id(stream)]
+        pool = slot.pool
         n = int(np.prod(shape))
         buf = pool.get(key)
         if buf is None or buf.numel() < n or buf.dtype != dtype:
@@ -436,12 +483,12 @@ class GkmServer:
             pool[key] = buf
         return buf[:n].view(shape)
 
-    def _staging(self, stream, strings) -> torch.Tensor:
-        """The stream's pinned staging buffer, large enough for the strings'
+    def _staging(self, slot, strings) -> torch.Tensor:
+        """The slot's pinned staging buffer, large enough for the strings'
         bytes, their int64 starts and int32 lengths."""
         n = len(strings)
         need = -(-sum(map(len, strings)) // 8) * 8 + 12 * n
-        return self._host_buffer(stream, "stage", (need,), torch.uint8)
+        return self._host_buffer(slot, "stage", (need,), torch.uint8)
 
     def _deliver(self, st):
         # rows with a letter other than ACGTN (flagged on the device) or empty
@@ -451,7 +498,7 @@ class GkmServer:
             bad |= st.empty
             if not bad.any():
                 bad = None
-        # one copy of the step's scores out of the stream's pinned buffer; each
+        # one copy of the step's scores out of the slot's pinned buffer; each
         # request gets its own slice of it; rows shorter than l score NaN
         scores = None
         if st.score_host is not None:
@@ -504,7 +551,7 @@ class GkmServer:
                 explains.append(req)
         return spans, strings, explains
 
-    def _explain_calls(self, reqs, stream, keep, ex_out):
+    def _explain_calls(self, reqs, slot, keep, ex_out):
         """Explain requests through gkmexplain, one call per mode (rows of any
         length; each row's attributions depend only on that row); a request
         with an empty string or a letter other than ACGTN fails alone."""
@@ -526,7 +573,7 @@ class GkmServer:
                 continue
             seqs = [q for req in good for q in req.seqs]
             res = gkmexplain(self.model, seqs, mode=mode, batch_size=len(seqs))
-            hv = self._host_buffer(stream, f"explain{i}", tuple(res.shape))
+            hv = self._host_buffer(slot, f"explain{i}", tuple(res.shape))
             hv.copy_(res, non_blocking=True)
             keep.append(res)
             rr = 0
@@ -534,7 +581,7 @@ class GkmServer:
                 ex_out.append((req, hv, rr, len(req.seqs), req.max_len(), None, None))
                 rr += len(req.seqs)
 
-    def _launch_tables(self, batch, stream) -> _Step:
+    def _launch_tables(self, batch, slot) -> _Step:
         """A step through the exact l-mer tables: the scoring/variant rows and
         the explain rows the explain tables serve go to the device in one
         pinned copy of their bytes, are decoded and packed there (rows with
@@ -546,6 +593,7 @@ class GkmServer:
         from .explain import _MODE_ALIASES
 
         kp, sv, model = self.kp, self.sv, self.model
+        stream = slot.stream
         spans, strings, explains = self._rows(batch, time.perf_counter())
         ns = len(strings)
         groups = {"hypothetical": [], "perturbation": []} if self.explain_tables else {}
@@ -567,7 +615,7 @@ class GkmServer:
             if strings:
                 xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
                                                      min_width=kp.l,
-                                                     staging=self._staging(stream, strings))
+                                                     staging=self._staging(slot, strings))
                 lens = xs.lengths
                 empty = lens == 0
                 short = lens < kp.l  # NaN rows, set when delivered
@@ -596,9 +644,9 @@ class GkmServer:
                                  E._p(dx[r0:]) if dx is not None else None, int(kp.normalize),
                                  int(pert), E._p(dev[off:]), sh))
                     off += n * 4 * w
-                host = self._host_buffer(stream, "out", (total,))
+                host = self._host_buffer(slot, "out", (total,))
                 host.copy_(dev, non_blocking=True)
-                bad_host = self._host_buffer(stream, "bad", (len(strings),), torch.int32)
+                bad_host = self._host_buffer(slot, "bad", (len(strings),), torch.int32)
                 bad_host.copy_(dbad, non_blocking=True)
                 keep += [xs, dev, dx, dbad]
                 if ns:
@@ -613,19 +661,20 @@ class GkmServer:
                         ex_out.append((req, hv, rr, len(req.seqs), req.max_len(), sg, r0 + rr))
                         rr += len(req.seqs)
                     off += n * 4 * w
-            self._explain_calls(others, stream, keep, ex_out)
+            self._explain_calls(others, slot, keep, ex_out)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return _Step(batch, stream, ev, score_host, score_short, bad_host, empty, spans, ex_out,
+        return _Step(batch, slot, ev, score_host, score_short, bad_host, empty, spans, ex_out,
                      keep)
 
-    def _launch(self, batch, stream) -> _Step:
+    def _launch(self, batch, slot) -> _Step:
         """A step through the per-SV kernels (or, with tables, _launch_tables):
         every scoring/variant row of the step decoded on the device from one
         pinned copy of their bytes and scored in one batch."""
         if self.table is not None:
-            return self._launch_tables(batch, stream)
+            return self._launch_tables(batch, slot)
         kp, sv, model = self.kp, self.sv, self.model
+        stream = slot.stream
         spans, strings, explains = self._rows(batch, time.perf_counter())
         score_host = score_short = bad_host = empty = None
         ex_out, keep = [], []
@@ -633,21 +682,21 @@ class GkmServer:
             if strings:
                 xs, dbad = E.DeviceSeqs.from_strings(strings, kp.device, rc=kp.rc, stream=stream,
                                                      min_width=kp.l,
-                                                     staging=self._staging(stream, strings))
+                                                     staging=self._staging(slot, strings))
                 empty = xs.lengths == 0
                 out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
                 E.score_rows(sv, xs, model.bias, out, stream=stream)
                 short = xs.lengths < kp.l
                 score_short = short if short.any() else None
-                score_host = self._host_buffer(stream, "score", out.shape)
+                score_host = self._host_buffer(slot, "score", out.shape)
                 score_host.copy_(out, non_blocking=True)
-                bad_host = self._host_buffer(stream, "bad", (len(strings),), torch.int32)
+                bad_host = self._host_buffer(slot, "bad", (len(strings),), torch.int32)
                 bad_host.copy_(dbad, non_blocking=True)
                 keep += [xs, out, dbad]
-            self._explain_calls(explains, stream, keep, ex_out)
+            self._explain_calls(explains, slot, keep, ex_out)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return _Step(batch, stream, ev, score_host, score_short, bad_host, empty, spans, ex_out,
+        return _Step(batch, slot, ev, score_host, score_short, bad_host, empty, spans, ex_out,
                      keep)
 
 
