@@ -2529,24 +2529,36 @@ GKM_API int gkm_index_hist(const uint32_t *xwin, const int32_t *xpos, const int3
 // chunks that hold its windows, every SV tile).
 // pair_first NULL: slots [chunk][B][n_tiles] of gkm_index_score; else the
 // persistent form's slots (pair_first[b] + chunk) * n_tiles + tile.
+// alias P > 0 (persistent form): the last P rows are alternate alleles that
+// were not probed; row b = B - P + p takes the level sums of rows
+// B - 4P + p (its reference allele) + B - 3P + p (its windows over the
+// variant) - B - 2P + p (the reference's windows there): the counts are
+// sums over windows, so these are its own level sums exactly.
 __global__ __launch_bounds__(WAVE) void k_index_score_reduce(
     const i128 *__restrict__ psum, int B, int n_tiles, int nm, int nchunks,
     const int32_t *__restrict__ xlen, int l, int chunk_windows, const double *__restrict__ tab,
     const double *__restrict__ dx, int shift, int normalize, double bias,
-    double *__restrict__ out, const int32_t *__restrict__ pair_first)
+    double *__restrict__ out, const int32_t *__restrict__ pair_first, int alias)
 {
     __shared__ i128 sums[16];
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
-    const int wx = xlen[b] - l + 1;
-    const int u = wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
+    const bool aliased = alias > 0 && b >= B - alias;
+    const int n_src = aliased ? 3 : 1;
     for (int lv = 0; lv < nm; ++lv) {
         i128 acc = 0;
-        for (int k = lane; k < u * n_tiles; k += WAVE) {
-            const int c = k / n_tiles, t = k - c * n_tiles;
-            const int64_t slot = pair_first ? ((int64_t)pair_first[b] + c) * n_tiles + t
-                                            : ((int64_t)c * B + b) * n_tiles + t;
-            acc += psum[slot * nm + lv];
+        for (int j = 0; j < n_src; ++j) {
+            const int r = aliased ? b - (3 - j) * alias : b;
+            const int wx = xlen[r] - l + 1;
+            const int u = wx > 0 ? min(nchunks, (wx + chunk_windows - 1) / chunk_windows) : 0;
+            i128 part = 0;
+            for (int k = lane; k < u * n_tiles; k += WAVE) {
+                const int c = k / n_tiles, t = k - c * n_tiles;
+                const int64_t slot = pair_first ? ((int64_t)pair_first[r] + c) * n_tiles + t
+                                                : ((int64_t)c * B + r) * n_tiles + t;
+                part += psum[slot * nm + lv];
+            }
+            acc = j == 2 ? acc - part : acc + part;
         }
         acc = wave_sum_i128(acc);
         if (lane == 0)
@@ -2640,7 +2652,7 @@ GKM_API int gkm_index_score(const uint32_t *xwin, const int32_t *xpos, const int
     const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
     hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
                        (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
-                       dx, shift, normalize, bias, out, nullptr);
+                       dx, shift, normalize, bias, out, nullptr, 0);
     GKM_CHECK_LAUNCH("gkm_index_score");
     return 0;
 }
@@ -2693,6 +2705,61 @@ GKM_API int64_t gkm_index_queue_bytes(int B, int n_pairs, int n_tiles, int nm)
     return queue_plan_bytes(B) + (int64_t)(n_pairs > 0 ? n_pairs : 0) * n_tiles * nm * 16;
 }
 
+GKM_API int gkm_index_score_queue_pairs(const uint32_t *xwin, const int32_t *xpos,
+                                        const int32_t *xlen, int B, int Wx,
+                                        const uint32_t *offsets, const uint16_t *ids,
+                                        const uint8_t *vals, const uint32_t *records,
+                                        int record_tables, const uint32_t *masks,
+                                        const uint8_t *levels, int n_masks, int l, int nm,
+                                        const uint8_t *wtab, int S, int tile, int n_tiles,
+                                        int chunk_windows, int pack_counts,
+                                        const long long *wfix, int n_pairs, int n_blocks,
+                                        void *workspace, const double *tab, const double *dx,
+                                        int shift, int normalize, double bias, double *out,
+                                        int pairs, void *stream)
+{
+    if (wfix == nullptr || workspace == nullptr || out == nullptr || tab == nullptr)
+        return fail(-1, "gkm_index_score_queue: wfix, workspace, tab and out are required");
+    if (normalize && dx == nullptr)
+        return fail(-1, "gkm_index_score_queue: normalize needs dx");
+    if ((uintptr_t)workspace % 256)
+        return fail(-1, "gkm_index_score_queue: workspace must be 256-byte aligned");
+    if (n_blocks < 1 || n_blocks > 65536)
+        return fail(-1, "gkm_index_score_queue: n_blocks must be 1..65536, got %d", n_blocks);
+    if (pairs < 0 || (int64_t)pairs * 4 > B)
+        return fail(-1, "gkm_index_score_queue_pairs: %d allele pairs need 4 rows each, %d rows",
+                    pairs, B);
+    if (pairs > 0 && (wtab != nullptr || xpos != nullptr))
+        return fail(-1, "gkm_index_score_queue_pairs: windows weighted by position cannot move "
+                        "to rows of their own");
+    if (B <= 0)
+        return 0;
+    if (chunk_windows < 1)
+        return fail(-1, "gkm_index_score_queue: chunk_windows must be >= 1");
+    // the rows probed: all but the last `pairs` (alternate alleles, summed in the reduce)
+    const int Bq = B - pairs;
+    // workspace: 8 queue counters | first[Bq] | level sums [n_pairs][n_tiles][nm]
+    int *queue = (int *)workspace;
+    int32_t *pair_first = (int32_t *)((uint8_t *)workspace + 256);
+    void *psum = (uint8_t *)workspace + queue_plan_bytes(Bq);
+    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
+    hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), xlen, Bq, l,
+                       chunk_windows, nchunks, queue, pair_first);
+    GKM_CHECK_LAUNCH("gkm_index_score_queue");
+    int count_bytes = 4;
+    int rc = launch_index_tile("gkm_index_score_queue", xwin, xpos, xlen, Bq, Wx, offsets, ids,
+                               vals, records, record_tables, masks, levels, n_masks, l, nm, wtab,
+                               S, tile, n_tiles, chunk_windows, pack_counts, -1, 0, psum, wfix,
+                               &count_bytes, stream, pair_first, n_pairs, queue, n_blocks);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
+                       (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
+                       dx, shift, normalize, bias, out, pair_first, pairs);
+    GKM_CHECK_LAUNCH("gkm_index_score_queue");
+    return 0;
+}
+
 GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
                                   int B, int Wx, const uint32_t *offsets, const uint16_t *ids,
                                   const uint8_t *vals, const uint32_t *records, int record_tables,
@@ -2704,38 +2771,11 @@ GKM_API int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, con
                                   int shift, int normalize, double bias, double *out,
                                   void *stream)
 {
-    if (wfix == nullptr || workspace == nullptr || out == nullptr || tab == nullptr)
-        return fail(-1, "gkm_index_score_queue: wfix, workspace, tab and out are required");
-    if (normalize && dx == nullptr)
-        return fail(-1, "gkm_index_score_queue: normalize needs dx");
-    if ((uintptr_t)workspace % 256)
-        return fail(-1, "gkm_index_score_queue: workspace must be 256-byte aligned");
-    if (n_blocks < 1 || n_blocks > 65536)
-        return fail(-1, "gkm_index_score_queue: n_blocks must be 1..65536, got %d", n_blocks);
-    if (B <= 0)
-        return 0;
-    if (chunk_windows < 1)
-        return fail(-1, "gkm_index_score_queue: chunk_windows must be >= 1");
-    // workspace: 8 queue counters | first[B] | level sums [n_pairs][n_tiles][nm]
-    int *queue = (int *)workspace;
-    int32_t *pair_first = (int32_t *)((uint8_t *)workspace + 256);
-    void *psum = (uint8_t *)workspace + queue_plan_bytes(B);
-    const int nchunks = Wx > 0 && S > 0 && n_masks > 0 ? (Wx + chunk_windows - 1) / chunk_windows : 0;
-    hipLaunchKernelGGL(k_queue_plan, dim3(1), dim3(PLAN_THREADS), 0, S_(stream), xlen, B, l,
-                       chunk_windows, nchunks, queue, pair_first);
-    GKM_CHECK_LAUNCH("gkm_index_score_queue");
-    int count_bytes = 4;
-    int rc = launch_index_tile("gkm_index_score_queue", xwin, xpos, xlen, B, Wx, offsets, ids,
-                               vals, records, record_tables, masks, levels, n_masks, l, nm, wtab,
-                               S, tile, n_tiles, chunk_windows, pack_counts, -1, 0, psum, wfix,
-                               &count_bytes, stream, pair_first, n_pairs, queue, n_blocks);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_index_score_reduce, dim3((unsigned)B), dim3(WAVE), 0, S_(stream),
-                       (const i128 *)psum, B, n_tiles, nm, nchunks, xlen, l, chunk_windows, tab,
-                       dx, shift, normalize, bias, out, pair_first);
-    GKM_CHECK_LAUNCH("gkm_index_score_queue");
-    return 0;
+    return gkm_index_score_queue_pairs(xwin, xpos, xlen, B, Wx, offsets, ids, vals, records,
+                                       record_tables, masks, levels, n_masks, l, nm, wtab, S,
+                                       tile, n_tiles, chunk_windows, pack_counts, wfix, n_pairs,
+                                       n_blocks, workspace, tab, dx, shift, normalize, bias, out,
+                                       0, stream);
 }
 
 GKM_API int gkm_index_hist_queue(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,

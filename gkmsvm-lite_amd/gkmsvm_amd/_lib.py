@@ -52,6 +52,8 @@ SIGNATURES = {
                       I, I, D, I, P, P, P],
     "gkm_index_score_queue": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, P,
                               I, I, P, P, P, I, I, D, P, P],
+    "gkm_index_score_queue_pairs": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I,
+                                    I, P, I, I, P, P, P, I, I, D, P, I, P],
     "gkm_index_hist_queue": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I,
                              I, I, P, P, P, P],
     "gkm_scratch_bytes": [],

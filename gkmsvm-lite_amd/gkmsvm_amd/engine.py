@@ -860,27 +860,77 @@ def n_cus(device) -> int:
 
 
 def index_score_queue(sv: SVModel, xs: DeviceSeqs, dx, bias: float, out: torch.Tensor,
-                      stream=None, chunk: int = QUEUE_CHUNK) -> torch.Tensor:
+                      stream=None, chunk: int = QUEUE_CHUNK, pairs: int = 0) -> torch.Tensor:
     """out[:] = every row's decision value from the l-mer index in one
     persistent launch (gkm_index_score_queue): the rows cut into work items
     of `chunk` windows (planned on the device from the uploaded lengths),
     popped by one block per CU from per-XCD queues.  The same bits as
-    index_score.  Kernels without RBF."""
+    index_score.  Kernels without RBF.  pairs > 0: the last 4 * pairs rows
+    are allele pairs in the layout of variant_rows, the alternate alleles
+    (the last pairs rows) scored from the level sums of the other three
+    blocks (gkm_index_score_queue_pairs) -- the same bits as probing them."""
     kp, idx = sv.kp, sv.index()
-    L = IndexLaunch(kp, idx, xs, 0, xs.n, sv.S, -1, stream, None, chunk=chunk)
+    if pairs < 0 or 4 * pairs > xs.n:
+        raise ValueError(f"index_score_queue: {pairs} allele pairs need 4 rows each, "
+                         f"{xs.n} rows")
+    bq = xs.n - pairs
+    L = IndexLaunch(kp, idx, xs, 0, bq, sv.S, -1, stream, None, chunk=chunk)
     # the chunk count the plan kernel computes from the lengths on the device
-    n_pairs = int(np.minimum(-(-np.clip(xs.lengths - kp.l + 1, 0, None) // L.cw),
+    n_pairs = int(np.minimum(-(-np.clip(xs.lengths[:bq] - kp.l + 1, 0, None) // L.cw),
                              L.nchunks).sum())
-    ws = torch.empty(_lib.load().gkm_index_queue_bytes(xs.n, n_pairs, idx.n_tiles, kp.nm),
+    ws = torch.empty(_lib.load().gkm_index_queue_bytes(bq, n_pairs, idx.n_tiles, kp.nm),
                      dtype=torch.uint8, device=kp.device)
-    args = L.args[:-2]
-    _timed("k_index_tile_queue", lambda: index_bytes(kp, idx, xs.lengths), stream,
+    args = list(L.args[:-2])
+    args[3] = xs.n  # every row's value; the plan and the probes take the first bq
+    _timed("k_index_tile_queue", lambda: index_bytes(kp, idx, xs.lengths[:bq]), stream,
            lambda: _lib.call(
-               "gkm_index_score_queue", *args, _p(sv.wfix), n_pairs, n_cus(kp.device), _p(ws),
-               _p(kp.tab),
+               "gkm_index_score_queue_pairs", *args, _p(sv.wfix), n_pairs, n_cus(kp.device),
+               _p(ws), _p(kp.tab),
                _p(dx) if dx is not None else None, sv.fix_shift, int(kp.normalize), float(bias),
-               _p(out), stream_handle(stream)))
+               _p(out), int(pairs), stream_handle(stream)))
     return out
+
+
+# An allele pair is scored through its reference's level sums when the
+# windows its variant changes are under this fraction of the allele's
+VARIANT_ALIAS_FRACTION = 0.5
+
+
+def variant_cut(ref: str, alt: str, l: int):
+    """For one pair of allele strings: the base range (c0, c1) of the
+    alternate's windows covering a base where the two differ (as
+    variant_cuts), or None where the pair is better scored as two rows
+    (lengths differ, identical, shorter than l, or changes spread over too
+    many windows).  The differing bases come from the XOR of the strings'
+    bytes read as one integer each: its highest set bit is in the first
+    differing byte, its lowest in the last."""
+    L = len(ref)
+    if len(alt) != L or L < l:
+        return None
+    x = (int.from_bytes(ref.encode("ascii", "replace"), "big")
+         ^ int.from_bytes(alt.encode("ascii", "replace"), "big"))
+    if not x:
+        return None
+    lo = L - 1 - (x.bit_length() - 1) // 8
+    hi = L - 1 - ((x & -x).bit_length() - 1) // 8
+    W = L - l + 1
+    w0, w1 = max(lo - l + 1, 0), min(hi, W - 1)
+    if w1 - w0 + 1 >= VARIANT_ALIAS_FRACTION * W:
+        return None
+    return w0, w1 + l
+
+
+def variant_cuts(lengths: np.ndarray, lo: np.ndarray, hi: np.ndarray, l: int):
+    """For allele pairs of equal length (lengths) differing in bases lo..hi:
+    (aliased mask, c0, c1) -- the pairs worth scoring through the reference's
+    level sums, and the base range [c0, c1) of each one's windows covering a
+    differing base (windows w0..w1: bases w0 .. w1 + l - 1)."""
+    W = lengths - l + 1
+    w0 = np.maximum(lo - l + 1, 0)
+    w1 = np.minimum(hi, W - 1)
+    nd = w1 - w0 + 1
+    ok = (lo >= 0) & (W > 0) & (nd >= 1) & (nd < VARIANT_ALIAS_FRACTION * W)
+    return ok, w0, w1 + l
 
 
 # ISM sweep (gkm_ism): window pairs of (query, SV) per second, measured on the
@@ -976,10 +1026,21 @@ def table_rows(kp: KernelParams, xs: DeviceSeqs, table: torch.Tensor, bias: floa
     return out
 
 
+def pairs_apply(sv: SVModel) -> bool:
+    """Whether score_rows can take allele pairs through their references'
+    level sums (the fused index scoring: no RBF, no position weights)."""
+    return sv.wfix is not None and not sv.kp.weighted
+
+
 def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stream=None,
-               shape=None, use_index="auto", table: torch.Tensor | None = None):
+               shape=None, use_index="auto", table: torch.Tensor | None = None,
+               pairs: int = 0):
     """out[b] = sum_s coef_s F(K(x_b, s)) + bias for every row of xs (through
-    the exact l-mer table when one is given)."""
+    the exact l-mer table when one is given).  pairs: the last 4 * pairs rows
+    are allele pairs in four blocks (references, each alternate's windows
+    over its variant, the references' windows there, the alternates); where
+    pairs_apply and the index is taken, the alternates are scored from the
+    other blocks' level sums instead of probed, with the same bits."""
     kp = sv.kp
     if table is not None:
         return table_rows(kp, xs, table, bias, out, stream=stream)
@@ -987,7 +1048,8 @@ def score_rows(sv: SVModel, xs: DeviceSeqs, bias: float, out: torch.Tensor, stre
     if sv.wfix is not None and sv.use_index_for(xs.lengths, use_index):
         # one persistent launch: balanced over rows of any lengths (a 103-row
         # serving batch 9.8 -> 7.9 ms against the grid of index_score)
-        return index_score_queue(sv, xs, dx, bias, out, stream=stream)
+        return index_score_queue(sv, xs, dx, bias, out, stream=stream,
+                                 pairs=pairs if pairs_apply(sv) else 0)
     step = rows_per_chunk(sv.S, kp.nm)
     hist = None
     for r0 in range(0, xs.n, step):

@@ -163,7 +163,8 @@ class _Step:
     score_short: np.ndarray | None  # scoring rows shorter than l (scored NaN)
     bad_host: torch.Tensor | None   # [rows] int32: a letter other than ACGTN (device flags)
     empty: np.ndarray | None        # [rows] empty strings
-    spans: list               # (request, start row, n rows) for scoring/variant rows
+    spans: list               # (request, rows, alt rows): scoring rows (a slice, None) or a
+                              # variant's reference and alternate rows (slices or lists)
     explain_out: list         # (request, host [rows, 4, L], first row, rows, length,
                               #  short rows, first row in the step's rows or None)
     keep: list                # device tensors kept alive until the step completes
@@ -248,9 +249,10 @@ class GkmServer:
         for st in self._slots:
             self._host_buffer(st, "stage", (rows * stage_len + 12 * rows + 8,), torch.uint8)
             self._host_buffer(st, "out", (self.max_rows + self.max_explain * 4 * stage_len,))
-            self._host_buffer(st, "bad", (rows,), torch.int32)
+            # variants scored through their references add two cut rows a pair
+            self._host_buffer(st, "bad", (rows + self.max_rows,), torch.int32)
             if self.table is None:
-                self._host_buffer(st, "score", (self.max_rows,))
+                self._host_buffer(st, "score", (2 * self.max_rows,))
             for i in range(2):
                 self._host_buffer(st, f"explain{i}", (self.max_explain * 4 * stage_len,))
 
@@ -506,16 +508,15 @@ class GkmServer:
             if st.score_short is not None:
                 scores[st.score_short] = np.nan
         results = {}
-        for req, r0, n in st.spans:
-            if bad is not None and bad[r0:r0 + n].any():
+        for req, rows, alt_rows in st.spans:
+            if bad is not None and (bad[rows].any()
+                                    or (alt_rows is not None and bad[alt_rows].any())):
                 results[id(req)] = ValueError(_BAD_SEQ)  # fails alone
                 continue
-            vals = scores[r0:r0 + n]
-            if req.kind == "variant":
-                half = n // 2
-                results[id(req)] = vals[half:] - vals[:half]  # alt - ref
+            if alt_rows is None:
+                results[id(req)] = scores[rows]
             else:
-                results[id(req)] = vals
+                results[id(req)] = scores[alt_rows] - scores[rows]  # alt - ref
         for req, host, r0, n, L, short, g0 in st.explain_out:
             if req.done():  # rejected at launch
                 continue
@@ -534,22 +535,55 @@ class GkmServer:
             items.append((req, None, res) if isinstance(res, Exception) else (req, res, None))
         self._finish(items)
 
-    def _rows(self, batch, t):
-        """The step's scoring/variant rows: (spans, strings), and its explain
-        requests."""
-        spans, strings, explains = [], [], []
+    def _rows(self, batch, t, pairs: bool = False):
+        """The step's scoring/variant rows: (spans, strings, number of allele
+        pairs in score_rows' pairs layout), and its explain requests.  A
+        variant's reference and alternate are two rows; with `pairs`, the
+        pairs E.variant_cut takes go last instead, in four blocks --
+        references, each alternate cut to its windows over the variant, the
+        references cut there, the alternates."""
+        spans, strings, explains, cuts = [], [], [], []
+        l = self.kp.l
         for req in batch:
             req.t_admit = t
             if req.kind == "score":
-                spans.append((req, len(strings), len(req.seqs)))
+                r0 = len(strings)
+                spans.append((req, slice(r0, r0 + len(req.seqs)), None))
                 strings.extend(req.seqs)
             elif req.kind == "variant":
-                spans.append((req, len(strings), 2 * len(req.seqs)))
-                strings.extend([r for r, _ in req.seqs])
-                strings.extend([a for _, a in req.seqs])
+                r0, m = len(strings), len(req.seqs)
+                if not pairs:
+                    for r, a in req.seqs:
+                        strings.append(r)
+                        strings.append(a)
+                    spans.append((req, slice(r0, r0 + 2 * m, 2), slice(r0 + 1, r0 + 2 * m, 2)))
+                    continue
+                ref_rows, alt_rows = [], []
+                for r, a in req.seqs:
+                    c = E.variant_cut(r, a, l)
+                    if c is None:
+                        ref_rows.append(len(strings))
+                        alt_rows.append(len(strings) + 1)
+                        strings.append(r)
+                        strings.append(a)
+                    else:  # rows set once the blocks are laid out
+                        cuts.append((ref_rows, alt_rows, len(ref_rows), r, a, c))
+                        ref_rows.append(-1)
+                        alt_rows.append(-1)
+                spans.append((req, ref_rows, alt_rows))
             else:
                 explains.append(req)
-        return spans, strings, explains
+        P = len(cuts)
+        if P:
+            b = len(strings)
+            strings.extend([r for _, _, _, r, _, _ in cuts])
+            strings.extend([a[c0:c1] for _, _, _, _, a, (c0, c1) in cuts])
+            strings.extend([r[c0:c1] for _, _, _, r, _, (c0, c1) in cuts])
+            strings.extend([a for _, _, _, _, a, _ in cuts])
+            for k, (ref_rows, alt_rows, i, _, _, _) in enumerate(cuts):
+                ref_rows[i] = b + k
+                alt_rows[i] = b + 3 * P + k
+        return spans, strings, explains, P
 
     def _explain_calls(self, reqs, slot, keep, ex_out):
         """Explain requests through gkmexplain, one call per mode (rows of any
@@ -594,7 +628,7 @@ class GkmServer:
 
         kp, sv, model = self.kp, self.sv, self.model
         stream = slot.stream
-        spans, strings, explains = self._rows(batch, time.perf_counter())
+        spans, strings, explains, _ = self._rows(batch, time.perf_counter())
         ns = len(strings)
         groups = {"hypothetical": [], "perturbation": []} if self.explain_tables else {}
         others = []
@@ -675,7 +709,8 @@ class GkmServer:
             return self._launch_tables(batch, slot)
         kp, sv, model = self.kp, self.sv, self.model
         stream = slot.stream
-        spans, strings, explains = self._rows(batch, time.perf_counter())
+        spans, strings, explains, pairs = self._rows(batch, time.perf_counter(),
+                                                     pairs=E.pairs_apply(sv))
         score_host = score_short = bad_host = empty = None
         ex_out, keep = [], []
         with torch.cuda.stream(stream):
@@ -685,7 +720,7 @@ class GkmServer:
                                                      staging=self._staging(slot, strings))
                 empty = xs.lengths == 0
                 out = torch.empty(len(strings), dtype=torch.float64, device=kp.device)
-                E.score_rows(sv, xs, model.bias, out, stream=stream)
+                E.score_rows(sv, xs, model.bias, out, stream=stream, pairs=pairs)
                 short = xs.lengths < kp.l
                 score_short = short if short.any() else None
                 score_host = self._host_buffer(slot, "score", out.shape)

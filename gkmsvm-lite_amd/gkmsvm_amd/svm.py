@@ -266,13 +266,15 @@ class GkmSVM:
 
     # -- scoring ------------------------------------------------------------------
 
-    def _score_codes(self, codes: np.ndarray, lengths: np.ndarray) -> torch.Tensor:
-        """[B] decision values of host codes (rows shorter than l included)."""
+    def _score_codes(self, codes: np.ndarray, lengths: np.ndarray,
+                     pairs: int = 0) -> torch.Tensor:
+        """[B] decision values of host codes (rows shorter than l included;
+        pairs: E.score_rows' allele pairs in the last 4 * pairs rows)."""
         sv = self._svs()
         xs = E.DeviceSeqs(codes, lengths, self._device, rc=self._params().rc)
         out = torch.empty(codes.shape[0], dtype=torch.float64, device=self._device)
         return E.score_rows(sv, xs, self.bias, out, use_index=self.use_lmer_index,
-                            table=self._lmer_table_for())
+                            table=self._lmer_table_for(), pairs=pairs)
 
     def __call__(self, x, *, lengths=None, batch_size: int | None = None,
                  verbose: bool = False) -> torch.Tensor:
@@ -311,6 +313,10 @@ class GkmSVM:
         the GkmExplain mutation impact ("gkmexplain", lsgkm gkmexplain -m 5)."""
         if method == "kernel":
             r, a = read_pair(ref, alt, ref_lengths, alt_lengths)
+            if batch_size is None:
+                effects = self._variant_pairs(r, a)
+                if effects is not None:
+                    return effects
             both = stacked(a, r)
             scores = self(both.codes, lengths=both.given_lengths(), batch_size=batch_size,
                           verbose=verbose)
@@ -322,6 +328,57 @@ class GkmSVM:
                                    alt_lengths=alt_lengths, batch_size=batch_size,
                                    verbose=verbose)
         raise ValueError(f"method must be 'kernel' or 'gkmexplain', got {method!r}")
+
+    def _variant_pairs(self, r, a) -> torch.Tensor | None:
+        """score(alt) - score(ref) in one call, the alternates that differ
+        from their reference in few windows (E.variant_cuts) scored from the
+        reference's level sums plus those windows (E.score_rows' pairs layout:
+        the same bits as scoring them); None where that does not apply (the
+        l-mer table, RBF, position weights, no pair worth it)."""
+        self._require_device()
+        sv = self._svs()
+        l = self.kernel.base.l
+        if not E.pairs_apply(sv) or r.width < l:
+            return None
+        self._plan_lmer_table(float(r.n_windows(l).sum() + a.n_windows(l).sum()))
+        if self._lmer_table_for() is not None:
+            return None
+        width = r.width
+        inside = np.arange(width)[None, :] < r.lengths[:, None]
+        diff = (r.codes != a.codes) & inside & (r.lengths == a.lengths)[:, None]
+        some = diff.any(axis=1)
+        lo = np.where(some, diff.argmax(axis=1), -1)
+        hi = np.where(some, width - 1 - diff[:, ::-1].argmax(axis=1), -1)
+        ok, c0, c1 = E.variant_cuts(r.lengths, lo, hi, l)
+        if not ok.any():
+            return None
+        plain, cut = np.flatnonzero(~ok), np.flatnonzero(ok)
+        n_plain, P = plain.size, cut.size
+        # rows: plain references, plain alternates, then the pairs layout
+        span = c1[cut] - c0[cut]
+        k = np.arange(int(span.max()))
+        cols = np.minimum(c0[cut][:, None] + k[None, :], width - 1)
+        keep = k[None, :] < span[:, None]
+        codes = np.concatenate([
+            r.codes[plain], a.codes[plain], r.codes[cut],
+            np.pad(np.where(keep, a.codes[cut[:, None], cols], 0), ((0, 0), (0, width - k.size))),
+            np.pad(np.where(keep, r.codes[cut[:, None], cols], 0), ((0, 0), (0, width - k.size))),
+            a.codes[cut]]).astype(np.uint8)
+        lens = np.concatenate([r.lengths[plain], a.lengths[plain], r.lengths[cut], span, span,
+                               a.lengths[cut]])
+        out = self._score_codes(codes, lens, pairs=P).cpu().numpy()
+        n = r.n
+        ref_row = np.empty(n, dtype=np.int64)
+        alt_row = np.empty(n, dtype=np.int64)
+        ref_row[plain] = np.arange(n_plain)
+        alt_row[plain] = n_plain + np.arange(n_plain)
+        ref_row[cut] = 2 * n_plain + np.arange(P)
+        alt_row[cut] = 2 * n_plain + 3 * P + np.arange(P)
+        effects = out[alt_row] - out[ref_row]
+        short = stacked(a, r).short(l)  # NaN (and the warning) where an allele has no window
+        if short is not None:
+            effects[short[:n] | short[n:]] = np.nan
+        return torch.from_numpy(effects[:, None]).to(self._device)
 
     def _sv_diag(self) -> torch.Tensor:
         """K_raw(sv, sv) [S] of the base kernel, in SV order."""

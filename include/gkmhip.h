@@ -331,6 +331,31 @@ int gkm_index_score_queue(const uint32_t *xwin, const int32_t *xpos, const int32
                           const double *tab, const double *dx, int shift, int normalize,
                           double bias, double *out, void *stream);
 
+/* gkm_index_score_queue over B rows whose last 4 * pairs are allele pairs
+ * laid out in four blocks of `pairs` rows: reference alleles, then each
+ * alternate allele's windows over its variant (the rows cut to windows
+ * w0..w1, those covering a base that differs), then the reference's windows
+ * at the same positions, then the alternate alleles themselves.  Only the
+ * first B - pairs rows are probed (xwin and xpos hold those, n_pairs counts
+ * their items, workspace is gkm_index_queue_bytes(B - pairs, ...)); alternate
+ * allele p gets the level sums reference + its windows - the reference's
+ * windows, which are its own level sums exactly (sums over windows of exact
+ * integers), and its own dx.  Windows without position weights only (wtab
+ * and xpos NULL when pairs > 0).  out holds all B rows (the cut rows score as
+ * the short sequences they are).  Replaces the two full scorings of
+ * GkmSVM.score_variants (svm.py:632-647: score(alt) - score(ref) over the
+ * concatenated alleles) with one scoring of the reference and of the
+ * windows a variant changes. */
+int gkm_index_score_queue_pairs(const uint32_t *xwin, const int32_t *xpos, const int32_t *xlen,
+                                int B, int Wx, const uint32_t *offsets, const uint16_t *ids,
+                                const uint8_t *vals, const uint32_t *records, int record_tables,
+                                const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                                int nm, const uint8_t *wtab, int S, int tile, int n_tiles,
+                                int chunk_windows, int pack_counts, const long long *wfix,
+                                int n_pairs, int n_blocks, void *workspace, const double *tab,
+                                const double *dx, int shift, int normalize, double bias,
+                                double *out, int pairs, void *stream);
+
 /* gkm_index_hist as one persistent launch (the work list and queues of
  * gkm_index_score_queue): the same per-chunk partial counts in `partial`
  * (layout and size of gkm_index_hist's), then, hist non-NULL, ADDED into hist

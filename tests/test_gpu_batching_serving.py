@@ -63,7 +63,7 @@ def test_batch_scorer_matches_plain_calls(gpu, kt, extra, table):
 def test_server_results_equal_alone_and_in_the_mix(gpu, tables):
     """Requests submitted concurrently from several threads (scoring of many
     lengths, variants, explain in the three modes with rows shorter than l),
-    served in overlapping steps on two streams, return what each gives alone,
+    served in overlapping steps (two in flight), return what each gives alone,
     and empty requests their empty results (per-SV kernels, or the l-mer tables the
     server builds at start)."""
     from gkmsvm_amd import gkmexplain
