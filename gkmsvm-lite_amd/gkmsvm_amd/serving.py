@@ -40,6 +40,7 @@ import gc
 import threading
 import time
 from collections import deque
+from itertools import chain
 from concurrent.futures import TimeoutError as FutureTimeout
 from dataclasses import dataclass
 
@@ -98,10 +99,10 @@ class Request:
         self.t_submit = self.t_admit = self.t_done = 0.0
         if kind == "variant":
             self.n_rows = 2 * len(seqs)
-            self.lmax = max((max(len(r), len(a)) for r, a in seqs), default=0)
+            self.lmax = max(map(len, chain.from_iterable(seqs)), default=0)
         else:
             self.n_rows = len(seqs)
-            self.lmax = max((len(q) for q in seqs), default=0)
+            self.lmax = max(map(len, seqs), default=0)
         self.nbytes = 0
 
     def rows(self) -> int:
@@ -230,6 +231,8 @@ class GkmServer:
         self.freeze_gc = bool(freeze_gc)
         self._froze = False
         self._pending: deque = deque()
+        # rows (scoring / variant), bytes and explain requests pending
+        self._pending_rows = self._pending_bytes = self._pending_explain = 0
         self._inflight: list = []
         self._lock = threading.Condition()
         # what every request's waiters sleep on (notified once per step)
@@ -298,25 +301,65 @@ class GkmServer:
 
     # -- client side ----------------------------------------------------------------
 
-    def submit(self, kind: str, seqs, mode: str = "hypothetical") -> Request:
-        """Queue a request; returns its handle (see Request)."""
+    def _request(self, kind, seqs, mode, t):
+        """(request, None) to queue, or (request, its result item) when it is
+        settled at submission: empty (its empty result) or larger than the
+        HBM budget (MemoryError)."""
         if kind not in ("score", "variant", "explain"):
             raise ValueError(f"unknown request kind {kind!r}")
         req = Request(kind, list(seqs), mode, self._delivered)
-        req.t_submit = time.perf_counter()
+        req.t_submit = t
         if req.n_rows == 0:  # nothing to compute: the empty result at once
-            req.t_admit = req.t_submit
-            self._finish([(req, np.empty((0, 4, 0) if kind == "explain" else 0), None)])
-            return req
+            req.t_admit = t
+            return req, (req, np.empty((0, 4, 0) if kind == "explain" else 0), None)
         need = req.nbytes = self._bytes(req)
         if need > self.hbm_budget:
-            self._finish([(req, None, MemoryError(
-                f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))])
+            return req, (req, None, MemoryError(
+                f"request needs {need} bytes of HBM, over the server's budget {self.hbm_budget}"))
+        return req, None
+
+    def submit(self, kind: str, seqs, mode: str = "hypothetical") -> Request:
+        """Queue a request; returns its handle (see Request)."""
+        req, settled = self._request(kind, seqs, mode, time.perf_counter())
+        if settled is not None:
+            self._finish([settled])
             return req
         with self._lock:
             self._pending.append(req)
+            self._count_pending(req, 1)
             self._lock.notify_all()
         return req
+
+    def _count_pending(self, req, sign):
+        self._pending_bytes += sign * req.nbytes
+        if req.kind == "explain":
+            self._pending_explain += sign
+        else:
+            self._pending_rows += sign * req.n_rows
+
+    def submit_many(self, items) -> list:
+        """Queue requests given as (kind, seqs) or (kind, seqs, mode) together,
+        under one lock and one wake-up of the launcher; returns their handles
+        in order, as submit would one at a time."""
+        t = time.perf_counter()
+        reqs, queued, settled = [], [], []
+        for item in items:
+            req, done = self._request(item[0], item[1],
+                                      item[2] if len(item) > 2 else "hypothetical", t)
+            reqs.append(req)
+            if done is None:
+                queued.append(req)
+            else:
+                settled.append(done)
+        if settled:
+            self._finish(settled)
+        if queued:
+            with self._lock:
+                self._pending.extend(queued)
+                for req in queued:
+                    self._count_pending(req, 1)
+                self._lock.notify_all()
+        return reqs
 
     def _finish(self, items):
         """Set results [(request, value, error)] (requests already done are
@@ -386,6 +429,12 @@ class GkmServer:
 
     def _admit(self):
         """Pop the requests of the next step (at least one if any is pending)."""
+        if (self._pending_bytes <= self.hbm_budget and self._pending_rows <= self.max_rows
+                and self._pending_explain <= self.max_explain):  # all of them
+            batch = list(self._pending)
+            self._pending.clear()
+            self._pending_rows = self._pending_bytes = self._pending_explain = 0
+            return batch
         batch, used, rows, n_explain = [], 0, 0, 0
         while self._pending:
             req = self._pending[0]
@@ -396,6 +445,7 @@ class GkmServer:
                           or (is_ex and n_explain >= self.max_explain)):
                 break
             self._pending.popleft()
+            self._count_pending(req, -1)
             batch.append(req)
             used += need
             if is_ex:
@@ -413,7 +463,7 @@ class GkmServer:
         if not self._inflight or self._stop or self.coalesce_s <= 0:
             return 0.0
         left = self._pending[0].t_submit + self.coalesce_s - time.perf_counter()
-        if left <= 0 or sum(r.n_rows for r in self._pending) >= self.max_rows:
+        if left <= 0 or self._pending_rows >= self.max_rows:
             return 0.0
         return left
 
@@ -507,32 +557,25 @@ class GkmServer:
             scores = st.score_host.numpy().copy()
             if st.score_short is not None:
                 scores[st.score_short] = np.nan
-        results = {}
+        items = []
         for req, rows, alt_rows in st.spans:
             if bad is not None and (bad[rows].any()
                                     or (alt_rows is not None and bad[alt_rows].any())):
-                results[id(req)] = ValueError(_BAD_SEQ)  # fails alone
-                continue
-            if alt_rows is None:
-                results[id(req)] = scores[rows]
+                items.append((req, None, ValueError(_BAD_SEQ)))  # fails alone
+            elif alt_rows is None:
+                items.append((req, scores[rows], None))
             else:
-                results[id(req)] = scores[alt_rows] - scores[rows]  # alt - ref
+                items.append((req, scores[alt_rows] - scores[rows], None))  # alt - ref
         for req, host, r0, n, L, short, g0 in st.explain_out:
             if req.done():  # rejected at launch
                 continue
             if g0 is not None and bad is not None and bad[g0:g0 + n].any():
-                results[id(req)] = ValueError(_BAD_SEQ)
+                items.append((req, None, ValueError(_BAD_SEQ)))
                 continue
             res = host[r0:r0 + n, :, :L].numpy().copy()
             if short is not None and short[r0:r0 + n].any():
                 res[short[r0:r0 + n]] = np.nan
-            results[id(req)] = res
-        items = []
-        for req in st.reqs:
-            res = results.get(id(req))
-            if res is None:
-                continue  # failed at launch
-            items.append((req, None, res) if isinstance(res, Exception) else (req, res, None))
+            items.append((req, res, None))
         self._finish(items)
 
     def _rows(self, batch, t, pairs: bool = False):
