@@ -417,7 +417,7 @@ class DeviceIndex:
             self.pack = 1
         else:
             self.pack = 2 if count_bits == 16 else index_pack(kp.nm, True)
-        self.tile = tile or min(index_tile(kp.nm, self.pack), max(256, -(-S // 256) * 256))
+        self.tile = tile or min(index_tile(kp.nm, self.pack, S), max(256, -(-S // 256) * 256))
         self.n_tiles = (S + self.tile - 1) // self.tile
         fwd, rcw = seqs.windows(l, stream)
         blocks = [fwd] + ([rcw] if kp.rc else [])
@@ -554,16 +554,30 @@ def index_pack(nm: int, unweighted: bool) -> int:
     return 4 if (nm + 3) // 4 < (nm + 1) // 2 else 2
 
 
-def index_tile(nm: int, pack: int | bool = 1) -> int:
+# LDS words a tile's counts may take: the kernels' 159 KiB for counts and the
+# records' 64 sink words (the rest of the CU's 160 KiB is their static LDS)
+INDEX_LDS_WORDS = 159 * 1024 // 4 - 64
+
+
+def index_tile(nm: int, pack: int | bool = 1, S: int | None = None) -> int:
     """SVs per index tile: the tile's counts ([tile, nm] uint32, or `pack`
-    counts per word; True: the unweighted packing index_pack picks) take up to
-    144 KiB of the CU's 160 KiB of LDS -- the index is probe-bound, so the
+    counts per word; True: the unweighted packing index_pack picks) take 144
+    KiB of the CU's 160 KiB of LDS -- the index is probe-bound, so the
     largest tile that fits wins (fewer tiles, fewer probes); at most 65,536
-    (uint16 ids)."""
+    (uint16 ids).  Given the SV count S, a tile up to INDEX_LDS_WORDS is taken
+    where that needs fewer tiles (20,000 SVs with weighted 16-bit counts: one
+    tile of 20,224 instead of two of 18,432), the SVs spread evenly."""
     if pack is True or pack is False:
         pack = index_pack(nm, pack)
     words = -(-nm // pack)
-    return min(65536, max(256, (36864 // words) // 256 * 256))
+    tile = min(65536, max(256, (36864 // words) // 256 * 256))
+    if S is None or S <= tile:
+        return tile
+    big = min(65536, (INDEX_LDS_WORDS // words) // 256 * 256)
+    n = -(-S // big)
+    if n < -(-S // tile):
+        return -(-(-(-S // n)) // 256) * 256
+    return tile
 
 
 MAX_PARTIAL_BYTES = 8 << 30  # per-chunk index counts of one launch (rows per launch)
@@ -770,7 +784,7 @@ class SVModel:
                       if self.kp.h[m] != 0)
         per_lmer = norient * ws / 4**self.kp.l
         probes = n_windows * n_masks
-        n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted))
+        n_tiles = -(-self.S // index_tile(self.kp.nm, not self.kp.weighted, self.S))
         rate = INDEX_PROBES_PER_S_WEIGHTED if self.kp.weighted else INDEX_PROBES_PER_S
         index_s = probes * n_tiles / rate + probes * per_lmer / INDEX_ENTRIES_PER_S
         return index_s, brute_s

@@ -250,7 +250,7 @@ def _path_seconds(sv, kp, x_lengths, ta, tb, cmax):
     wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
     per_lmer = norient * float(np.clip(sv.seqs.lengths - kp.l + 1, 0, None).sum()) / 4**kp.l
     probes = wx * n_masks
-    n_tiles = -(-sv.S // E.index_tile(kp.nm, not kp.weighted))
+    n_tiles = -(-sv.S // E.index_tile(kp.nm, not kp.weighted, sv.S))
     if not kp.weighted and n_tiles <= E.LREC_MAX_TILES:
         n_tiles = 1  # one l-mer record per probe covers every tile
     index_s = (probes * n_tiles / INDEX_EXPLAIN_PROBES_PER_S

@@ -27,7 +27,7 @@ def _gram_index(kp, x_lengths, ys, force: bool = False):
     wy = float(np.clip(ys.lengths - kp.l + 1, 0, None).sum())
     wx = float(np.clip(np.asarray(x_lengths) - kp.l + 1, 0, None).sum())
     n_masks = sum(comb(kp.l, m) * 3**m for m in range(kp.dmax + 1) if kp.h[m] != 0)
-    n_tiles = -(-ys.n // E.index_tile(kp.nm, not kp.weighted))
+    n_tiles = -(-ys.n // E.index_tile(kp.nm, not kp.weighted, ys.n))
     probes = wx * n_masks
     rate = E.INDEX_PROBES_PER_S_WEIGHTED if kp.weighted else E.INDEX_PROBES_PER_S
     index_s = (probes * n_tiles / rate

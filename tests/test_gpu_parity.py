@@ -557,15 +557,17 @@ def test_index_count_widths(gpu, d):
         np.testing.assert_array_equal(g[upper], g_want[upper])
 
 
-def test_index_two_full_tiles(gpu):
-    """8-bit counts over 2 full SV tiles (40,000 SVs: 36,864 + 3,136) give the
+@pytest.mark.parametrize("S,tiles,tile", [(42000, 2, 36864), (40000, 1, 40192)])
+def test_index_two_full_tiles(gpu, S, tiles, tile):
+    """8-bit counts over 2 full SV tiles (42,000 SVs: 36,864 + 5,136) and over
+    the largest tile LDS holds (40,000 SVs in one tile of 40,192) give the
     brute-force integers -- random SVs, poly-A SVs against a poly-A query
     stretch (the 16-bit fallback, in two passes over the full tile), rows of
     several lengths."""
     from gkmsvm_amd import engine as E
 
     rng = np.random.default_rng(23)
-    S, L = 40000, 60
+    L = 60
     sv = rng.integers(0, 4, (S, L)).astype(np.uint8)
     sv[:50] = 0
     m = _model(2, {"L": 11, "k": 7, "d": 3}, sv, np.full(S, L), rng.normal(size=S))
@@ -577,7 +579,7 @@ def test_index_two_full_tiles(gpu):
     want = E.cross_hist(s.kp, xs, 0, xs.n, s.seqs, s.groups).cpu().numpy()
     assert want[0, :, 0].max() > 255
     idx = s.index()
-    assert idx.n_tiles == 2 and idx.pack == 4 and idx.tile == 36864
+    assert idx.n_tiles == tiles and idx.pack == 4 and idx.tile == tile
     got = E.index_hist(s.kp, idx, xs, 0, xs.n, S).cpu().numpy()
     np.testing.assert_array_equal(got, want)
 
