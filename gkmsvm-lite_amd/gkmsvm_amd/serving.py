@@ -175,12 +175,14 @@ class GkmServer:
     """Serve one model; see the module docstring."""
 
     def __init__(self, model, *, hbm_budget: int = 8 << 30, max_rows: int = 4096,
-                 streams: int = 2, max_explain: int = 64, tables: bool = True,
+                 streams: int | None = None, max_explain: int = 64, tables: bool = True,
                  history: int = 1 << 16, stage_len: int = 2048, freeze_gc: bool = True,
                  coalesce_s: float | None = None):
         """hbm_budget: device bytes a step may hold; max_rows / max_explain:
         scoring+variant rows and explain requests per step; streams: steps in
-        flight (slots with their own pinned buffers, on one HIP stream);
+        flight (slots with their own pinned buffers, on one HIP stream; by
+        default 2 on the per-SV kernels, 1 with the tables, whose steps are
+        host work -- a second one only adds thread contention);
         tables: build the exact l-mer tables at start where the model
         allows them; history: completed requests kept for latency_stats;
         stage_len: pinned host buffers are sized at start for steps of rows up
@@ -218,6 +220,8 @@ class GkmServer:
             else float(coalesce_s)
         self.max_explain = int(max_explain)
         stream = torch.cuda.Stream(device=self.kp.device)
+        if streams is None:
+            streams = 1 if self.table is not None else 2
         self._slots = [_Slot(stream) for _ in range(max(1, int(streams)))]
         self._free = list(self._slots)
         # pinned staging and result buffers, one set per slot: a slot's step
