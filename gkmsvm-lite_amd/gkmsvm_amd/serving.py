@@ -11,7 +11,8 @@ letter other than ACGTN are flagged to fail alone) and launches the index or pai
 kernel (or the l-mer table lookup) once for all of them; the step's explain
 requests follow as one batch per mode.  A harvester thread waits on each
 step's completion event and hands out its results.  Up to ``streams`` steps
-are in flight, each with its own pinned buffers, so a request that arrives
+(2 on the per-SV kernels, 1 with the tables by default) are in flight, each
+with its own pinned buffers, so a request that arrives
 while others are on the GPU is admitted as soon as a slot frees up: the host
 decodes, stages and launches its step while the one before runs.  The steps
 share one HIP stream and so run on the device in admission order: the GPU
@@ -23,9 +24,10 @@ ordering them with hipStreamWaitEvent blocked the launching thread until the
 event had completed.)  While a step is in flight the launcher lets arrivals
 join the next step for up to ``coalesce_s`` past the oldest one's (that step
 would wait behind the one in flight anyway), so a burst that arrives over a
-fraction of a millisecond becomes one step rather than several.  Admission is by memory: a step takes requests while the bytes it
-would hold (histograms, planes, outputs) stay within ``hbm_budget``; a
-request larger than the whole budget is rejected with MemoryError.
+fraction of a millisecond becomes one step rather than several.  Admission
+is by memory: a step takes requests while the bytes it would hold
+(histograms, planes, outputs) stay within ``hbm_budget``; a request larger
+than the whole budget is rejected with MemoryError.
 
 Each row's result depends only on that row (the per-row reduction runs in a
 fixed order), so a request scores identically alone and in any mix.  For
