@@ -269,6 +269,8 @@ HBM_KINDS = {
                           "RMW); the persistent launch",
     "k_index_explain": "16 B per (window, mask) probe + 12 B per hit (4 B id, 8 B FP64 RMW)",
     "k_lmer_explain": "13 B per (window, mask) probe: the mask, its level, V[v] (FP64)",
+    "k_lmer_explain_sorted": "13 B per (window, mask) probe: the mask, its level, V[v] (FP64); "
+                             "windows sorted by l-mer",
     "k_table_score": "4 B packed window + 8 B table entry per query window",
     "k_table_explain": "4 B window + 32 l B table row per window + 32 B output per position",
 }

@@ -2225,6 +2225,106 @@ __global__ __launch_bounds__(IXEXP_THREADS) void k_lmer_explain(IndexExplainArgs
     index_explain_block<LT, 4>(a);
 }
 
+// The l-mer weights form over the windows of all rows sorted by l-mer: xwin
+// holds n_win keys in ascending order (PAD last), xloc[w] = b * Wx + i the
+// row and window index of key w.  A block takes chunk_windows consecutive
+// keys, which share their high digits, so the V entries their neighbourhoods
+// reach lie in few lines that stay in the L2 between them (in position order a
+// window's 4,984 gathers are lines of its own); each window adds its 4 l
+// values to out with FP64 atomics.  Consecutive chunks go to one XCD.
+template <int LT>
+__global__ __launch_bounds__(IXEXP_THREADS) void k_lmer_explain_sorted(IndexExplainArgs a,
+                                                                      const int32_t *xloc,
+                                                                      int n_win)
+{
+    constexpr int LM = LT > 0 ? LT : 15;
+    __shared__ double stab[32];
+    const int l = LT > 0 ? LT : a.l;
+    const int nchunk = (n_win + a.chunk_windows - 1) / a.chunk_windows;
+    const int per = (nchunk + 7) / 8;
+    const int c = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (c >= nchunk)
+        return;
+    if (threadIdx.x < 16) {
+        const int t = threadIdx.x;
+        stab[t] = t < a.l + 2 ? a.ta[t] : 0.0;
+        stab[16 + t] = t < a.l + 2 ? a.tb[t] : 0.0;
+    }
+    __syncthreads();
+    const int w0 = c * a.chunk_windows, w1 = min(n_win, w0 + a.chunk_windows);
+    const uint32_t nl = 1u << (2 * l);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    for (int w = w0 + wave; w < w1; w += IXEXP_THREADS / WAVE) {
+        const uint32_t u = a.xwin[w];
+        if (u >= nl)
+            continue;  // PAD
+        const int loc = xloc[w];
+        const int b = loc / a.Wx, i = loc - b * a.Wx;
+        double am[LM], ad[LM][3];
+#pragma unroll
+        for (int k = 0; k < LM; k++) {
+            am[k] = 0.0;
+            ad[k][0] = ad[k][1] = ad[k][2] = 0.0;
+        }
+        for (int t0 = 0; t0 < a.n_masks; t0 += WAVE) {
+            const int t = t0 + lane;
+            if (t < a.n_masks) {
+                const uint32_t mask = a.masks[t];
+                const double sv = a.lw[u ^ mask];
+                if (sv != 0.0) {
+                    const int m = a.levels[t];
+                    const double sa = sv * stab[m], sb = sv * stab[16 + m];
+#pragma unroll
+                    for (int k = 0; k < LM; k++) {
+                        const uint32_t d = (mask >> (2 * k)) & 3u;
+                        am[k] += d == 0 ? sa : 0.0;
+                        ad[k][0] += d == 1 ? sb : 0.0;
+                        ad[k][1] += d == 2 ? sb : 0.0;
+                        ad[k][2] += d == 3 ? sb : 0.0;
+                    }
+                }
+            }
+        }
+        const int wx = a.xlen[b] - l + 1;
+        double aw = a.wtab ? (double)window_weight(a.wtab, wx, i) : 1.0;
+        if (a.normalize)
+            aw *= 1.0 / sqrt(fmax(a.dx[b], 1e-20));
+#pragma unroll
+        for (int k = 0; k < LM; k++) {
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                am[k] += __shfl_xor(am[k], off, WAVE);
+                ad[k][0] += __shfl_xor(ad[k][0], off, WAVE);
+                ad[k][1] += __shfl_xor(ad[k][1], off, WAVE);
+                ad[k][2] += __shfl_xor(ad[k][2], off, WAVE);
+            }
+        }
+        if (lane < 4 * l) {
+            const int k = lane >> 2, slot = lane & 3;
+            double val = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < LM; kk++)  // static indexes only (no scratch)
+                if (kk == k)
+                    val = slot == 0 ? am[kk] : slot == 1 ? ad[kk][0] : slot == 2 ? ad[kk][1]
+                                                                            : ad[kk][2];
+            val *= aw;
+            const uint32_t uk = (u >> (2 * k)) & 3u;
+            double *ob = a.out + (int64_t)b * 4 * a.lmax + i + k;
+            if (val != 0.0) {
+                if (slot > 0) {
+                    atomicAdd(&ob[(int64_t)(uk ^ (uint32_t)slot) * a.lmax], val);
+                } else if (!a.pert) {
+                    atomicAdd(&ob[(int64_t)uk * a.lmax], val);
+                } else {
+                    for (uint32_t h = 0; h < 4; h++)
+                        if (h != uk)
+                            atomicAdd(&ob[(int64_t)h * a.lmax], val);
+                }
+            }
+        }
+    }
+}
+
 // hist[b][s][m] (+)= sum over chunks of partial[chunk][b][s][m], over the
 // chunks that hold windows of row b (k_index_tile writes no others).
 template <typename T>
@@ -3708,6 +3808,53 @@ GKM_API int gkm_lmer_explain(int perturbation, const uint32_t *xwin, const int32
                       : l == 12 ? k_lmer_explain<12> : k_lmer_explain<0>;
     hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), lds, S_(stream), a);
     GKM_CHECK_LAUNCH("gkm_lmer_explain");
+    return 0;
+}
+
+GKM_API int gkm_lmer_explain_sorted(int perturbation, const uint32_t *xwin, const int32_t *xloc,
+                                    int n_win, const int32_t *xlen, int B, int Wx, int lmax,
+                                    const double *lw, const double *dx, int normalize,
+                                    const uint32_t *masks, const uint8_t *levels, int n_masks,
+                                    int l, const uint8_t *wtab, int chunk_windows,
+                                    const double *ta, const double *tb, double *out, void *stream)
+{
+    if (l < 1 || l > 13)
+        return fail(-1, "gkm_lmer_explain_sorted: the l-mer weights need 1 <= l <= 13, got %d", l);
+    if (B < 0 || B > 65535 || Wx < 0 || lmax < Wx + l - 1 || chunk_windows < 1 || n_win < 0 ||
+        (int64_t)n_win > (int64_t)B * Wx)
+        return fail(-1, "gkm_lmer_explain_sorted: need 0 <= B <= 65535, lmax >= Wx + l - 1, "
+                        "chunk_windows >= 1 and n_win <= B * Wx");
+    if (lw == nullptr || xloc == nullptr || (normalize && dx == nullptr))
+        return fail(-1, "gkm_lmer_explain_sorted: needs the l-mer weights, xloc (and dx when "
+                        "normalising)");
+    if (B == 0 || Wx == 0 || n_win == 0 || n_masks <= 0)
+        return 0;
+    IndexExplainArgs a = {};
+    a.xwin = xwin;
+    a.xlen = xlen;
+    a.masks = masks;
+    a.levels = levels;
+    a.wtab = wtab;
+    a.ta = ta;
+    a.tb = tb;
+    a.lw = lw;
+    a.dx = dx;
+    a.normalize = normalize != 0;
+    a.out = out;
+    a.B = B;
+    a.Wx = Wx;
+    a.lmax = lmax;
+    a.n_masks = n_masks;
+    a.l = l;
+    a.chunk_windows = chunk_windows;
+    a.pert = perturbation != 0;
+    const int64_t nchunk = ((int64_t)n_win + chunk_windows - 1) / chunk_windows;
+    dim3 grid((unsigned)(8 * ((nchunk + 7) / 8)));
+    typedef void (*sorted_fn)(IndexExplainArgs, const int32_t *, int);
+    const sorted_fn fn = l == 10 ? k_lmer_explain_sorted<10> : l == 11 ? k_lmer_explain_sorted<11>
+                       : l == 12 ? k_lmer_explain_sorted<12> : k_lmer_explain_sorted<0>;
+    hipLaunchKernelGGL(fn, grid, dim3(IXEXP_THREADS), 0, S_(stream), a, xloc, n_win);
+    GKM_CHECK_LAUNCH("gkm_lmer_explain_sorted");
     return 0;
 }
 

@@ -39,6 +39,7 @@ SIGNATURES = {
     "gkm_index_explain": [I, P, P, I, I, I, P, P, P, P, P, P, I, I, P, I, I, I, I, P, P, P, P,
                           P],
     "gkm_lmer_explain": [I, P, P, I, I, I, P, P, I, P, P, I, I, P, I, P, P, P, P],
+    "gkm_lmer_explain_sorted": [I, P, P, I, P, I, I, I, P, P, I, P, P, I, I, P, I, P, P, P, P],
     "gkm_index_hist": [P, P, P, I, I, P, P, P, P, I, P, P, I, I, I, P, I, I, I, I, I, I64, I64,
                        P, P, P],
     "gkm_index_reduce": [P, I, P, I, I, I, I, I, I, I, P, P],

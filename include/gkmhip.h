@@ -217,6 +217,21 @@ int gkm_lmer_explain(int perturbation, const uint32_t *xwin, const int32_t *xlen
                      const uint8_t *wtab, int chunk_windows, const double *ta, const double *tb,
                      double *out, void *stream);
 
+/* gkm_lmer_explain over the windows of every row sorted by l-mer: xwin
+ * [n_win] ascending (PAD, 0xFFFFFFFF, last), xloc [n_win] each key's
+ * b * Wx + i in the rows' [B][Wx] windows, xlen / dx / out as there.  A block
+ * takes chunk_windows consecutive keys -- which share their high digits, so
+ * the V entries of their neighbourhoods share cache lines -- and each window
+ * adds its attributions to out [B][4][lmax] with FP64 atomics.  The same
+ * values as gkm_lmer_explain up to the order of FP64 additions.
+ * explain/lmer.py:31-188 (_explain_lmer_cpu / _explain_lmer_cuda with V). */
+int gkm_lmer_explain_sorted(int perturbation, const uint32_t *xwin, const int32_t *xloc,
+                            int n_win, const int32_t *xlen, int B, int Wx, int lmax,
+                            const double *lw, const double *dx, int normalize,
+                            const uint32_t *masks, const uint8_t *levels, int n_masks, int l,
+                            const uint8_t *wtab, int chunk_windows, const double *ta,
+                            const double *tb, double *out, void *stream);
+
 /* The same histograms through an l-mer index of the SV windows, ADDED into
  * hist [B][S][nm]: xwin [B][Wx] packed query windows (gkm_pack_windows), in any
  * order within a row (PAD last), with xpos [B][Wx] their positions in the row
