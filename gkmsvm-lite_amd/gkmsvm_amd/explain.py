@@ -151,7 +151,6 @@ def _explain_rows(model, codes, lens, resolved, out):
     dx = E.self_kraw(kp, xs) if kp.normalize else torch.ones(B, dtype=torch.float64,
                                                             device=kp.device)
     ta_d, tb_d = _explain_ab(sv, ta, tb, pert)
-    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
     kernel = _explain_kernel(model, sv, kp, xs, ta, tb, cmax)
     if kernel == "lmer":
         if "V" not in sv.tables:
@@ -168,6 +167,11 @@ def _explain_rows(model, codes, lens, resolved, out):
         flat = fwd.reshape(-1)
         keys, order = torch.sort(torch.where(flat < 0, torch.full_like(flat, 2**31 - 1), flat))
         xloc = order.to(torch.int32)
+        # the kernel adds into out itself where it is the zero-filled [B, 4, lmax]
+        # block gkmexplain passes
+        direct = out.is_contiguous() and tuple(out.shape) == (B, 4, xs.lmax)
+        buf = out if direct else torch.zeros((B, 4, xs.lmax), dtype=torch.float64,
+                                             device=kp.device)
         probes = float(np.clip(xs.lengths - kp.l + 1, 0, None).sum()) * len(masks)
         E._timed("k_lmer_explain_sorted", lambda: probes * LMER_PROBE_BYTES, None,
                  lambda: _lib.call(
@@ -176,8 +180,10 @@ def _explain_rows(model, codes, lens, resolved, out):
                      E._p(dx) if kp.normalize else None, int(kp.normalize), E._p(masks),
                      E._p(levels), len(masks), kp.l, E._p(wq), LMER_SORTED_CHUNK, E._p(ta_d),
                      E._p(tb_d), E._p(buf), st))
-        out += buf
+        if not direct:
+            out += buf
         return
+    buf = torch.zeros((B, 4, xs.lmax), dtype=torch.float64, device=kp.device)
     hist = None
     if kp.rbf:
         hist = E.sv_hist(sv, xs, 0, B, use_index=model.use_lmer_index)
